@@ -1,0 +1,162 @@
+// nova-mi355x: belief-set expansion on the host (random trajectories, distinct beliefs).
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <vector>
+
+#include <nova/pomdp/utilities/pomdp_expand_cpu.h>
+#include <nova/pomdp/utilities/pomdp_model_cpu.h>
+
+#include "nova_common.h"
+
+namespace nova_amd {
+
+// Dense copy of sparse belief i (src/pomdp/utilities/pomdp_expand_cpu.cpp:39-54).
+static void dense_belief(const nova::POMDP *p, unsigned i, float *b)
+{
+    std::memset(b, 0, sizeof(float) * p->n);
+    const int *Zi = p->Z + (size_t)i * p->rz;
+    for (unsigned k = 0; k < p->rz && Zi[k] >= 0; k++)
+        b[Zi[k]] = p->B[(size_t)i * p->rz + k];
+}
+
+// Pr(o | b, a), summed per state then weighted (pomdp_expand_cpu.cpp:57-79).
+static float prob_obs(const nova::POMDP *p, const float *b, unsigned a, unsigned o)
+{
+    const float *Oa = p->O + (size_t)a * p->n * p->z;
+    float pr = 0.0f;
+    for (unsigned s = 0; s < p->n; s++) {
+        const size_t row = idx3(s, a, 0, p->m, p->ns);
+        float val = 0.0f;
+        for (unsigned l = 0; l < p->ns; l++) {
+            const int sp = p->S[row + l];
+            if (sp < 0)
+                break;
+            val += p->T[row + l] * Oa[(size_t)sp * p->z + o];
+        }
+        pr += val * b[s];
+    }
+    return pr;
+}
+
+// (unsigned)(rand()/RAND_MAX * k) as the reference draws indices, kept below k: the float
+// quotient reaches 1.0 for the top 64 values of rand().
+static unsigned draw_index(unsigned k)
+{
+    unsigned v = (unsigned)((float)std::rand() / (float)RAND_MAX * (float)k);
+    return (v >= k && k > 0) ? k - 1 : v;
+}
+
+static bool expand_args_ok(const nova::POMDP *p)
+{
+    return p != nullptr && p->n != 0 && p->ns != 0 && p->m != 0 && p->z != 0 && p->r != 0 &&
+           p->rz != 0 && p->S != nullptr && p->T != nullptr && p->O != nullptr &&
+           p->R != nullptr && p->Z != nullptr && p->B != nullptr;
+}
+
+} // namespace nova_amd
+
+namespace nova {
+
+// Random trajectories (src/pomdp/utilities/pomdp_expand_cpu.cpp:82-163): start from a random
+// existing belief, walk a random number h <= horizon of steps with a random action and an
+// observation drawn by cumulative Pr(o|b,a) (first o whose running sum reaches the target and
+// whose own probability is positive, else o = 0), append every visited belief.
+int pomdp_expand_random_cpu(POMDP *pomdp, unsigned int numBeliefsToAdd)
+{
+    if (!nova_amd::expand_args_ok(pomdp) || pomdp->horizon < 1 || numBeliefsToAdd == 0) {
+        NOVA_ERR("pomdp_expand_random_cpu", "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    std::srand((unsigned)std::time(nullptr));
+    const unsigned n = pomdp->n;
+    std::vector<float> b(n), Bnew((size_t)numBeliefsToAdd * n, 0.0f);
+    unsigned added = 0;
+    while (added < numBeliefsToAdd) {
+        nova_amd::dense_belief(pomdp, nova_amd::draw_index(pomdp->r), b.data());
+        const unsigned h = (unsigned)((float)std::rand() / (float)RAND_MAX *
+                                      (float)(pomdp->horizon + 1));
+        for (unsigned t = 0; t < h; t++) {
+            const unsigned a = nova_amd::draw_index(pomdp->m);
+            const float target = (float)std::rand() / (float)RAND_MAX;
+            float cum = 0.0f;
+            unsigned o = 0;
+            for (unsigned op = 0; op < pomdp->z; op++) {
+                const float pr = nova_amd::prob_obs(pomdp, b.data(), a, op);
+                cum += pr;
+                if (cum >= target && pr > 0.0f) {
+                    o = op;
+                    break;
+                }
+            }
+            float *bp = nullptr;
+            pomdp_belief_update_cpu(pomdp, b.data(), a, o, bp);
+            std::memcpy(b.data(), bp, sizeof(float) * n);
+            delete[] bp;
+            std::memcpy(&Bnew[(size_t)added * n], b.data(), sizeof(float) * n);
+            if (++added >= numBeliefsToAdd)
+                break;
+        }
+    }
+    return pomdp_add_new_raw_beliefs_cpu(pomdp, numBeliefsToAdd, Bnew.data());
+}
+
+// Distinct beliefs (src/pomdp/utilities/pomdp_expand_cpu.cpp:166-266): for each belief, the
+// successor tau(b,a,o) over (a, o) with Pr(o|b,a) != 0 maximising min_j ||b_j - tau||_1
+// (dense sum in state order; strict '<' from 1e10 for the min, strict '>' from -1e10 for the
+// max). When no (a, o) qualifies the reference appends an uninitialised buffer; this appends
+// zeros (an empty support).
+int pomdp_expand_distinct_beliefs_cpu(POMDP *pomdp)
+{
+    if (!nova_amd::expand_args_ok(pomdp)) {
+        NOVA_ERR("pomdp_expand_distinct_beliefs_cpu", "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const unsigned n = pomdp->n, r = pomdp->r;
+    std::vector<float> Bnew((size_t)r * n, 0.0f), b(n), bt(n);
+    for (unsigned i = 0; i < r; i++) {
+        nova_amd::dense_belief(pomdp, i, b.data());
+        float starValue = nova_amd::kLow;
+        float *out = &Bnew[(size_t)i * n];
+        for (unsigned a = 0; a < pomdp->m; a++) {
+            for (unsigned o = 0; o < pomdp->z; o++) {
+                if (nova_amd::prob_obs(pomdp, b.data(), a, o) == 0.0f)
+                    continue;
+                float *bp = nullptr;
+                const int rc = pomdp_belief_update_cpu(pomdp, b.data(), a, o, bp);
+                if (rc != NOVA_SUCCESS) {
+                    delete[] bp;
+                    continue;
+                }
+                float dmin = nova_amd::kHigh;
+                for (unsigned j = 0; j < r; j++) {
+                    nova_amd::dense_belief(pomdp, j, bt.data());
+                    float d = 0.0f;
+                    for (unsigned s = 0; s < n; s++)
+                        d += std::fabs(bt[s] - bp[s]);
+                    if (d < dmin)
+                        dmin = d;
+                }
+                if (dmin > starValue) {
+                    std::memcpy(out, bp, sizeof(float) * n);
+                    starValue = dmin;
+                }
+                delete[] bp;
+            }
+        }
+    }
+    return pomdp_add_new_raw_beliefs_cpu(pomdp, r, Bnew.data());
+}
+
+// PEMA is outside this library's scope: the reference's alpha argmax inside it is defective
+// (pomdp_expand_cpu.cpp:391-402 assigns instead of accumulating and compares the wrong way).
+int pomdp_expand_pema_cpu(POMDP *pomdp, const POMDPAlphaVectors *policy)
+{
+    (void)pomdp;
+    (void)policy;
+    NOVA_ERR("pomdp_expand_pema_cpu", "PEMA expansion is not provided by this library.");
+    return NOVA_ERROR_INVALID_DATA;
+}
+
+} // namespace nova

@@ -1,0 +1,35 @@
+// nova-mi355x: HIP runtime helpers shared by the device-side translation units.
+#ifndef NOVA_AMD_HIP_H
+#define NOVA_AMD_HIP_H
+
+#include <hip/hip_runtime.h>
+
+#include "nova_common.h"
+
+namespace nova_amd {
+
+// The library's stream on the current device: created on first use with the default
+// priority and as a blocking stream, so it orders after work on the legacy null stream (the
+// stream PyTorch and plain HIP callers use by default). Every ABI call is synchronous at
+// return: it synchronises this stream before returning.
+hipStream_t stream();
+
+// Reports a HIP failure as "Error[fn]: msg (hip error text)" and returns code, or returns
+// NOVA_SUCCESS when e == hipSuccess.
+int hip_fail(hipError_t e, const char *fn, const char *msg, int code);
+
+// hipMalloc + registry tracking; device pointers allocated here can be released again by a
+// repeated initialize (see registry_track).
+int device_alloc(void **p, size_t bytes, const char *fn, const char *what);
+// hipFree of a pointer when this library allocated it (or unconditionally when force).
+int device_free(void *p, const char *fn, const char *what);
+int device_free_if_owned(void *p, const char *fn, const char *what);
+
+// Checks that every successor index in the valid prefix of each list lies in [0, limit):
+// the kernels gather V[sp]/Gamma[sp] without further checks. Returns the first bad flat index
+// or -1.
+long long first_bad_successor(const int *S, size_t rows, unsigned len, unsigned limit);
+
+} // namespace nova_amd
+
+#endif

@@ -1,0 +1,335 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to the oracle and to nova's own CPU build.
+
+  Oracle  - oracle/liboracle.so, the C restatement (oracle.c / oracle_impl.h); every method
+            takes numpy arrays and a dtype (float32 = the faithful twin, float64 = the FP64 twin).
+  RefLib  - oracle/_ref/libnova_ref.so, nova's CPU sources compiled where they lie (Makefile
+            target `ref`), driven through nova's own C ABI.
+Only tests/, the smoke check and the benchmark's CPU-baseline leg use this module.
+"""
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libnova_ref.so")
+
+_f32p = ct.POINTER(ct.c_float)
+_f64p = ct.POINTER(ct.c_double)
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+
+
+def _p(a):
+    return ct.c_void_p(a.ctypes.data) if a is not None else ct.c_void_p(0)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        self.lib = ct.CDLL(ORACLE_SO)
+        for sfx in ("f32", "f64"):
+            R = ct.c_float if sfx == "f32" else ct.c_double
+            getattr(self.lib, "orc_prob_obs_" + sfx).restype = R
+            getattr(self.lib, "orc_sparse_vb_" + sfx).restype = R
+        self.lib.orc_vi_solve_f64.restype = ct.c_uint
+        self.lib.orc_adjoint_f64.restype = ct.c_uint
+
+    @staticmethod
+    def _sfx(dtype):
+        return "f32" if np.dtype(dtype) == np.float32 else "f64"
+
+    # ---- value iteration (src/mdp/algorithms/mdp_vi_cpu.cpp:36-223)
+    def vi_sweep(self, S, T, R, gamma, V, dtype=np.float32):
+        n, m, ns = S.shape
+        S, T, R = _c(S, np.int32), _c(T, np.float32), _c(R, np.float32)
+        V = _c(V, dtype)
+        Vp = np.empty(n, dtype)
+        pi = np.zeros(n, np.uint32)
+        getattr(self.lib, "orc_vi_sweep_" + self._sfx(dtype))(
+            n, ns, m, ct.c_float(gamma), _p(S), _p(T), _p(R), _p(V), _p(Vp), _p(pi))
+        return Vp, pi
+
+    def vi_run(self, S, T, R, gamma, horizon, V0=None, dtype=np.float32):
+        n, m, ns = S.shape
+        S, T, R = _c(S, np.int32), _c(T, np.float32), _c(R, np.float32)
+        V0 = None if V0 is None else _c(V0, np.float32)
+        work = np.empty(2 * n, dtype)
+        V = np.empty(n, dtype)
+        pi = np.zeros(n, np.uint32)
+        getattr(self.lib, "orc_vi_run_" + self._sfx(dtype))(
+            n, ns, m, ct.c_float(gamma), horizon, _p(S), _p(T), _p(R), _p(V0), _p(work),
+            _p(V), _p(pi))
+        return V, pi
+
+    # ---- PBVI (src/pomdp/algorithms/pomdp_pbvi_cpu.cpp:37-300)
+    def pbvi_backup(self, P, Gamma, i0=0, i1=None, dtype=np.float32, diag=True):
+        """Backs up beliefs [i0, i1) against Gamma (r0 x n). Returns dict with GammaPrime rows
+        (i1-i0 x n), pi, and when diag: jstar[i,a,o], gap[i,a,o], aval[i,a]."""
+        S, T, O, R = (_c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32),
+                      _c(P["R"], np.float32))
+        Z, B = _c(P["Z"], np.int32), _c(P["B"], np.float32)
+        n, m, ns = S.shape
+        z = O.shape[2]
+        r, rz = Z.shape
+        i1 = r if i1 is None else i1
+        G = _c(Gamma, dtype)
+        r0 = G.shape[0]
+        Gp = np.zeros((r, n), dtype)
+        pi = np.zeros(r, np.uint32)
+        alpha = np.empty(n, dtype)
+        k = i1 - i0
+        jstar = np.empty((k, m, z), np.int32) if diag else None
+        gap = np.empty((k, m, z), dtype) if diag else None
+        aval = np.empty((k, m), dtype) if diag else None
+        getattr(self.lib, "orc_pbvi_backup_" + self._sfx(dtype))(
+            n, ns, m, z, rz, ct.c_float(P["gamma"]), _p(S), _p(T), _p(O), _p(R), _p(Z), _p(B),
+            _p(G), r0, i0, i1, _p(Gp), _p(pi), _p(alpha), _p(jstar), _p(gap), _p(aval))
+        out = dict(GammaPrime=Gp[i0:i1], pi=pi[i0:i1])
+        if diag:
+            out.update(jstar=jstar, gap=gap, aval=aval)
+        return out
+
+    def pbvi_run(self, P, horizon=None, Gamma0=None, dtype=np.float32):
+        S, T, O, R = (_c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32),
+                      _c(P["R"], np.float32))
+        Z, B = _c(P["Z"], np.int32), _c(P["B"], np.float32)
+        n, m, ns = S.shape
+        z = O.shape[2]
+        r, rz = Z.shape
+        horizon = P["horizon"] if horizon is None else horizon
+        G0 = _c(P["Gamma0"] if Gamma0 is None else Gamma0, np.float32)
+        work = np.empty(2 * r * n + n, dtype)
+        G = np.empty((r, n), dtype)
+        pi = np.zeros(r, np.uint32)
+        getattr(self.lib, "orc_pbvi_run_" + self._sfx(dtype))(
+            n, ns, m, z, r, rz, ct.c_float(P["gamma"]), horizon, _p(S), _p(T), _p(O), _p(R),
+            _p(Z), _p(B), _p(G0), _p(work), _p(G), _p(pi))
+        return G, pi
+
+    # ---- beliefs (src/pomdp/utilities/pomdp_model_cpu.cpp:89-137, pomdp_expand_cpu.cpp:39-266)
+    def belief_update(self, P, b, a, o, dtype=np.float32):
+        S, T, O = _c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32)
+        n, m, ns = S.shape
+        z = O.shape[2]
+        b = _c(b, dtype)
+        bp = np.empty(n, dtype)
+        rc = getattr(self.lib, "orc_belief_update_" + self._sfx(dtype))(
+            n, ns, m, z, _p(S), _p(T), _p(O), _p(b), a, o, _p(bp))
+        return rc, bp
+
+    def prob_obs(self, P, b, a, o, dtype=np.float32):
+        S, T, O = _c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32)
+        n, m, ns = S.shape
+        z = O.shape[2]
+        b = _c(b, dtype)
+        return getattr(self.lib, "orc_prob_obs_" + self._sfx(dtype))(
+            n, ns, m, z, _p(S), _p(T), _p(O), _p(b), a, o)
+
+    def expand_distinct(self, P, dtype=np.float32):
+        S, T, O = _c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32)
+        Z, B = _c(P["Z"], np.int32), _c(P["B"], np.float32)
+        n, m, ns = S.shape
+        z = O.shape[2]
+        r, rz = Z.shape
+        work = np.empty(3 * n, dtype)
+        Bnew = np.empty((r, n), dtype)
+        ba = np.empty(r, np.int32)
+        bo = np.empty(r, np.int32)
+        getattr(self.lib, "orc_expand_distinct_" + self._sfx(dtype))(
+            n, ns, m, z, r, rz, _p(S), _p(T), _p(O), _p(Z), _p(B), _p(work), _p(Bnew), _p(ba),
+            _p(bo))
+        return Bnew, ba, bo
+
+    def alpha_value(self, Gamma, pi, b, dtype=np.float32):
+        G = _c(Gamma, dtype)
+        r, n = G.shape
+        pi = _c(pi, np.uint32)
+        b = _c(b, dtype)
+        Vb = (ct.c_float if dtype == np.float32 else ct.c_double)()
+        a = ct.c_uint()
+        arg = getattr(self.lib, "orc_alpha_value_" + self._sfx(dtype))(
+            n, r, _p(G), _p(pi), _p(b), ct.byref(Vb), ct.byref(a))
+        return Vb.value, a.value, arg
+
+    # ---- FP64 references for the differentiable solve
+    def vi_solve_f64(self, S, T, R, gamma, V0=None, max_iter=100000, tol=1e-12):
+        n, m, ns = S.shape
+        S, T, R = _c(S, np.int32), _c(T, np.float32), _c(R, np.float32)
+        V = np.zeros(n) if V0 is None else _c(V0, np.float64).copy()
+        work = np.empty(n)
+        pi = np.zeros(n, np.uint32)
+        res = ct.c_double()
+        it = self.lib.orc_vi_solve_f64(n, ns, m, ct.c_float(gamma), _p(S), _p(T), _p(R), _p(V),
+                                       _p(work), _p(pi), max_iter, ct.c_double(tol),
+                                       ct.byref(res))
+        return V, pi, it, res.value
+
+    def adjoint_f64(self, S, T, gamma, pi, rho, max_iter=100000, tol=1e-14):
+        n, m, ns = S.shape
+        S, T = _c(S, np.int32), _c(T, np.float32)
+        pi = _c(pi, np.uint32)
+        rho = _c(rho, np.float64)
+        lam = np.empty(n)
+        work = np.empty(n)
+        self.lib.orc_adjoint_f64(n, ns, m, ct.c_float(gamma), _p(S), _p(T), _p(pi), _p(rho),
+                                 _p(lam), _p(work), max_iter, ct.c_double(tol))
+        return lam
+
+    def vi_grads_f64(self, S, gamma, pi, lam, V):
+        n, m, ns = S.shape
+        S = _c(S, np.int32)
+        dR = np.empty((n, m))
+        dT = np.empty((n, m, ns))
+        self.lib.orc_vi_grads_f64(n, ns, m, ct.c_float(gamma), _p(S), _p(_c(pi, np.uint32)),
+                                  _p(_c(lam, np.float64)), _p(_c(V, np.float64)), _p(dR), _p(dT))
+        return dR, dT
+
+
+class RefLib:
+    """nova's own CPU path (oracle/_ref/libnova_ref.so) driven through its C ABI."""
+
+    def __init__(self):
+        if not os.path.exists(REF_SO):
+            raise FileNotFoundError(REF_SO)
+        from nova_amd import _abi  # ABI structure layouts (interface facts)
+        self._abi = _abi
+        self.lib = ct.CDLL(REF_SO)
+        for name, argtypes in _abi.ABI_SIGNATURES.items():
+            fn = getattr(self.lib, name, None)
+            if fn is not None:
+                fn.argtypes = argtypes
+                fn.restype = ct.c_int
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_SO)
+
+    def _mdp(self, S, T, R, gamma, horizon):
+        a = self._abi
+        n, m, ns = S.shape
+        mdp = a.NovaMDP()
+        keep = [_c(S, np.int32), _c(T, np.float32), _c(R, np.float32)]
+        mdp.n, mdp.ns, mdp.m = n, ns, m
+        mdp.gamma, mdp.horizon, mdp.epsilon = float(gamma), int(horizon), 0.0
+        mdp.S = ct.cast(keep[0].ctypes.data, a.c_int_p)
+        mdp.T = ct.cast(keep[1].ctypes.data, a.c_float_p)
+        mdp.R = ct.cast(keep[2].ctypes.data, a.c_float_p)
+        return mdp, keep
+
+    def vi_execute(self, S, T, R, gamma, horizon, V0=None):
+        a = self._abi
+        mdp, keep = self._mdp(S, T, R, gamma, horizon)
+        vi = a.NovaMDPVICPU()
+        if V0 is not None:
+            v0 = _c(V0, np.float32)
+            keep.append(v0)
+            vi.VInitial = ct.cast(v0.ctypes.data, a.c_float_p)
+        pol = a.NovaMDPValueFunction()
+        rc = self.lib.mdp_vi_execute_cpu(ct.byref(mdp), ct.byref(vi), ct.byref(pol))
+        assert rc == 0, rc
+        n = S.shape[0]
+        V = np.ctypeslib.as_array(pol.V, (n,)).copy()
+        pi = np.ctypeslib.as_array(pol.pi, (n,)).copy()
+        self.lib.mdp_value_function_uninitialize(ct.byref(pol))
+        return V, pi
+
+    def vi_time_sweeps(self, S, T, R, gamma, sweeps, V0=None):
+        """Seconds for `sweeps` calls of nova's mdp_vi_update_cpu (after its initialize)."""
+        import time
+        a = self._abi
+        mdp, keep = self._mdp(S, T, R, gamma, sweeps)
+        vi = a.NovaMDPVICPU()
+        if V0 is not None:
+            v0 = _c(V0, np.float32)
+            keep.append(v0)
+            vi.VInitial = ct.cast(v0.ctypes.data, a.c_float_p)
+        assert self.lib.mdp_vi_initialize_cpu(ct.byref(mdp), ct.byref(vi)) == 0
+        t = time.perf_counter()
+        for _ in range(sweeps):
+            self.lib.mdp_vi_update_cpu(ct.byref(mdp), ct.byref(vi))
+        dt = time.perf_counter() - t
+        self.lib.mdp_vi_uninitialize_cpu(ct.byref(mdp), ct.byref(vi))
+        return dt
+
+    def _pomdp(self, P):
+        a = self._abi
+        S = P["S"]
+        n, m, ns = S.shape
+        z = P["O"].shape[2]
+        r, rz = P["Z"].shape
+        keep = [_c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32),
+                _c(P["R"], np.float32), _c(P["Z"], np.int32), _c(P["B"], np.float32)]
+        p = a.NovaPOMDP()
+        p.n, p.ns, p.m, p.z, p.r, p.rz = n, ns, m, z, r, rz
+        p.gamma, p.horizon = float(P["gamma"]), int(P["horizon"])
+        p.S = ct.cast(keep[0].ctypes.data, a.c_int_p)
+        p.T = ct.cast(keep[1].ctypes.data, a.c_float_p)
+        p.O = ct.cast(keep[2].ctypes.data, a.c_float_p)
+        p.R = ct.cast(keep[3].ctypes.data, a.c_float_p)
+        p.Z = ct.cast(keep[4].ctypes.data, a.c_int_p)
+        p.B = ct.cast(keep[5].ctypes.data, a.c_float_p)
+        return p, keep
+
+    def pbvi_execute(self, P, horizon=None, Gamma0=None):
+        a = self._abi
+        P = dict(P)
+        if horizon is not None:
+            P["horizon"] = horizon
+        p, keep = self._pomdp(P)
+        g0 = _c(P["Gamma0"] if Gamma0 is None else Gamma0, np.float32)
+        keep.append(g0)
+        pb = a.NovaPOMDPPBVICPU()
+        pb.GammaInitial = ct.cast(g0.ctypes.data, a.c_float_p)
+        pol = a.NovaPOMDPAlphaVectors()
+        rc = self.lib.pomdp_pbvi_execute_cpu(ct.byref(p), ct.byref(pb), ct.byref(pol))
+        assert rc == 0, rc
+        r, n = p.r, p.n
+        G = np.ctypeslib.as_array(pol.Gamma, (r * n,)).reshape(r, n).copy()
+        pi = np.ctypeslib.as_array(pol.pi, (r,)).copy()
+        self.lib.pomdp_alpha_vectors_uninitialize(ct.byref(pol))
+        return G, pi
+
+    def belief_update(self, P, b, a_, o):
+        a = self._abi
+        p, keep = self._pomdp(P)
+        b = _c(b, np.float32)
+        bp = a.c_float_p()
+        rc = self.lib.pomdp_belief_update_cpu(ct.byref(p), b.ctypes.data_as(a.c_float_p), a_, o,
+                                              ct.byref(bp))
+        out = np.ctypeslib.as_array(bp, (p.n,)).copy() if bp else None
+        return rc, out
+
+    def expand_distinct(self, P):
+        """Runs nova's pomdp_expand_distinct_beliefs_cpu on a library-owned copy of P; returns
+        the new (Z, B) arrays."""
+        a = self._abi
+        p = a.NovaPOMDP()
+        S = P["S"]
+        n, m, ns = S.shape
+        z = P["O"].shape[2]
+        r, rz = P["Z"].shape
+        rc = self.lib.pomdp_initialize_cpu(ct.byref(p), n, ns, m, z, r, rz,
+                                           ct.c_float(P["gamma"]), max(1, int(P["horizon"])))
+        assert rc == 0
+        for name, arr, dt in (("S", P["S"], np.int32), ("T", P["T"], np.float32),
+                              ("O", P["O"], np.float32), ("R", P["R"], np.float32),
+                              ("Z", P["Z"], np.int32), ("B", P["B"], np.float32)):
+            src = _c(arr, dt)
+            ct.memmove(getattr(p, name), src.ctypes.data, src.nbytes)
+        rc = self.lib.pomdp_expand_distinct_beliefs_cpu(ct.byref(p))
+        assert rc == 0
+        r2, rz2 = p.r, p.rz
+        Z2 = np.ctypeslib.as_array(p.Z, (r2 * rz2,)).reshape(r2, rz2).copy()
+        B2 = np.ctypeslib.as_array(p.B, (r2 * rz2,)).reshape(r2, rz2).copy()
+        self.lib.pomdp_uninitialize_cpu(ct.byref(p))
+        return Z2, B2

@@ -80,7 +80,15 @@ def bench_vi(args):
 
     p = synth.WORKLOADS["vi_n1e6"]
     w = synth.workload("vi_n1e6")
-    n, m, ns, sweeps = p["n"], p["m"], p["ns"], p["horizon"]
+    n, m, ns = p["n"], p["m"], p["ns"]
+    sweeps = args.sweeps or p["horizon"]
+    if args.calibrate_copy_mb:
+        # a plain device copy of known size, for scaling HBM byte counters
+        a = torch.ones(args.calibrate_copy_mb << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     mdp = MDP.from_arrays(w["S"], w["T"], w["R"], gamma=w["gamma"], horizon=sweeps)
     mdp.initialize_gpu()
     vi = MDPValueIterationGPU(mdp, Vinitial=np.zeros(n, np.float32))
@@ -137,6 +145,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sweeps", type=int, default=10)
+    ap.add_argument("--sweeps", type=int, default=0,
+                    help="sweeps per step (default: the workload's 200; fewer only for profiling)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="run one plain device copy of this many MB first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path is single-GPU; see the "

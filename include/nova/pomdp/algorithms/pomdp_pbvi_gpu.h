@@ -2,8 +2,10 @@
 // (replaces include/nova/pomdp/algorithms/pomdp_pbvi_gpu.h:44-100).
 // x86-64 sizeof 48: GammaInitial@0 numThreads@8 currentHorizon@12 d_Gamma@16
 // d_GammaPrime@24 d_pi@32 d_alphaBA@40. The r*m*n alphaBA scratch of the reference is not
-// needed by this implementation: d_alphaBA holds the (small) per-iteration argmax record
-// (one int per belief, action and observation) instead.
+// needed by this implementation; d_alphaBA instead points at the library's device workspace:
+// the model products O*T (m*z*n*ns floats), a state-major copy of Gamma (n*r floats) and the
+// per-iteration argmax record (per belief, action and observation). It is allocated by
+// pomdp_pbvi_initialize_gpu and released by pomdp_pbvi_uninitialize_gpu.
 #ifndef NOVA_POMDP_PBVI_GPU_H
 #define NOVA_POMDP_PBVI_GPU_H
 

@@ -189,6 +189,11 @@ ABI_SIGNATURES = {
 EXTRA_SIGNATURES = {
     "mdp_vi_run_gpu": [_MDP, _VIG, ct.c_void_p, U, _P(ct.c_float)],
     "nova_amd_build_info": [ct.c_char_p, U],
+    "nova_amd_pbvi_mode": [ct.c_int],
+    "pomdp_pbvi_run_gpu": [_POMDP, _PBG, U, _P(ct.c_float)],
+    "pomdp_pbvi_argmax_gpu": [_POMDP, _PBG, c_int_p, c_uint_p],
+    "pomdp_expand_random_seeded_gpu": [_POMDP, U, U, _P(ct.c_uint), _P(c_float_p)],
+    "nova_amd_free_floats": [c_float_p],
 }
 
 _lib = None

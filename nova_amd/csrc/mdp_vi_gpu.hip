@@ -64,7 +64,8 @@ template <int E, int G>
 __global__ __launch_bounds__(kBlock) void vi_sweep_tiled(
         unsigned n, unsigned m, unsigned ns, unsigned statesPerTile, float gamma,
         const int *__restrict__ S, const float *__restrict__ T, const float *__restrict__ R,
-        const float *__restrict__ V, float *__restrict__ Vp, unsigned *__restrict__ pi)
+        const float *__restrict__ V, float *__restrict__ Vp, unsigned *__restrict__ pi,
+        int writePi)
 {
     __shared__ float q[kTileRows];
 
@@ -144,7 +145,8 @@ __global__ __launch_bounds__(kBlock) void vi_sweep_tiled(
             }
         }
         Vp[s0 + t] = best;
-        pi[s0 + t] = bestA;
+        if (writePi)
+            pi[s0 + t] = bestA;
     }
 }
 
@@ -152,7 +154,8 @@ __global__ __launch_bounds__(kBlock) void vi_sweep_tiled(
 __global__ __launch_bounds__(kBlock) void vi_sweep_plain(
         unsigned n, unsigned m, unsigned ns, float gamma,
         const int *__restrict__ S, const float *__restrict__ T, const float *__restrict__ R,
-        const float *__restrict__ V, float *__restrict__ Vp, unsigned *__restrict__ pi)
+        const float *__restrict__ V, float *__restrict__ Vp, unsigned *__restrict__ pi,
+        int writePi)
 {
     const unsigned s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n)
@@ -174,11 +177,12 @@ __global__ __launch_bounds__(kBlock) void vi_sweep_plain(
         }
     }
     Vp[s] = best;
-    pi[s] = bestA;
+    if (writePi)
+        pi[s] = bestA;
 }
 
 typedef void (*TiledFn)(unsigned, unsigned, unsigned, unsigned, float, const int *,
-                        const float *, const float *, const float *, float *, unsigned *);
+                        const float *, const float *, const float *, float *, unsigned *, int);
 
 template <int E> TiledFn tiled_for(unsigned G)
 {
@@ -232,15 +236,20 @@ static ViPlan vi_plan(const nova::MDP *mdp)
     return p;
 }
 
+// writePi: the policy buffer is shared by all sweeps and only the last sweep's greedy actions
+// are ever read (src/mdp/algorithms/mdp_vi_cpu.cpp:187-191, :215-220), so a run of sweeps
+// writes pi on its final sweep only.
 static void vi_launch(const ViPlan &p, const nova::MDP *mdp, const float *V, float *Vp,
-                      unsigned *pi, hipStream_t st)
+                      unsigned *pi, bool writePi, hipStream_t st)
 {
     if (p.tiled)
         hipLaunchKernelGGL(p.tiled, dim3(p.grid), dim3(kBlock), 0, st, mdp->n, mdp->m, mdp->ns,
-                           p.statesPerTile, mdp->gamma, mdp->d_S, mdp->d_T, mdp->d_R, V, Vp, pi);
+                           p.statesPerTile, mdp->gamma, mdp->d_S, mdp->d_T, mdp->d_R, V, Vp, pi,
+                           (int)writePi);
     else
         hipLaunchKernelGGL(vi_sweep_plain, dim3(p.grid), dim3(kBlock), 0, st, mdp->n, mdp->m,
-                           mdp->ns, mdp->gamma, mdp->d_S, mdp->d_T, mdp->d_R, V, Vp, pi);
+                           mdp->ns, mdp->gamma, mdp->d_S, mdp->d_T, mdp->d_R, V, Vp, pi,
+                           (int)writePi);
 }
 
 // Cached executable graph of a run of sweeps (one cache entry; re-captured when any launch
@@ -275,7 +284,7 @@ static int vi_run_sweeps(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned swee
         for (unsigned k = 0; k < sweeps; k++) {
             const bool even = ((parity + k) % 2) == 0;
             vi_launch(plan, mdp, even ? vi->d_V : vi->d_VPrime, even ? vi->d_VPrime : vi->d_V,
-                      vi->d_pi, st);
+                      vi->d_pi, k + 1 == sweeps, st);
         }
         e = hipStreamEndCapture(st, &graph);
         if (e != hipSuccess)
@@ -386,7 +395,8 @@ int mdp_vi_update_gpu(const MDP *mdp, MDPVIGPU *vi)
     hipStream_t st = stream();
     const ViPlan plan = vi_plan(mdp);
     const bool even = (vi->currentHorizon % 2) == 0;
-    vi_launch(plan, mdp, even ? vi->d_V : vi->d_VPrime, even ? vi->d_VPrime : vi->d_V, vi->d_pi, st);
+    vi_launch(plan, mdp, even ? vi->d_V : vi->d_VPrime, even ? vi->d_VPrime : vi->d_V, vi->d_pi,
+              true, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return hip_fail(e, fn, "Failed to execute the 'Bellman update' kernel.",

@@ -1,0 +1,114 @@
+"""The POMDP model object (mirrors python/nova/pomdp.py:40-402 of the reference)."""
+import ctypes as ct
+
+import numpy as np
+
+from . import _abi
+from ._arrays import fill, view
+
+
+class POMDP(_abi.NovaPOMDP):
+    """A POMDP whose arrays live in library-owned memory (new[] in pomdp_initialize_cpu)."""
+
+    def __init__(self):
+        super().__init__()
+        self.gamma = 0.9
+        self.horizon = 1
+        self.Rmin = None
+        self.Rmax = None
+        self.cpuIsInitialized = False
+        self.gpuIsInitialized = False
+        self._lib = _abi.load()
+
+    def __del__(self):
+        try:
+            self.uninitialize_gpu()
+            self.uninitialize()
+        except Exception:
+            pass
+
+    def initialize(self, n, ns, m, z, r, rz, gamma, horizon):
+        if self.cpuIsInitialized:
+            return
+        if self._lib.pomdp_initialize_cpu(self, n, ns, m, z, r, rz, gamma, horizon) != 0:
+            raise Exception("Failed to initialize the POMDP.")
+        self.cpuIsInitialized = True
+
+    def uninitialize(self):
+        if not self.cpuIsInitialized:
+            return
+        if self._lib.pomdp_uninitialize_cpu(self) != 0:
+            raise Exception("Failed to uninitialize the POMDP.")
+        self.cpuIsInitialized = False
+
+    def initialize_gpu(self):
+        if self.gpuIsInitialized:
+            return
+        if self._lib.pomdp_initialize_gpu(self) != 0:
+            raise Exception("Failed to initialize the GPU variables for the POMDP.")
+        self.gpuIsInitialized = True
+
+    def uninitialize_gpu(self):
+        if not self.gpuIsInitialized:
+            return
+        if self._lib.pomdp_uninitialize_gpu(self) != 0:
+            raise Exception("Failed to uninitialize the GPU variables for the POMDP.")
+        self.gpuIsInitialized = False
+
+    @classmethod
+    def from_arrays(cls, S, T, O, R, Z, B, gamma, horizon):
+        """S[n,m,ns] int32 (-1 ends a list), T[n,m,ns], O[m,n,z], R[n,m], Z[r,rz] int32,
+        B[r,rz]."""
+        S = np.asarray(S)
+        n, m, ns = S.shape
+        z = np.asarray(O).shape[2]
+        r, rz = np.asarray(Z).shape
+        p = cls()
+        p.initialize(n, ns, m, z, r, rz, float(gamma), int(horizon))
+        fill(p.S, S, np.int32)
+        fill(p.T, T, np.float32)
+        fill(p.O, O, np.float32)
+        fill(p.R, R, np.float32)
+        fill(p.Z, Z, np.int32)
+        fill(p.B, B, np.float32)
+        R32 = np.asarray(R, np.float32)
+        p.Rmin = float(R32.min())
+        p.Rmax = float(R32.max())
+        return p
+
+    @classmethod
+    def from_dict(cls, P):
+        return cls.from_arrays(P["S"], P["T"], P["O"], P["R"], P["Z"], P["B"], P["gamma"],
+                               P["horizon"])
+
+    def beliefs(self):
+        """(Z, B) as numpy copies shaped [r, rz]."""
+        return (view(self.Z, self.r * self.rz, np.int32).reshape(self.r, self.rz),
+                view(self.B, self.r * self.rz, np.float32).reshape(self.r, self.rz))
+
+    def expand(self, method="random", numBeliefsToAdd=1000, pemaPolicy=None):
+        """Expands the belief set (python/nova/pomdp.py:202-229). Host algorithms; call
+        initialize_gpu() again afterwards to refresh the device copy of Z/B."""
+        if method == "random":
+            rc = self._lib.pomdp_expand_random_cpu(self, numBeliefsToAdd)
+        elif method == "distinct_beliefs":
+            rc = self._lib.pomdp_expand_distinct_beliefs_cpu(self)
+        elif method == "pema":
+            rc = self._lib.pomdp_expand_pema_cpu(self, ct.byref(pemaPolicy))
+        else:
+            raise Exception("Failed to expand. Method '%s' is not defined." % method)
+        if rc != 0:
+            raise Exception("Belief expansion '%s' failed with code %d." % (method, rc))
+
+    def belief_update(self, b, a, o):
+        """tau(b, a, o) through pomdp_belief_update_cpu; returns (code, b')."""
+        b = np.ascontiguousarray(b, np.float32)
+        bp = _abi.c_float_p()
+        rc = self._lib.pomdp_belief_update_cpu(self, b.ctypes.data_as(_abi.c_float_p), a, o,
+                                               ct.byref(bp))
+        out = view(bp, self.n, np.float32) if bp else None
+        if bp:
+            # the library allocated bp with new[]; nova's ABI has no matching free, so keep the
+            # convention of its Python layer and let it be reclaimed by the process.
+            pass
+        return rc, out

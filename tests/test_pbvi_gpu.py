@@ -1,0 +1,182 @@
+"""POMDP PBVI backups on the GPU, through the C ABI, against the oracle.
+
+exact mode: nova's rounding order (src/pomdp/algorithms/pomdp_pbvi_cpu.cpp:37-162), so Gamma
+            and pi must equal the oracle's float twin bit for bit over whole runs.
+fast mode : FMA accumulation. From the same Gamma, the chosen alphas must match the oracle's
+            except at near-ties, stated here as a best-minus-runner-up gap below
+            FAST_TIE_REL * max(1, |best score|); the new alphas of beliefs whose choices all
+            match agree within 1e-5 relative; whole-run values b.alpha at the beliefs agree
+            with the FP64 oracle within 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits
+from nova_amd import synth
+from nova_amd.pomdp import POMDP
+from nova_amd.pomdp_pbvi import EXACT, FAST, POMDPPBVIGPU, set_mode
+
+pytestmark = pytest.mark.gpu
+
+FAST_TIE_REL = 1e-5
+
+
+def gpu_solve(P, mode, horizon=None, Gamma0=None):
+    set_mode(mode)
+    P = dict(P)
+    if horizon is not None:
+        P["horizon"] = horizon
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    s = POMDPPBVIGPU(p, GammaInitial=P["Gamma0"] if Gamma0 is None else Gamma0)
+    pol = s.solve()
+    G, pi = pol.Gamma_array(), pol.pi_array()
+    del s
+    set_mode(FAST)
+    return G, pi
+
+
+def simple_pomdp(gamma0):
+    # tests/unit/pomdp/pomdp_tests.cpp:34-67: n = m = z = r = rz = 1, R = 1, gamma = 0.9
+    return dict(S=np.zeros((1, 1, 1), np.int32), T=np.ones((1, 1, 1), np.float32),
+                O=np.ones((1, 1, 1), np.float32), R=np.ones((1, 1), np.float32),
+                Z=np.zeros((1, 1), np.int32), B=np.ones((1, 1), np.float32),
+                gamma=np.float32(0.9), horizon=3,
+                Gamma0=np.full((1, 1), gamma0, np.float32))
+
+
+def two_state_pomdp(c):
+    # tests/unit/pomdp/pomdp_tests.cpp:69-135
+    S = np.array([[[0, 1], [0, 1]], [[0, 1], [0, 1]]], np.int32)
+    T = np.array([[[0.8, 0.2], [0.2, 0.8]], [[0.8, 0.2], [0.2, 0.8]]], np.float32)
+    O = np.array([[[0.8, 0.2], [0.2, 0.8]], [[0.8, 0.2], [0.2, 0.8]]], np.float32)
+    R = np.array([[-1.0, 1.0], [1.0, -1.0]], np.float32)
+    return dict(S=S, T=T, O=O, R=R, Z=np.zeros((1, 1), np.int32), B=np.ones((1, 1), np.float32),
+                gamma=np.float32(0.9), horizon=1, Gamma0=np.full((1, 2), c, np.float32))
+
+
+@pytest.mark.parametrize("mode", [EXACT, FAST])
+def test_closed_forms(mode):
+    # alpha_{t+1} = 1 + 0.9 alpha_t: Gamma0 = 0, h = 3 -> 2.71; Gamma0 = 10 is the fixed point
+    G, pi = gpu_solve(simple_pomdp(0.0), mode)
+    assert abs(G[0, 0] - 2.71) < 1e-5 and pi[0] == 0
+    G, pi = gpu_solve(simple_pomdp(10.0), mode, horizon=7)
+    assert abs(G[0, 0] - 10.0) < 1e-5
+    # two-state POMDP with constant Gamma0 = c: alpha_1(s) = R(s,a) + gamma c, a* = 1 at b = e0
+    c = -4.0
+    G, pi = gpu_solve(two_state_pomdp(c), mode)
+    np.testing.assert_allclose(G[0], [1 + 0.9 * c, -1 + 0.9 * c], atol=1e-5)
+    assert pi[0] == 1
+
+
+def ragged(P, seed, frac=0.3):
+    """Cuts successor lists and belief supports short with -1 (the entries after a -1 must be
+    ignored even when they are valid states)."""
+    P = dict(P)
+    rng = np.random.default_rng(seed)
+    S = P["S"].copy()
+    n, m, ns = S.shape
+    cut = rng.random((n, m)) < frac
+    pos = rng.integers(1, ns, size=(n, m)) if ns > 1 else np.zeros((n, m), int)
+    for s, a in zip(*np.nonzero(cut)):
+        if ns > 1:
+            S[s, a, pos[s, a]] = -1
+    Z = P["Z"].copy()
+    r, rz = Z.shape
+    cutb = rng.random(r) < frac
+    posb = rng.integers(1, rz, size=r) if rz > 1 else np.zeros(r, int)
+    for i in np.nonzero(cutb)[0]:
+        if rz > 1:
+            Z[i, posb[i]] = -1
+    P["S"], P["Z"] = S, Z
+    return P
+
+
+SHAPES = [
+    # name, params, horizon, ragged
+    ("pbvi_cpu_n64", dict(synth.WORKLOADS["pbvi_cpu_n64"]), 100, False),
+    ("ragged_two_tiles", dict(kind="pomdp", n=150, m=3, z=20, ns=6, r=600, rz=12, gamma=0.95,
+                              horizon=4, seed=31), 4, True),
+    ("tiny", dict(kind="pomdp", n=5, m=2, z=3, ns=5, r=7, rz=5, gamma=0.9, horizon=20,
+                  seed=32), 20, True),
+]
+
+
+def _params(p):
+    p = dict(p)
+    p.pop("kind")
+    return p
+
+
+@pytest.mark.parametrize("name,params,horizon,rag", SHAPES, ids=[s[0] for s in SHAPES])
+def test_exact_bitwise_vs_oracle(oracle, name, params, horizon, rag):
+    P = synth.pomdp_arrays(**_params(params))
+    if rag:
+        P = ragged(P, params["seed"])
+    G, pi = gpu_solve(P, EXACT, horizon=horizon)
+    Go, pio = oracle.pbvi_run(P, horizon=horizon)
+    assert np.array_equal(bits(G), bits(Go)), "Gamma differs from the float oracle"
+    assert np.array_equal(pi, pio)
+    G64, _ = oracle.pbvi_run(P, horizon=horizon, dtype=np.float64)
+    err = np.max(np.abs(G - G64)) / max(1.0, np.max(np.abs(G64)))
+    print("%s: exact mode, ||Gamma - Gamma_fp64||inf/||Gamma||inf = %.3g" % (name, err))
+
+
+def sparse_values(P, G):
+    """V(b_i) = max_j b_i . Gamma_j on every belief of the set (FP64 sums)."""
+    Z, B = P["Z"], P["B"]
+    r, rz = Z.shape
+    V = np.empty(r)
+    for i in range(r):
+        k = np.argmax(Z[i] < 0) if (Z[i] < 0).any() else rz
+        V[i] = np.max(G[:, Z[i, :k]].astype(np.float64) @ B[i, :k].astype(np.float64))
+    return V
+
+
+@pytest.mark.parametrize("name,params,horizon,rag", SHAPES, ids=[s[0] for s in SHAPES])
+def test_fast_step_vs_oracle(oracle, name, params, horizon, rag):
+    """From the same Gamma (the oracle's after t iterations), one fast GPU iteration chooses
+    the oracle's alphas except at near-ties."""
+    P = synth.pomdp_arrays(**_params(params))
+    if rag:
+        P = ragged(P, params["seed"])
+    steps = [0, 1, min(3, horizon - 1)]
+    mismatch = 0
+    total = 0
+    for t in sorted(set(steps)):
+        Gt, _ = oracle.pbvi_run(P, horizon=t) if t > 0 else (P["Gamma0"], None)
+        ref = oracle.pbvi_backup(P, Gt)
+        set_mode(FAST)
+        p = POMDP.from_dict(P)
+        p.initialize_gpu()
+        s = POMDPPBVIGPU(p, GammaInitial=Gt)
+        s.update()
+        js, act = s.argmax_record()
+        G1 = s.get_policy().Gamma_array()
+        jo, gap = ref["jstar"], ref["gap"]
+        best_scale = np.maximum(1.0, np.abs(ref["aval"]))[:, :, None]
+        bad = (js != jo)
+        tie = gap <= FAST_TIE_REL * best_scale
+        assert not np.any(bad & ~tie), "alpha choice differs outside near-ties"
+        mismatch += int(bad.sum())
+        total += bad.size
+        ok_rows = ~bad.any(axis=(1, 2)) & (act == ref["pi"].astype(np.int32))
+        if ok_rows.any():
+            a = G1[ok_rows].astype(np.float64)
+            b = ref["GammaPrime"][ok_rows].astype(np.float64)
+            assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, np.max(np.abs(b)))
+        del s
+    print("%s: fast mode, %d of %d alpha choices differ (all near-ties)" % (name, mismatch, total))
+
+
+@pytest.mark.parametrize("name,params,horizon,rag", SHAPES[:2], ids=[s[0] for s in SHAPES[:2]])
+def test_fast_run_values_vs_fp64(oracle, name, params, horizon, rag):
+    P = synth.pomdp_arrays(**_params(params))
+    if rag:
+        P = ragged(P, params["seed"])
+    G, _ = gpu_solve(P, FAST, horizon=horizon)
+    G64, _ = oracle.pbvi_run(P, horizon=horizon, dtype=np.float64)
+    v, v64 = sparse_values(P, G), sparse_values(P, G64)
+    err = np.max(np.abs(v - v64)) / max(1.0, np.max(np.abs(v64)))
+    print("%s: fast mode, max_b |V(b) - V_fp64(b)| / ||V|| = %.3g" % (name, err))
+    assert err < 1e-4

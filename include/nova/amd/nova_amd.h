@@ -19,6 +19,28 @@ namespace nova {
 extern "C" int mdp_vi_run_gpu(const MDP *mdp, MDPVIGPU *vi, const float *d_V0,
         unsigned int sweeps, float *elapsedMs);
 
+// ---- differentiable value iteration
+
+// Solves V* = max_a (R + gamma T.V*) on the device to an FP64 residual max|V' - V| <= tol.
+// Starts from vi->d_V (float; vi initialized or its d_* fields set by the caller) with float
+// sweeps while their residual still halves every 8 sweeps, then continues in FP64 until the
+// residual is <= tol or has stopped falling (the FP64 rounding floor). d_V64 is
+// caller device memory of 2n doubles; on return d_V64[0..n) holds V* and vi->d_pi the greedy
+// policy of the last sweep. flags bit 0: skip the float phase and start from d_V64[0..n)
+// (a warm start). sweepsDone / residual (nullable) report the sweeps run and the last FP64
+// residual. Only device arrays of mdp (d_S, d_T, d_R) are used.
+extern "C" int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
+        unsigned int maxSweeps, unsigned int flags, unsigned int *sweepsDone, double *residual);
+
+// The adjoint of that solve at policy d_pi (n) and values d_V64 (n doubles): solves
+// (I - gamma P_pi)^T lambda = rho for d_rho (n floats) by FP32 fixed-point iterations until
+// max|lambda_{k+1} - lambda_k| <= tol * max|lambda - rho| (or maxIters), writes lambda to
+// d_lambda (n floats) and, when non-null, dL/dR into d_dR (n*m floats) and dL/dT into d_dT
+// (n*m*ns floats). residual64 gets max|lambda - rho - gamma P_pi^T lambda| in FP64.
+extern "C" int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi,
+        const double *d_V64, const float *d_rho, float *d_lambda, float *d_dR, float *d_dT,
+        float tol, unsigned int maxIters, unsigned int *itersDone, double *residual64);
+
 // ---- PBVI (include/nova/pomdp/algorithms/pomdp_pbvi_gpu.h)
 
 // Selects the arithmetic of the GPU PBVI backup for later calls in this process and returns

@@ -194,6 +194,11 @@ EXTRA_SIGNATURES = {
     "pomdp_pbvi_argmax_gpu": [_POMDP, _PBG, c_int_p, c_uint_p],
     "pomdp_expand_random_seeded_gpu": [_POMDP, U, U, _P(ct.c_uint), _P(c_float_p)],
     "nova_amd_free_floats": [c_float_p],
+    "mdp_vi_solve_gpu": [_MDP, _VIG, ct.c_void_p, ct.c_double, U, U, _P(ct.c_uint),
+                         _P(ct.c_double)],
+    "mdp_vi_adjoint_gpu": [_MDP, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                           ct.c_void_p, ct.c_void_p, ct.c_float, U, _P(ct.c_uint),
+                           _P(ct.c_double)],
 }
 
 _lib = None

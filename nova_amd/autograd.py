@@ -1,0 +1,98 @@
+"""PyTorch autograd for the Bellman fixed point V* = max_a (R + gamma T V*).
+
+BellmanFixedPoint.apply(R, T, S, gamma) returns V* (float32, n) for R (n x m) and T
+(n x m x ns) float32 CUDA tensors and the successor lists S (n x m x ns) int32 CUDA tensor.
+The tensors are used in place by the library (zero copy): MDP.d_S/d_T/d_R point at them.
+backward(g) solves the adjoint (I - gamma P_pi)^T lambda = g on the device and returns
+dL/dR and dL/dT (the successor lists and gamma get no gradient).
+"""
+import ctypes as ct
+
+import numpy as np
+import torch
+
+from . import _abi
+
+
+def _mdp_view(R, T, S, gamma):
+    n, m, ns = S.shape
+    mdp = _abi.NovaMDP()
+    mdp.n, mdp.m, mdp.ns = n, m, ns
+    mdp.gamma = float(gamma)
+    mdp.horizon = 1
+    mdp.d_S = ct.cast(S.data_ptr(), _abi.c_int_p)
+    mdp.d_T = ct.cast(T.data_ptr(), _abi.c_float_p)
+    mdp.d_R = ct.cast(R.data_ptr(), _abi.c_float_p)
+    return mdp
+
+
+def solve(R, T, S, gamma, tol=1e-6, max_sweeps=100000, V0=None, warm64=None):
+    """(V64, pi, sweeps, residual): the FP64 fixed point on the device and its greedy policy.
+    warm64: optional (n,) float64 CUDA tensor to start the FP64 phase from."""
+    lib = _abi.load()
+    n = S.shape[0]
+    dev = R.device
+    mdp = _mdp_view(R, T, S, gamma)
+    vi = _abi.NovaMDPVIGPU()
+    V = torch.zeros(n, dtype=torch.float32, device=dev) if V0 is None else V0.clone()
+    Vp = V.clone()
+    pi = torch.zeros(n, dtype=torch.int32, device=dev)
+    vi.d_V = ct.cast(V.data_ptr(), _abi.c_float_p)
+    vi.d_VPrime = ct.cast(Vp.data_ptr(), _abi.c_float_p)
+    vi.d_pi = ct.cast(pi.data_ptr(), _abi.c_uint_p)
+    V64 = torch.empty(2 * n, dtype=torch.float64, device=dev)
+    flags = 0
+    if warm64 is not None:
+        V64[:n].copy_(warm64)
+        flags = 1
+    torch.cuda.synchronize()
+    sweeps = ct.c_uint(0)
+    res = ct.c_double(0)
+    rc = lib.mdp_vi_solve_gpu(ct.byref(mdp), ct.byref(vi), V64.data_ptr(), tol, max_sweeps, flags,
+                              ct.byref(sweeps), ct.byref(res))
+    if rc != 0:
+        raise RuntimeError("mdp_vi_solve_gpu failed with code %d" % rc)
+    return V64[:n].clone(), pi, sweeps.value, res.value
+
+
+def adjoint(R, T, S, gamma, pi, V64, rho, tol=1e-7, max_iters=200000, grads=True):
+    """(lambda, dR, dT, iterations, fp64 residual) for the adjoint at (pi, V64)."""
+    lib = _abi.load()
+    n, m, ns = S.shape
+    dev = R.device
+    mdp = _mdp_view(R, T, S, gamma)
+    lam = torch.empty(n, dtype=torch.float32, device=dev)
+    dR = torch.empty((n, m), dtype=torch.float32, device=dev) if grads else None
+    dT = torch.empty((n, m, ns), dtype=torch.float32, device=dev) if grads else None
+    rho = rho.to(torch.float32).contiguous()
+    torch.cuda.synchronize()
+    it = ct.c_uint(0)
+    res = ct.c_double(0)
+    rc = lib.mdp_vi_adjoint_gpu(ct.byref(mdp), pi.data_ptr(), V64.data_ptr(), rho.data_ptr(),
+                                lam.data_ptr(), dR.data_ptr() if grads else None,
+                                dT.data_ptr() if grads else None, tol, max_iters,
+                                ct.byref(it), ct.byref(res))
+    if rc != 0:
+        raise RuntimeError("mdp_vi_adjoint_gpu failed with code %d" % rc)
+    return lam, dR, dT, it.value, res.value
+
+
+class BellmanFixedPoint(torch.autograd.Function):
+    """V* of an MDP as a differentiable function of (R, T)."""
+
+    @staticmethod
+    def forward(ctx, R, T, S, gamma, tol=1e-6):
+        R = R.contiguous()
+        T = T.contiguous()
+        S = S.contiguous()
+        V64, pi, sweeps, res = solve(R, T, S, gamma, tol=tol)
+        ctx.save_for_backward(R, T, S, pi, V64)
+        ctx.gamma = gamma
+        ctx.solve_info = (sweeps, res)
+        return V64.to(torch.float32)
+
+    @staticmethod
+    def backward(ctx, g):
+        R, T, S, pi, V64 = ctx.saved_tensors
+        _, dR, dT, _, _ = adjoint(R, T, S, ctx.gamma, pi, V64, g)
+        return dR, dT, None, None, None

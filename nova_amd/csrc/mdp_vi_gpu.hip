@@ -30,7 +30,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTileRows = 2048;   // (s,a) rows whose Q values a workgroup stages in LDS
-constexpr int kUnroll = 4;        // rows in flight per lane group
+constexpr int kUnroll = 8;        // rows in flight per lane group (8 measured ~4% faster than 4)
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
@@ -315,6 +315,12 @@ static int vi_run_sweeps(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned swee
                         NOVA_ERROR_DEVICE_SYNCHRONIZE);
     vi->currentHorizon += sweeps;
     return NOVA_SUCCESS;
+}
+
+int vi_sweeps_float(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned sweeps, hipStream_t st,
+                    const char *fn)
+{
+    return vi_run_sweeps(mdp, vi, sweeps, st, nullptr, nullptr, fn);
 }
 
 static bool mdp_device_ready(const nova::MDP *mdp)

@@ -1,0 +1,104 @@
+"""Differentiable value iteration (no reference counterpart): the residual-stopped solve, the
+adjoint (I - gamma P_pi)^T lambda = rho and the gradients dL/dR, dL/dT, against the FP64
+oracle and against FP64 central finite differences of device FP64 solves."""
+import numpy as np
+import pytest
+import torch
+
+from nova_amd import synth
+from nova_amd.autograd import BellmanFixedPoint, adjoint, solve
+
+pytestmark = pytest.mark.gpu
+
+
+def tensors(w):
+    dev = "cuda"
+    return (torch.from_numpy(w["R"]).to(dev), torch.from_numpy(w["T"]).to(dev),
+            torch.from_numpy(w["S"]).to(dev))
+
+
+def test_solve_adjoint_vs_oracle(oracle):
+    w = synth.mdp_arrays(n=600, m=4, ns=8, seed=51, gamma=0.95)
+    R, T, S = tensors(w)
+    V64, pi, sweeps, res = solve(R, T, S, float(w["gamma"]), tol=1e-10)
+    assert res <= 1e-10
+    Vo, pio, _, _ = oracle.vi_solve_f64(w["S"], w["T"], w["R"], w["gamma"], tol=1e-13)
+    np.testing.assert_allclose(V64.cpu().numpy(), Vo, rtol=0, atol=1e-8)
+    assert np.array_equal(pi.cpu().numpy().astype(np.uint32), pio)
+    rho = np.random.default_rng(0).uniform(0, 1, 600)
+    lam, dR, dT, it, r64 = adjoint(R, T, S, float(w["gamma"]), pi, V64,
+                                   torch.from_numpy(rho.astype(np.float32)).cuda())
+    lo = oracle.adjoint_f64(w["S"], w["T"], w["gamma"], pio, rho.astype(np.float32))
+    dRo, dTo = oracle.vi_grads_f64(w["S"], w["gamma"], pio, lo, Vo)
+    scale = np.max(np.abs(lo))
+    assert np.max(np.abs(lam.cpu().numpy() - lo)) <= 1e-5 * scale
+    assert np.max(np.abs(dR.cpu().numpy() - dRo)) <= 1e-5 * np.max(np.abs(dRo))
+    assert np.max(np.abs(dT.cpu().numpy() - dTo)) <= 1e-5 * np.max(np.abs(dTo))
+    print("adjoint: %d iterations, FP64 residual %.3g (scale %.3g)" % (it, r64, scale))
+    assert r64 <= 1e-5 * scale
+
+
+def test_autograd_matches_adjoint():
+    w = synth.mdp_arrays(n=300, m=3, ns=5, seed=52, gamma=0.9)
+    R, T, S = tensors(w)
+    R.requires_grad_(True)
+    T.requires_grad_(True)
+    V = BellmanFixedPoint.apply(R, T, S, float(w["gamma"]))
+    rho = torch.full((300,), 1.0 / 300, device="cuda")
+    (rho * V).sum().backward()
+    V64, pi, _, _ = solve(R.detach(), T.detach(), S, float(w["gamma"]))
+    _, dR, dT, _, _ = adjoint(R.detach(), T.detach(), S, float(w["gamma"]), pi, V64, rho)
+    assert torch.equal(R.grad, dR) and torch.equal(T.grad, dT)
+
+
+def test_finite_differences_full_size():
+    """n = 200,000, m = 8, ns = 16, gamma = 0.99, L = rho.V* with rho uniform: dL/dR and dL/dT
+    against FP64 central differences (h = 1e-4, 64 random coordinates, half of them on the
+    greedy action) of warm-started FP64 device solves."""
+    w = synth.workload("vi_diff_n2e5")
+    g = float(w["gamma"])
+    R, T, S = tensors(w)
+    n, m, ns = w["S"].shape
+    V64, pi, sweeps, res = solve(R, T, S, g, tol=1e-6)
+    print("forward: %d sweeps, FP64 residual %.3g" % (sweeps, res))
+    assert res <= 1e-6
+    rho = torch.full((n,), 1.0 / n, device="cuda")
+    _, dR, dT, it, r64 = adjoint(R, T, S, g, pi, V64, rho, tol=1e-8)
+    print("adjoint: %d iterations, FP64 residual %.3g" % (it, r64))
+    V64t, _, _, _ = solve(R, T, S, g, tol=1e-15, warm64=V64)
+    rng = np.random.default_rng(5)
+    pic = pi.cpu().numpy()
+    h = 1e-4
+    errs = []
+
+    def L(Rp, Tp):
+        Vx, _, _, r = solve(Rp, Tp, S, g, tol=1e-15, warm64=V64t)
+        return float((Vx * rho.double()).sum())
+
+    for c in range(64):
+        s = int(rng.integers(n))
+        on = c % 2 == 0
+        a = int(pic[s]) if on else int((pic[s] + 1 + rng.integers(m - 1)) % m)
+        if c < 32:
+            base = float(R[s, a])
+            Rp, Rm = R.clone(), R.clone()
+            Rp[s, a] = base + h
+            Rm[s, a] = base - h
+            dp = float(Rp[s, a]) - base
+            dm = base - float(Rm[s, a])
+            fd = (L(Rp, T) - L(Rm, T)) / (dp + dm)
+            an = float(dR[s, a])
+        else:
+            i = int(rng.integers(ns))
+            base = float(T[s, a, i])
+            Tp, Tm = T.clone(), T.clone()
+            Tp[s, a, i] = base + h
+            Tm[s, a, i] = base - h
+            dp = float(Tp[s, a, i]) - base
+            dm = base - float(Tm[s, a, i])
+            fd = (L(R, Tp) - L(R, Tm)) / (dp + dm)
+            an = float(dT[s, a, i])
+        scale = max(abs(fd), abs(an), 1e-12)
+        errs.append(abs(fd - an) / scale if max(abs(fd), abs(an)) > 1e-9 else abs(fd - an))
+    print("finite differences: max relative error %.3g over 64 coordinates" % max(errs))
+    assert max(errs) <= 1e-3

@@ -56,6 +56,15 @@ extern "C" int nova_amd_pbvi_mode(int mode);
 extern "C" int pomdp_pbvi_run_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi,
         unsigned int iterations, float *elapsedMs);
 
+// Batched PBVI: solves count POMDPs (each resident on the device, each solver struct with
+// GammaInitial set) exactly as pomdp_pbvi_execute_gpu would solve each one, but with every
+// iteration of every instance captured into one HIP graph of grouped launches (one launch per
+// stage for all instances). policies (nullable) is an array of count empty policy structs that
+// receive the results. elapsedMs (nullable) gets the device time of the graph.
+extern "C" int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps,
+        POMDPPBVIGPU *const *pbvis, unsigned int count, POMDPAlphaVectors *policies,
+        float *elapsedMs);
+
 // Copies the argmax record of the last iteration to the host: jstar[(i*m + a)*z + o] = the
 // alpha chosen for belief i, action a, observation o (r*m*z ints, nullable) and actions[i]
 // = the action chosen for belief i, or -1 when none beat -1e10 (r ints, nullable).

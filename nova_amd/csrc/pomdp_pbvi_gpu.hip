@@ -29,7 +29,16 @@
 // Model constant built once per initialize: OT[a][o][s][l] = O[a,S[s,a,l],o]*T[s,a,l] (0 past
 // the end of the successor list). It, GT and the per-iteration argmax records live in one
 // device workspace referenced by POMDPPBVIGPU::d_alphaBA.
+//
+// Every kernel reads the sizes and pointers of its problem from a device-resident instance
+// descriptor (PbviInst) and maps its workgroup to an instance through per-stage prefix offsets,
+// so one launch per stage serves any number of POMDPs of uneven size: a plain solve is a batch
+// of one, and pomdp_pbvi_batch_execute_gpu captures the iterations of many POMDPs in one graph
+// (each instance's results are identical to its plain solve). The scoring kernel has variants
+// (alphas per lane 1, 2 or 4); a tuner times them once per size class and caches the choice.
+#include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -40,20 +49,60 @@
 
 namespace nova_amd {
 
+// One problem of a (possibly single-element) batch, as the kernels see it.
+struct PbviInst {
+    unsigned n, ns, m, z, r, rz, tiles, horizon;
+    unsigned jpt;          // alphas per lane of the scoring variant chosen for this instance
+    int useLds;            // materialisation stages n floats in LDS
+    float gamma;
+    unsigned pad;
+    const int *S;
+    const float *T, *O, *R;
+    const int *Z;
+    const float *B;
+    float *G0, *G1;        // Gamma / GammaPrime
+    unsigned *pi;
+    float *OT, *GT, *partVal;
+    int *partJ, *jstar;
+    float *aval;
+    int *act;
+};
+
 namespace {
 
 constexpr int kPB = 256;            // lanes per workgroup
 constexpr int kUMax = 512;          // max valid (k, l) pairs of one belief-action in LDS
 constexpr int kKMax = 512;          // max belief support length handled in LDS
 constexpr int kZC = 16;             // observations per scoring chunk
-constexpr int kJPT = 2;             // alphas per lane
-constexpr int kJT = kPB * kJPT;     // alphas per scoring tile
 constexpr int kLdsRowFloats = 36 * 1024;  // Gamma row / h vector staged in LDS (144 KB)
+constexpr unsigned kNoHorizon = 0xffffffffu;
 
 __device__ __forceinline__ bool better(float v, int j, float bv, int bj)
 {
     // larger value wins; equal values: the smaller alpha index (the reference's first maximiser)
     return v > bv || (v == bv && j < bj);
+}
+
+// Instance of workgroup `id` given prefix offsets off[0..count] (off[b] <= id < off[b+1]).
+__device__ __forceinline__ unsigned find_inst(const unsigned *__restrict__ off, unsigned count,
+                                              unsigned id)
+{
+    unsigned lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const unsigned mid = (lo + hi + 1) / 2;
+        if (off[mid] <= id) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ const float *src_gamma(const PbviInst &P, unsigned iter, unsigned flip)
+{
+    return ((iter + flip) % 2 == 0) ? P.G0 : P.G1;
+}
+
+__device__ __forceinline__ float *dst_gamma(const PbviInst &P, unsigned iter, unsigned flip)
+{
+    return ((iter + flip) % 2 == 0) ? P.G1 : P.G0;
 }
 
 // --------------------------------------------------------------------- model constant
@@ -87,21 +136,28 @@ __global__ __launch_bounds__(kPB) void pbvi_build_ot(unsigned n, unsigned ns, un
 
 // ----------------------------------------------------------------------- 1. transpose
 
-__global__ __launch_bounds__(kPB) void gamma_transpose(unsigned r, unsigned n,
-        const float *__restrict__ G, float *__restrict__ GT)
+__global__ __launch_bounds__(kPB) void pbvi_transpose(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     __shared__ float tile[64][65];
-    const unsigned j0 = blockIdx.y * 64, s0 = blockIdx.x * 64;
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned local = blockIdx.x - off[b];
+    const unsigned tilesX = (P.n + 63) / 64;
+    const unsigned s0 = (local % tilesX) * 64, j0 = (local / tilesX) * 64;
+    const float *G = src_gamma(P, iter, flip);
     const unsigned tx = threadIdx.x % 64, ty = threadIdx.x / 64;
     for (unsigned q = ty; q < 64; q += 4) {
         const unsigned j = j0 + q, s = s0 + tx;
-        tile[q][tx] = (j < r && s < n) ? G[(size_t)j * n + s] : 0.0f;
+        tile[q][tx] = (j < P.r && s < P.n) ? G[(size_t)j * P.n + s] : 0.0f;
     }
     __syncthreads();
     for (unsigned q = ty; q < 64; q += 4) {
         const unsigned s = s0 + q, j = j0 + tx;
-        if (s < n && j < r)
-            GT[(size_t)s * r + j] = tile[tx][q];
+        if (s < P.n && j < P.r)
+            P.GT[(size_t)s * P.r + j] = tile[tx][q];
     }
 }
 
@@ -170,11 +226,9 @@ __device__ void build_support(SupportLds &L, unsigned i, unsigned a, unsigned m,
     __syncthreads();
 }
 
-template <bool EXACT>
-__global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsigned m,
-        unsigned z, unsigned r, unsigned rz, float gamma, const int *__restrict__ S,
-        const int *__restrict__ Z, const float *__restrict__ B, const float *__restrict__ OT,
-        const float *__restrict__ GT, float *__restrict__ partVal, int *__restrict__ partJ)
+template <bool EXACT, int JPT>
+__global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     __shared__ SupportLds L;
     __shared__ float w[kUMax * kZC];         // [u][o] factors of the current chunk
@@ -182,19 +236,28 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
     __shared__ float redV[kPB / 64][kZC];
     __shared__ int redJ[kPB / 64][kZC];
 
-    const unsigned ia = blockIdx.x;              // belief * m + action
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r;
+    const float gamma = P.gamma;
+    const unsigned local = blockIdx.x - off[b];
+    const unsigned ia = local % (r * m);         // belief * m + action
+    const unsigned tile = local / (r * m);
     const unsigned i = ia / m, a = ia % m;
-    const unsigned tile = blockIdx.y;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
-    build_support(L, i, a, m, ns, rz, S, Z, B, scratch);
+    build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);
     const int kEnd = L.kEnd, uEnd = L.uEnd;
+    const float *__restrict__ GT = P.GT;
+    const float *__restrict__ OT = P.OT;
 
-    int jq[kJPT];
-    bool live[kJPT];
+    int jq[JPT];
+    bool live[JPT];
 #pragma unroll
-    for (int q = 0; q < kJPT; q++) {
-        jq[q] = (int)(tile * kJT + t + q * kPB);
+    for (int q = 0; q < JPT; q++) {
+        jq[q] = (int)(tile * (kPB * JPT) + t + q * kPB);
         live[q] = jq[q] < (int)r;
     }
 
@@ -220,12 +283,12 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
         }
         __syncthreads();
 
-        float val[kZC][kJPT];
-        float vtk[EXACT ? kZC : 1][kJPT];
+        float val[kZC][JPT];
+        float vtk[EXACT ? kZC : 1][JPT];
 #pragma unroll
         for (int oo = 0; oo < kZC; oo++)
 #pragma unroll
-            for (int q = 0; q < kJPT; q++)
+            for (int q = 0; q < JPT; q++)
                 val[oo][q] = 0.0f;
 
         for (int k = 0; k < kEnd; k++) {
@@ -233,22 +296,22 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
 #pragma unroll
                 for (int oo = 0; oo < kZC; oo++)
 #pragma unroll
-                    for (int q = 0; q < kJPT; q++)
-                        vtk[oo][q] = 0.0f;
+                    for (int q = 0; q < JPT; q++)
+                        vtk[oo % (EXACT ? kZC : 1)][q] = 0.0f;
             }
             const int u1 = L.startK[k + 1];
             for (int u = L.startK[k]; u < u1; u++) {
                 const size_t rowOff = (size_t)L.spU[u] * r;
-                float g[kJPT];
+                float g[JPT];
 #pragma unroll
-                for (int q = 0; q < kJPT; q++)
+                for (int q = 0; q < JPT; q++)
                     g[q] = live[q] ? GT[rowOff + jq[q]] : 0.0f;
                 const float *wu = w + u * kZC;
 #pragma unroll
                 for (int oo = 0; oo < kZC; oo++) {
                     const float f = wu[oo];
 #pragma unroll
-                    for (int q = 0; q < kJPT; q++) {
+                    for (int q = 0; q < JPT; q++) {
                         if (EXACT)
                             vtk[oo % (EXACT ? kZC : 1)][q] =
                                 __fadd_rn(vtk[oo % (EXACT ? kZC : 1)][q], __fmul_rn(f, g[q]));
@@ -262,7 +325,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
 #pragma unroll
                 for (int oo = 0; oo < kZC; oo++)
 #pragma unroll
-                    for (int q = 0; q < kJPT; q++) {
+                    for (int q = 0; q < JPT; q++) {
                         const float v = __fmul_rn(vtk[oo % (EXACT ? kZC : 1)][q], gamma);
                         val[oo][q] = __fadd_rn(val[oo][q], __fmul_rn(v, bk));
                     }
@@ -277,7 +340,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
             bv[oo] = kLow;
             bj[oo] = INT_MAX;
 #pragma unroll
-            for (int q = 0; q < kJPT; q++)
+            for (int q = 0; q < JPT; q++)
                 if (live[q] && val[oo][q] > bv[oo]) {
                     bv[oo] = val[oo][q];
                     bj[oo] = jq[q];
@@ -286,9 +349,9 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
         // wave reduction, then across the 4 waves
 #pragma unroll
         for (int oo = 0; oo < kZC; oo++) {
-            for (int off = 32; off >= 1; off >>= 1) {
-                const float ov = __shfl_xor(bv[oo], off);
-                const int oj = __shfl_xor(bj[oo], off);
+            for (int off2 = 32; off2 >= 1; off2 >>= 1) {
+                const float ov = __shfl_xor(bv[oo], off2);
+                const int oj = __shfl_xor(bj[oo], off2);
                 if (better(ov, oj, bv[oo], bj[oo])) {
                     bv[oo] = ov;
                     bj[oo] = oj;
@@ -309,8 +372,8 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
                     j = redJ[wv][t];
                 }
             const size_t idx = ((size_t)tile * r * m + ia) * z + o0 + t;
-            partVal[idx] = v;
-            partJ[idx] = j;
+            P.partVal[idx] = v;
+            P.partJ[idx] = j;
         }
         __syncthreads();
     }
@@ -318,26 +381,27 @@ __global__ __launch_bounds__(kPB) void pbvi_score(unsigned n, unsigned ns, unsig
 
 // -------------------------------------------------------------------------- 3. action
 
-template <bool EXACT>
-__global__ __launch_bounds__(kPB) void pbvi_action(unsigned n, unsigned ns, unsigned m,
-        unsigned z, unsigned r, unsigned rz, unsigned tiles, float gamma,
-        const int *__restrict__ S, const float *__restrict__ R, const int *__restrict__ Z,
-        const float *__restrict__ B, const float *__restrict__ OT,
-        const float *__restrict__ G, const float *__restrict__ partVal,
-        const int *__restrict__ partJ, int *__restrict__ jstar, float *__restrict__ aval,
-        int *__restrict__ act)
+__global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     __shared__ int js[kZC * 8];
     __shared__ float alphaK[kKMax];
     __shared__ float bS[kKMax];
     __shared__ int kEndS;
-    const unsigned i = blockIdx.x, t = threadIdx.x;
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r, rz = P.rz;
+    const float gamma = P.gamma;
+    const float *__restrict__ G = src_gamma(P, iter, flip);
+    const unsigned i = blockIdx.x - off[b], t = threadIdx.x;
     if (t == 0)
         kEndS = (int)rz;
     __syncthreads();
     for (unsigned k = t; k < rz; k += kPB) {
-        bS[k] = B[(size_t)i * rz + k];
-        if (Z[(size_t)i * rz + k] < 0)
+        bS[k] = P.B[(size_t)i * rz + k];
+        if (P.Z[(size_t)i * rz + k] < 0)
             atomicMin(&kEndS, (int)k);
     }
     __syncthreads();
@@ -352,30 +416,30 @@ __global__ __launch_bounds__(kPB) void pbvi_action(unsigned n, unsigned ns, unsi
             for (unsigned oo = t; oo < oc; oo += kPB) {
                 float v = kLow;
                 int j = INT_MAX;
-                for (unsigned tl = 0; tl < tiles; tl++) {
+                for (unsigned tl = 0; tl < P.tiles; tl++) {
                     const size_t idx = ((size_t)tl * r * m + ia) * z + o0 + oo;
-                    if (partVal[idx] > v) {
-                        v = partVal[idx];
-                        j = partJ[idx];
+                    if (P.partVal[idx] > v) {
+                        v = P.partVal[idx];
+                        j = P.partJ[idx];
                     }
                 }
                 if (j == INT_MAX)
                     j = 0;   // no alpha above -1e10: the reference's index is undefined
                 js[oo] = j;
-                jstar[ia * z + o0 + oo] = j;
+                P.jstar[ia * z + o0 + oo] = j;
             }
             __syncthreads();
             // alpha_a on the support, observation-major accumulation as the reference
             for (int k = t; k < kEnd; k += kPB) {
-                const int s = Z[(size_t)i * rz + k];
+                const int s = P.Z[(size_t)i * rz + k];
                 const size_t row = ((size_t)s * m + a) * ns;
-                float al = (o0 == 0) ? R[(size_t)s * m + a] : alphaK[k];
+                float al = (o0 == 0) ? P.R[(size_t)s * m + a] : alphaK[k];
                 for (unsigned oo = 0; oo < oc; oo++) {
                     const float *Gj = G + (size_t)js[oo] * n;
-                    const float *ot = OT + ((size_t)(a * z + o0 + oo) * n + s) * ns;
+                    const float *ot = P.OT + ((size_t)(a * z + o0 + oo) * n + s) * ns;
                     float vt = 0.0f;
                     for (unsigned l = 0; l < ns; l++) {
-                        const int sp = S[row + l];
+                        const int sp = P.S[row + l];
                         if (sp < 0)
                             break;
                         vt = __fadd_rn(vt, __fmul_rn(ot[l], Gj[sp]));
@@ -390,7 +454,7 @@ __global__ __launch_bounds__(kPB) void pbvi_action(unsigned n, unsigned ns, unsi
             float v = 0.0f;
             for (int k = 0; k < kEnd; k++)
                 v = __fadd_rn(v, __fmul_rn(alphaK[k], bS[k]));
-            aval[ia] = v;
+            P.aval[ia] = v;
             if (v > bestValue) {
                 bestValue = v;
                 bestA = (int)a;
@@ -399,30 +463,34 @@ __global__ __launch_bounds__(kPB) void pbvi_action(unsigned n, unsigned ns, unsi
         __syncthreads();
     }
     if (t == 0)
-        act[i] = bestA;
+        P.act[i] = bestA;
 }
 
 // --------------------------------------------------------------------- 4. materialize
 
 template <bool EXACT>
-__global__ __launch_bounds__(kPB) void pbvi_materialize(unsigned n, unsigned ns, unsigned m,
-        unsigned z, float gamma, int useLds, const int *__restrict__ S,
-        const float *__restrict__ T, const float *__restrict__ O, const float *__restrict__ R,
-        const float *__restrict__ OT, const float *__restrict__ G,
-        const int *__restrict__ jstar, const int *__restrict__ act,
-        float *__restrict__ Gp, unsigned *__restrict__ pi)
+__global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
-    extern __shared__ float row[];   // kLdsRowFloats when useLds
-    const unsigned i = blockIdx.x, t = threadIdx.x;
-    const int a = act[i];
+    extern __shared__ float row[];   // n floats when the instance stages rows in LDS
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z;
+    const float gamma = P.gamma;
+    const int useLds = P.useLds;
+    const unsigned i = blockIdx.x - off[b], t = threadIdx.x;
+    const int a = P.act[i];
     if (a < 0)
         return;   // no action beat -1e10: Gamma'[i] and pi[i] keep their values
-    const int *js = jstar + ((size_t)i * m + a) * z;
-    float *out = Gp + (size_t)i * n;
+    const float *__restrict__ G = src_gamma(P, iter, flip);
+    float *out = dst_gamma(P, iter, flip) + (size_t)i * n;
+    const int *js = P.jstar + ((size_t)i * m + a) * z;
     if (EXACT || !useLds) {
         // alpha(s) = R[s,a] + sum_o (gamma * sum_l (O*T) * Gamma[j*_o, sp]), o ascending
         for (unsigned s = t; s < n; s += kPB)
-            out[s] = R[(size_t)s * m + a];
+            out[s] = P.R[(size_t)s * m + a];
         for (unsigned o = 0; o < z; o++) {
             const float *Gj = G + (size_t)js[o] * n;
             if (useLds) {
@@ -431,16 +499,16 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(unsigned n, unsigned ns,
                     row[s] = Gj[s];
                 __syncthreads();
             }
-            const float *src = useLds ? row : Gj;
-            const float *ot = OT + ((size_t)(a * z + o) * n) * ns;
+            const float *srcv = useLds ? row : Gj;
+            const float *ot = P.OT + ((size_t)(a * z + o) * n) * ns;
             for (unsigned s = t; s < n; s += kPB) {
                 const size_t rw = ((size_t)s * m + a) * ns;
                 float vt = 0.0f;
                 for (unsigned l = 0; l < ns; l++) {
-                    const int sp = S[rw + l];
+                    const int sp = P.S[rw + l];
                     if (sp < 0)
                         break;
-                    vt = __fadd_rn(vt, __fmul_rn(ot[(size_t)s * ns + l], src[sp]));
+                    vt = __fadd_rn(vt, __fmul_rn(ot[(size_t)s * ns + l], srcv[sp]));
                 }
                 out[s] = __fadd_rn(out[s], __fmul_rn(vt, gamma));
             }
@@ -448,7 +516,7 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(unsigned n, unsigned ns,
     } else {
         // h(sp) = sum_o O[a,sp,o] * Gamma[j*_o, sp] in LDS, then R + gamma*sum_l T*h(S)
         for (unsigned sp = t; sp < n; sp += kPB) {
-            const float *Os = O + ((size_t)a * n + sp) * z;
+            const float *Os = P.O + ((size_t)a * n + sp) * z;
             float h = 0.0f;
             for (unsigned o = 0; o < z; o++)
                 h = __builtin_fmaf(Os[o], G[(size_t)js[o] * n + sp], h);
@@ -459,16 +527,16 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(unsigned n, unsigned ns,
             const size_t rw = ((size_t)s * m + a) * ns;
             float acc = 0.0f;
             for (unsigned l = 0; l < ns; l++) {
-                const int sp = S[rw + l];
+                const int sp = P.S[rw + l];
                 if (sp < 0)
                     break;
-                acc = __builtin_fmaf(T[rw + l], row[sp], acc);
+                acc = __builtin_fmaf(P.T[rw + l], row[sp], acc);
             }
-            out[s] = __builtin_fmaf(gamma, acc, R[(size_t)s * m + a]);
+            out[s] = __builtin_fmaf(gamma, acc, P.R[(size_t)s * m + a]);
         }
     }
     if (t == 0)
-        pi[i] = (unsigned)a;
+        P.pi[i] = (unsigned)a;
 }
 
 } // namespace
@@ -476,23 +544,15 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(unsigned n, unsigned ns,
 // Scoring and materialisation arithmetic: 0 = fast (default), 1 = exact reference order.
 static int g_pbvi_mode = 0;
 
-// Device workspace carved out of one allocation (POMDPPBVIGPU::d_alphaBA).
-struct PbviWs {
-    float *OT;
-    float *GT;
-    float *partVal;
-    int *partJ;
-    int *jstar;
-    float *aval;
-    int *act;
-    unsigned tiles;
-};
-
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-static size_t pbvi_ws_bytes(const nova::POMDP *p, unsigned tiles)
+static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + kPB * jpt - 1) / (kPB * jpt); }
+
+// Bytes of one instance's workspace (tiles sized for the widest variant, jpt = 1).
+static size_t inst_ws_bytes(const nova::POMDP *p)
 {
     const size_t rmz = (size_t)p->r * p->m * p->z;
+    const unsigned tiles = tiles_for(p->r, 1);
     return align256(sizeof(float) * (size_t)p->m * p->z * p->n * p->ns) +
            align256(sizeof(float) * (size_t)p->n * p->r) +
            align256(sizeof(float) * tiles * rmz) + align256(sizeof(int) * tiles * rmz) +
@@ -500,26 +560,25 @@ static size_t pbvi_ws_bytes(const nova::POMDP *p, unsigned tiles)
            align256(sizeof(int) * p->r);
 }
 
-static PbviWs pbvi_ws(const nova::POMDP *p, void *base)
+// Carves an instance's workspace (OT first) out of base.
+static void inst_ws(const nova::POMDP *p, char *base, PbviInst &I)
 {
-    PbviWs w;
-    w.tiles = (p->r + kJT - 1) / kJT;
     const size_t rmz = (size_t)p->r * p->m * p->z;
-    char *c = static_cast<char *>(base);
-    w.OT = reinterpret_cast<float *>(c);
+    const unsigned tiles = tiles_for(p->r, 1);
+    char *c = base;
+    I.OT = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->m * p->z * p->n * p->ns);
-    w.GT = reinterpret_cast<float *>(c);
+    I.GT = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->n * p->r);
-    w.partVal = reinterpret_cast<float *>(c);
-    c += align256(sizeof(float) * w.tiles * rmz);
-    w.partJ = reinterpret_cast<int *>(c);
-    c += align256(sizeof(int) * w.tiles * rmz);
-    w.jstar = reinterpret_cast<int *>(c);
+    I.partVal = reinterpret_cast<float *>(c);
+    c += align256(sizeof(float) * tiles * rmz);
+    I.partJ = reinterpret_cast<int *>(c);
+    c += align256(sizeof(int) * tiles * rmz);
+    I.jstar = reinterpret_cast<int *>(c);
     c += align256(sizeof(int) * rmz);
-    w.aval = reinterpret_cast<float *>(c);
+    I.aval = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->r * p->m);
-    w.act = reinterpret_cast<int *>(c);
-    return w;
+    I.act = reinterpret_cast<int *>(c);
 }
 
 static bool pbvi_shape_ok(const nova::POMDP *p)
@@ -527,8 +586,8 @@ static bool pbvi_shape_ok(const nova::POMDP *p)
     return p->rz <= (unsigned)kKMax && (size_t)p->rz * p->ns <= (size_t)kUMax;
 }
 
-// Whether a Gamma row / h vector of n floats can be staged in LDS by the materialize kernels
-// (their dynamic LDS must be raised above the 64 KB default once per process).
+// Whether n floats can be staged in LDS by the materialize kernels (their dynamic LDS must be
+// raised above the 64 KB default once per process).
 static bool pbvi_lds_rows(unsigned n)
 {
     static int state = 0;   // 0 unknown, 1 ok, 2 unavailable
@@ -545,29 +604,206 @@ static bool pbvi_lds_rows(unsigned n)
     return state == 1 && n <= (unsigned)kLdsRowFloats;
 }
 
-// Enqueues one PBVI iteration G -> Gp on stream st.
-static void pbvi_enqueue(const nova::POMDP *p, const PbviWs &w, const float *G, float *Gp,
-                         unsigned *pi, int mode, hipStream_t st)
+static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, unsigned *pi,
+                      char *ws, unsigned horizon)
 {
-    const unsigned n = p->n, r = p->r, m = p->m;
-    dim3 tg((n + 63) / 64, (r + 63) / 64);
-    hipLaunchKernelGGL(gamma_transpose, tg, dim3(kPB), 0, st, r, n, G, w.GT);
-    dim3 sg(r * m, w.tiles);
-    if (mode == 1)
-        hipLaunchKernelGGL(pbvi_score<true>, sg, dim3(kPB), 0, st, n, p->ns, m, p->z, r, p->rz,
-                           p->gamma, p->d_S, p->d_Z, p->d_B, w.OT, w.GT, w.partVal, w.partJ);
-    else
-        hipLaunchKernelGGL(pbvi_score<false>, sg, dim3(kPB), 0, st, n, p->ns, m, p->z, r, p->rz,
-                           p->gamma, p->d_S, p->d_Z, p->d_B, w.OT, w.GT, w.partVal, w.partJ);
-    auto actFn = (mode == 1) ? pbvi_action<true> : pbvi_action<false>;
-    hipLaunchKernelGGL(actFn, dim3(r), dim3(kPB), 0, st, n, p->ns, m, p->z, r, p->rz, w.tiles,
-                       p->gamma, p->d_S, p->d_R, p->d_Z, p->d_B, w.OT, G, w.partVal, w.partJ,
-                       w.jstar, w.aval, w.act);
-    const int useLds = pbvi_lds_rows(n) ? 1 : 0;
-    const size_t lds = useLds ? sizeof(float) * n : 0;
+    std::memset(&I, 0, sizeof(I));
+    I.n = p->n; I.ns = p->ns; I.m = p->m; I.z = p->z; I.r = p->r; I.rz = p->rz;
+    I.horizon = horizon;
+    I.gamma = p->gamma;
+    I.jpt = 2;
+    I.tiles = tiles_for(p->r, I.jpt);
+    I.useLds = pbvi_lds_rows(p->n) ? 1 : 0;
+    I.S = p->d_S; I.T = p->d_T; I.O = p->d_O; I.R = p->d_R; I.Z = p->d_Z; I.B = p->d_B;
+    I.G0 = G0; I.G1 = G1; I.pi = pi;
+    inst_ws(p, ws, I);
+}
+
+// A launch plan over a set of instances: device descriptors and per-stage offsets.
+struct PbviPlan {
+    unsigned count = 0;
+    PbviInst *dInst = nullptr;       // device copy of the descriptors
+    unsigned *dOff = nullptr;        // [tr | sc(jpt=1) | sc(2) | sc(4) | be], count+1 each
+    unsigned totTr = 0, totSc[3] = {0, 0, 0}, totBe = 0;
+    size_t ldsBytes = 0;
+    unsigned maxHorizon = 0;
+    std::vector<PbviInst> host;
+};
+
+static int jpt_slot(unsigned jpt) { return jpt == 1 ? 0 : (jpt == 2 ? 1 : 2); }
+
+// Writes descriptors and offsets into dst (device memory of plan_bytes(count)).
+static size_t plan_bytes(unsigned count)
+{
+    return align256(sizeof(PbviInst) * count) + align256(sizeof(unsigned) * 5 * (count + 1));
+}
+
+static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
+{
+    const unsigned c = pl.count;
+    std::vector<unsigned> off(5 * (size_t)(c + 1), 0);
+    unsigned *tr = &off[0], *sc1 = &off[c + 1], *sc2 = &off[2 * (c + 1)],
+             *sc4 = &off[3 * (c + 1)], *be = &off[4 * (c + 1)];
+    unsigned *scs[3] = {sc1, sc2, sc4};
+    pl.ldsBytes = 0;
+    pl.maxHorizon = 0;
+    for (unsigned b = 0; b < c; b++) {
+        PbviInst &I = pl.host[b];
+        I.tiles = tiles_for(I.r, I.jpt);
+        tr[b + 1] = tr[b] + ((I.n + 63) / 64) * ((I.r + 63) / 64);
+        for (int v = 0; v < 3; v++)
+            scs[v][b + 1] = scs[v][b] + ((jpt_slot(I.jpt) == v) ? I.r * I.m * I.tiles : 0u);
+        be[b + 1] = be[b] + I.r;
+        if (I.useLds)
+            pl.ldsBytes = std::max(pl.ldsBytes, sizeof(float) * (size_t)I.n);
+        if (I.horizon != kNoHorizon)
+            pl.maxHorizon = std::max(pl.maxHorizon, I.horizon);
+    }
+    pl.totTr = tr[c];
+    for (int v = 0; v < 3; v++)
+        pl.totSc[v] = scs[v][c];
+    pl.totBe = be[c];
+    pl.dInst = static_cast<PbviInst *>(dst);
+    pl.dOff = reinterpret_cast<unsigned *>(static_cast<char *>(dst) +
+                                           align256(sizeof(PbviInst) * c));
+    hipError_t e = hipMemcpyAsync(pl.dInst, pl.host.data(), sizeof(PbviInst) * c,
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pl.dOff, off.data(), sizeof(unsigned) * off.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    return hip_fail(e, fn, "Failed to upload the PBVI launch plan.", NOVA_ERROR_MEMCPY_TO_DEVICE);
+}
+
+// Enqueues iteration `iter` of every instance of the plan.
+static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int mode,
+                         hipStream_t st)
+{
+    const unsigned c = pl.count;
+    const unsigned *tr = pl.dOff, *be = pl.dOff + 4 * (c + 1);
+    hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
+    for (int v = 0; v < 3; v++) {
+        if (pl.totSc[v] == 0)
+            continue;
+        const unsigned *sc = pl.dOff + (1 + v) * (c + 1);
+        dim3 g(pl.totSc[v]);
+        if (mode == 1) {
+            if (v == 0) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
+            if (v == 1) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
+            if (v == 2) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
+        } else {
+            if (v == 0) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
+            if (v == 1) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
+            if (v == 2) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
+        }
+    }
+    hipLaunchKernelGGL(pbvi_action, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
-    hipLaunchKernelGGL(matFn, dim3(r), dim3(kPB), lds, st, n, p->ns, m, p->z, p->gamma, useLds,
-                       p->d_S, p->d_T, p->d_O, p->d_R, w.OT, G, w.jstar, w.act, Gp, pi);
+    hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, iter,
+                       flip);
+}
+
+// Captures `iters` iterations of a plan into an executable graph.
+static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mode,
+                      hipStream_t st, hipGraphExec_t *exec, const char *fn)
+{
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Failed to begin graph capture.", NOVA_ERROR_KERNEL_EXECUTION);
+    for (unsigned k = 0; k < iters; k++)
+        plan_enqueue(pl, k, flip, mode, st);
+    e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Failed to capture the PBVI iterations.", NOVA_ERROR_KERNEL_EXECUTION);
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return hip_fail(e, fn, "Failed to instantiate the PBVI graph.", NOVA_ERROR_KERNEL_EXECUTION);
+    }
+    return NOVA_SUCCESS;
+}
+
+// ------------------------------------------------------------------------ the tuner
+//
+// The scoring kernel's variants (1, 2 or 4 alphas per lane) trade occupancy against reuse of
+// the staged O*T factors. For each size class (r, belief-support pairs rz*ns and z, each
+// bucketed to a power of two) the first instance seen is timed with every variant on one
+// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4
+// forces a variant; NOVA_AMD_AUTOTUNE=0 disables timing (default variant 2).
+
+struct TuneKey {
+    unsigned r, u, z;
+    bool operator==(const TuneKey &o) const { return r == o.r && u == o.u && z == o.z; }
+};
+static std::vector<std::pair<TuneKey, unsigned>> g_tuned;
+
+static unsigned bucket(unsigned x)
+{
+    unsigned p = 1;
+    while (p < x)
+        p <<= 1;
+    return p;
+}
+
+static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
+{
+    if (const char *f = std::getenv("NOVA_AMD_PBVI_JPT")) {
+        const int v = std::atoi(f);
+        if (v == 1 || v == 2 || v == 4)
+            return (unsigned)v;
+    }
+    if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
+        if (std::atoi(f) == 0)
+            return 2;
+    const TuneKey key{bucket(I.r), bucket(I.rz * I.ns), bucket(I.z)};
+    for (auto &kv : g_tuned)
+        if (kv.first == key)
+            return kv.second;
+    unsigned best = 2;
+    float bestMs = 1e30f;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (unsigned jpt : {1u, 2u, 4u}) {
+        PbviPlan pl;
+        pl.count = 1;
+        I.jpt = jpt;
+        I.horizon = kNoHorizon;
+        pl.host.assign(1, I);
+        if (plan_upload(pl, planMem, st, "pbvi tuner") != NOVA_SUCCESS)
+            continue;
+        const unsigned tr = pl.totTr;
+        hipLaunchKernelGGL(pbvi_transpose, dim3(tr), dim3(kPB), 0, st, pl.dInst, pl.dOff, 1u, 0u, 0u);
+        const unsigned *sc = pl.dOff + (1 + jpt_slot(jpt)) * 2;
+        dim3 g(pl.totSc[jpt_slot(jpt)]);
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; rep++) {   // first launch warms up
+            hipEventRecord(e0, st);
+            if (mode == 1) {
+                if (jpt == 1) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
+                if (jpt == 2) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
+                if (jpt == 4) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
+            } else {
+                if (jpt == 1) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
+                if (jpt == 2) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
+                if (jpt == 4) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
+            }
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (ms < bestMs) {
+            bestMs = ms;
+            best = jpt;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    g_tuned.push_back({key, best});
+    return best;
 }
 
 static bool pomdp_device_ready(const nova::POMDP *p)
@@ -577,59 +813,70 @@ static bool pomdp_device_ready(const nova::POMDP *p)
            p->d_R != nullptr && p->d_Z != nullptr && p->d_B != nullptr;
 }
 
-// Cached executable graph of `iters` iterations.
+// The plain solver's workspace: [instance workspace | plan (descriptor + offsets)].
+static char *plain_plan_mem(const nova::POMDP *p, nova::POMDPPBVIGPU *pb)
+{
+    return reinterpret_cast<char *>(pb->d_alphaBA) + align256(inst_ws_bytes(p));
+}
+
+// Cached executable graph of a plain solver's run.
 struct PbviGraph {
     hipGraphExec_t exec = nullptr;
     const void *key[12] = {};
-    unsigned iters = 0, parity = 0, dims[6] = {};
+    unsigned iters = 0, flip = 0, dims[6] = {};
     float gamma = 0.0f;
     int mode = -1;
+    unsigned jpt = 0;
 };
 static PbviGraph g_pgraph;
+
+// jpt chosen for a plain solver at initialize (by the tuner), per workspace pointer
+static std::vector<std::pair<const void *, unsigned>> g_plain_jpt;
+
+static unsigned plain_jpt(const void *ws)
+{
+    for (auto &kv : g_plain_jpt)
+        if (kv.first == ws)
+            return kv.second;
+    return 2;
+}
 
 static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned iters,
                           hipStream_t st, hipEvent_t t0, hipEvent_t t1, const char *fn)
 {
-    const PbviWs w = pbvi_ws(p, pb->d_alphaBA);
-    const unsigned parity = pb->currentHorizon % 2;
+    const unsigned flip = pb->currentHorizon % 2;
+    const unsigned jpt = plain_jpt(pb->d_alphaBA);
     const void *key[12] = {p->d_S, p->d_T, p->d_O, p->d_R, p->d_Z, p->d_B, pb->d_Gamma,
                            pb->d_GammaPrime, pb->d_pi, pb->d_alphaBA, nullptr, nullptr};
     const unsigned dims[6] = {p->n, p->ns, p->m, p->z, p->r, p->rz};
-    const bool hit = g_pgraph.exec && g_pgraph.iters == iters && g_pgraph.parity == parity &&
+    const bool hit = g_pgraph.exec && g_pgraph.iters == iters && g_pgraph.flip == flip &&
                      g_pgraph.gamma == p->gamma && g_pgraph.mode == g_pbvi_mode &&
-                     std::memcmp(g_pgraph.key, key, sizeof(key)) == 0 &&
+                     g_pgraph.jpt == jpt && std::memcmp(g_pgraph.key, key, sizeof(key)) == 0 &&
                      std::memcmp(g_pgraph.dims, dims, sizeof(dims)) == 0;
     if (!hit) {
         if (g_pgraph.exec) {
             hipGraphExecDestroy(g_pgraph.exec);
             g_pgraph.exec = nullptr;
         }
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess)
-            return hip_fail(e, fn, "Failed to begin graph capture.", NOVA_ERROR_KERNEL_EXECUTION);
-        for (unsigned k = 0; k < iters; k++) {
-            const bool even = ((parity + k) % 2) == 0;
-            pbvi_enqueue(p, w, even ? pb->d_Gamma : pb->d_GammaPrime,
-                         even ? pb->d_GammaPrime : pb->d_Gamma, pb->d_pi, g_pbvi_mode, st);
-        }
-        e = hipStreamEndCapture(st, &graph);
-        if (e != hipSuccess)
-            return hip_fail(e, fn, "Failed to capture the PBVI iterations.",
-                            NOVA_ERROR_KERNEL_EXECUTION);
-        e = hipGraphInstantiate(&g_pgraph.exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            g_pgraph.exec = nullptr;
-            return hip_fail(e, fn, "Failed to instantiate the PBVI graph.",
-                            NOVA_ERROR_KERNEL_EXECUTION);
-        }
+        PbviPlan pl;
+        pl.count = 1;
+        pl.host.resize(1);
+        fill_inst(p, pl.host[0], pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
+                  reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
+        pl.host[0].jpt = jpt;
+        int rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
+        if (rc != NOVA_SUCCESS)
+            return rc;
+        rc = plan_graph(pl, iters, flip, g_pbvi_mode, st, &g_pgraph.exec, fn);
+        if (rc != NOVA_SUCCESS)
+            return rc;
         std::memcpy(g_pgraph.key, key, sizeof(key));
         std::memcpy(g_pgraph.dims, dims, sizeof(dims));
         g_pgraph.iters = iters;
-        g_pgraph.parity = parity;
+        g_pgraph.flip = flip;
         g_pgraph.gamma = p->gamma;
         g_pgraph.mode = g_pbvi_mode;
+        g_pgraph.jpt = jpt;
     }
     if (t0) hipEventRecord(t0, st);
     hipError_t e = hipGraphLaunch(g_pgraph.exec, st);
@@ -642,6 +889,15 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
                         NOVA_ERROR_DEVICE_SYNCHRONIZE);
     pb->currentHorizon += iters;
     return NOVA_SUCCESS;
+}
+
+static int build_ot(const nova::POMDP *p, float *OT, hipStream_t st)
+{
+    const size_t total = (size_t)p->m * p->z * p->n * p->ns;
+    const unsigned grid = (unsigned)std::min<size_t>((total + kPB - 1) / kPB, 65536);
+    hipLaunchKernelGGL(pbvi_build_ot, dim3(grid), dim3(kPB), 0, st, p->n, p->ns, p->m, p->z,
+                       p->d_S, p->d_T, p->d_O, OT);
+    return hipGetLastError() == hipSuccess ? NOVA_SUCCESS : NOVA_ERROR_KERNEL_EXECUTION;
 }
 
 } // namespace nova_amd
@@ -681,14 +937,14 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     pbvi->currentHorizon = 0;
 
     const size_t rn = (size_t)pomdp->r * pomdp->n;
-    const unsigned tiles = (pomdp->r + kJT - 1) / kJT;
     int rc = device_alloc((void **)&pbvi->d_Gamma, rn * sizeof(float), fn, "Gamma");
     if (rc == NOVA_SUCCESS)
         rc = device_alloc((void **)&pbvi->d_GammaPrime, rn * sizeof(float), fn, "Gamma (prime)");
     if (rc == NOVA_SUCCESS)
         rc = device_alloc((void **)&pbvi->d_pi, pomdp->r * sizeof(unsigned), fn, "pi");
     if (rc == NOVA_SUCCESS)
-        rc = device_alloc((void **)&pbvi->d_alphaBA, pbvi_ws_bytes(pomdp, tiles), fn,
+        rc = device_alloc((void **)&pbvi->d_alphaBA,
+                          align256(inst_ws_bytes(pomdp)) + plan_bytes(1), fn,
                           "the PBVI workspace");
     if (rc != NOVA_SUCCESS)
         return rc;
@@ -702,17 +958,24 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     if (e != hipSuccess)
         return hip_fail(e, fn, "Failed to copy memory from host to device for Gamma.",
                         NOVA_ERROR_MEMCPY_TO_DEVICE);
-    const PbviWs w = pbvi_ws(pomdp, pbvi->d_alphaBA);
-    const size_t total = (size_t)pomdp->m * pomdp->z * pomdp->n * pomdp->ns;
-    const unsigned grid = (unsigned)std::min<size_t>((total + kPB - 1) / kPB, 65536);
     hipStream_t st = stream();
-    hipLaunchKernelGGL(pbvi_build_ot, dim3(grid), dim3(kPB), 0, st, pomdp->n, pomdp->ns,
-                       pomdp->m, pomdp->z, pomdp->d_S, pomdp->d_T, pomdp->d_O, w.OT);
-    e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(st);
-    return hip_fail(e, fn, "Failed to build the observation-transition products.",
-                    NOVA_ERROR_KERNEL_EXECUTION);
+    PbviInst I;
+    fill_inst(pomdp, I, pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
+              reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
+    rc = build_ot(pomdp, I.OT, st);
+    if (rc != NOVA_SUCCESS)
+        return hip_fail(hipGetLastError(), fn, "Failed to build the O*T products.", rc);
+    const unsigned jpt = tune_jpt(I, plain_plan_mem(pomdp, pbvi), st, g_pbvi_mode);
+    bool found = false;
+    for (auto &kv : g_plain_jpt)
+        if (kv.first == pbvi->d_alphaBA) {
+            kv.second = jpt;
+            found = true;
+        }
+    if (!found)
+        g_plain_jpt.push_back({pbvi->d_alphaBA, jpt});
+    e = hipStreamSynchronize(st);
+    return hip_fail(e, fn, "Failed to prepare the PBVI workspace.", NOVA_ERROR_KERNEL_EXECUTION);
 }
 
 int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
@@ -725,6 +988,10 @@ int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     }
     int rc = NOVA_SUCCESS;
     pbvi->currentHorizon = 0;
+    if (g_pgraph.exec && g_pgraph.key[9] == pbvi->d_alphaBA) {
+        hipGraphExecDestroy(g_pgraph.exec);
+        g_pgraph.exec = nullptr;
+    }
     if (device_free_if_owned(pbvi->d_Gamma, fn, "Gamma") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(pbvi->d_GammaPrime, fn, "Gamma (prime)") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(pbvi->d_pi, fn, "pi") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
@@ -818,15 +1085,96 @@ int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi, int *jst
         NOVA_ERR(fn, "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    const PbviWs w = pbvi_ws(pomdp, pbvi->d_alphaBA);
+    PbviInst I;
+    inst_ws(pomdp, reinterpret_cast<char *>(pbvi->d_alphaBA), I);
     hipError_t e = hipSuccess;
     if (jstar)
-        e = hipMemcpy(jstar, w.jstar, sizeof(int) * (size_t)pomdp->r * pomdp->m * pomdp->z,
+        e = hipMemcpy(jstar, I.jstar, sizeof(int) * (size_t)pomdp->r * pomdp->m * pomdp->z,
                       hipMemcpyDeviceToHost);
     if (e == hipSuccess && actions)
-        e = hipMemcpy(actions, w.act, sizeof(int) * pomdp->r, hipMemcpyDeviceToHost);
+        e = hipMemcpy(actions, I.act, sizeof(int) * pomdp->r, hipMemcpyDeviceToHost);
     return hip_fail(e, fn, "Failed to copy the argmax record to the host.",
                     NOVA_ERROR_MEMCPY_TO_HOST);
+}
+
+// Batched PBVI: every POMDP must be resident on the device (pomdp_initialize_gpu) and each
+// solver struct carry GammaInitial. Equivalent to pomdp_pbvi_execute_gpu on each pair, with
+// all iterations of all instances captured in one graph (grouped launches per stage).
+int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps, POMDPPBVIGPU *const *pbvis,
+        unsigned int count, POMDPAlphaVectors *policies, float *elapsedMs)
+{
+    const char *fn = "pomdp_pbvi_batch_execute_gpu";
+    if (pomdps == nullptr || pbvis == nullptr || count == 0) {
+        NOVA_ERR(fn, "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    for (unsigned b = 0; b < count; b++) {
+        const POMDP *p = pomdps[b];
+        if (!pomdp_device_ready(p) || p->horizon < 1 || pbvis[b] == nullptr ||
+                pbvis[b]->GammaInitial == nullptr || !pbvi_shape_ok(p)) {
+            NOVA_ERR(fn, "Invalid instance (device-resident POMDP, GammaInitial and supported "
+                         "belief support lengths required).");
+            return NOVA_ERROR_INVALID_DATA;
+        }
+    }
+    int rc = NOVA_SUCCESS;
+    for (unsigned b = 0; b < count && rc == NOVA_SUCCESS; b++)
+        rc = pomdp_pbvi_initialize_gpu(pomdps[b], pbvis[b]);
+    void *planMem = nullptr;
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc(&planMem, plan_bytes(count), fn, "the batch plan");
+    hipStream_t st = stream();
+    PbviPlan pl;
+    if (rc == NOVA_SUCCESS) {
+        pl.count = count;
+        pl.host.resize(count);
+        for (unsigned b = 0; b < count; b++) {
+            fill_inst(pomdps[b], pl.host[b], pbvis[b]->d_Gamma, pbvis[b]->d_GammaPrime,
+                      pbvis[b]->d_pi, reinterpret_cast<char *>(pbvis[b]->d_alphaBA),
+                      pomdps[b]->horizon);
+            pl.host[b].jpt = plain_jpt(pbvis[b]->d_alphaBA);
+        }
+        rc = plan_upload(pl, planMem, st, fn);
+    }
+    hipGraphExec_t exec = nullptr;
+    if (rc == NOVA_SUCCESS)
+        rc = plan_graph(pl, pl.maxHorizon, 0, g_pbvi_mode, st, &exec, fn);
+    if (rc == NOVA_SUCCESS) {
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (elapsedMs) {
+            hipEventCreate(&t0);
+            hipEventCreate(&t1);
+            hipEventRecord(t0, st);
+        }
+        hipError_t e = hipGraphLaunch(exec, st);
+        if (elapsedMs)
+            hipEventRecord(t1, st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            rc = hip_fail(e, fn, "Failed to execute the batch graph.", NOVA_ERROR_KERNEL_EXECUTION);
+        if (elapsedMs) {
+            float ms = 0.0f;
+            if (rc == NOVA_SUCCESS)
+                hipEventElapsedTime(&ms, t0, t1);
+            *elapsedMs = ms;
+            hipEventDestroy(t0);
+            hipEventDestroy(t1);
+        }
+        hipGraphExecDestroy(exec);
+    }
+    for (unsigned b = 0; b < count; b++) {
+        if (rc == NOVA_SUCCESS) {
+            pbvis[b]->currentHorizon = pomdps[b]->horizon;
+            if (policies)
+                rc = pomdp_pbvi_get_policy_gpu(pomdps[b], pbvis[b], &policies[b]);
+        }
+    }
+    for (unsigned b = 0; b < count; b++)
+        pomdp_pbvi_uninitialize_gpu(pomdps[b], pbvis[b]);
+    if (planMem)
+        device_free(planMem, fn, "the batch plan");
+    return rc;
 }
 
 } // namespace nova

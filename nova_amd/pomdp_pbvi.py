@@ -114,3 +114,31 @@ class POMDPPBVIGPU(_abi.NovaPOMDPPBVIGPU):
         if rc != 0:
             raise Exception("pomdp_pbvi_argmax_gpu failed")
         return js, act
+
+
+def batch_execute(solvers, policies=True):
+    """Runs pomdp_pbvi_batch_execute_gpu over POMDPPBVIGPU solvers (their POMDPs resident on
+    the device). Returns (policies or None, device ms)."""
+    lib = _abi.load()
+    count = len(solvers)
+    PP = ct.POINTER(_abi.NovaPOMDP) * count
+    SP = ct.POINTER(_abi.NovaPOMDPPBVIGPU) * count
+    parr = PP(*[ct.pointer(s.pomdp) for s in solvers])
+    sarr = SP(*[ct.pointer(s) for s in solvers])
+    pols = (_abi.NovaPOMDPAlphaVectors * count)() if policies else None
+    ms = ct.c_float(0.0)
+    rc = lib.pomdp_pbvi_batch_execute_gpu(ct.cast(parr, ct.c_void_p), ct.cast(sarr, ct.c_void_p),
+                                          count, ct.cast(pols, ct.c_void_p) if pols else None,
+                                          ct.byref(ms))
+    if rc != 0:
+        raise Exception("pomdp_pbvi_batch_execute_gpu failed with code %d" % rc)
+    out = None
+    if pols is not None:
+        out = []
+        for k in range(count):
+            p = pols[k]
+            G = np.ctypeslib.as_array(p.Gamma, (p.r * p.n,)).reshape(p.r, p.n).copy()
+            pi = np.ctypeslib.as_array(p.pi, (p.r,)).copy()
+            lib.pomdp_alpha_vectors_uninitialize(ct.byref(p))
+            out.append((G, pi))
+    return out, ms.value

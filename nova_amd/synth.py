@@ -87,3 +87,21 @@ def workload(name, **overrides):
     p.update(overrides)
     kind = p.pop("kind")
     return (mdp_arrays if kind == "mdp" else pomdp_arrays)(**p)
+
+
+def batch_instances(count=256, seed=4, n_range=(500, 8000), r_range=(256, 2048), rz=16,
+                    gamma=0.95, horizon=50, m_choices=range(4, 17), z_choices=range(4, 33),
+                    ns_choices=(4, 8)):
+    """The uneven batch: per instance n and r log-uniform, m, z, ns uniform over their choices,
+    rz fixed; each instance drawn by its own generator seeded (seed, index)."""
+    meta = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        n = int(round(np.exp(meta.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        r = int(round(np.exp(meta.uniform(np.log(r_range[0]), np.log(r_range[1])))))
+        m = int(meta.choice(list(m_choices)))
+        z = int(meta.choice(list(z_choices)))
+        ns = int(meta.choice(list(ns_choices)))
+        out.append(pomdp_arrays(n=n, m=m, z=z, ns=ns, r=r, rz=min(rz, n), gamma=gamma,
+                                horizon=horizon, seed=(seed << 20) + k))
+    return out

@@ -70,6 +70,7 @@ def test_finite_differences_full_size():
     pic = pi.cpu().numpy()
     h = 1e-4
     errs = []
+    rows = []
 
     def L(Rp, Tp):
         Vx, _, _, r = solve(Rp, Tp, S, g, tol=1e-15, warm64=V64t)
@@ -100,5 +101,9 @@ def test_finite_differences_full_size():
             an = float(dT[s, a, i])
         scale = max(abs(fd), abs(an), 1e-12)
         errs.append(abs(fd - an) / scale if max(abs(fd), abs(an)) > 1e-9 else abs(fd - an))
+        rows.append((c, "R" if c < 32 else "T", s, a, on, fd, an, errs[-1]))
+    for row in rows:
+        if row[-1] > 1e-3:
+            print("  coord %d %s s=%d a=%d on=%s fd=%.6g analytic=%.6g err=%.3g" % row)
     print("finite differences: max relative error %.3g over 64 coordinates" % max(errs))
     assert max(errs) <= 1e-3

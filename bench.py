@@ -135,6 +135,103 @@ def bench_vi(args):
         out["cpu_baseline"] = cpu_baseline_vi(w, args.cpu_sweeps)
     else:
         out["cpu_baseline"] = None
+    print("vi leg done", file=sys.stderr, flush=True)
+    return out
+
+
+def pbvi_flops_per_iteration(P):
+    """2 * sum over (belief, action) of valid (support, successor) pairs * z * r: the
+    multiply-adds of the scoring kernel (the dominant one)."""
+    S, Z = P["S"], P["Z"]
+    n, m, ns = S.shape
+    z = P["O"].shape[2]
+    r, rz = Z.shape
+    valid_len = np.where((S < 0).any(axis=2), np.argmax(S < 0, axis=2), ns)   # n x m
+    total = 0
+    for i in range(r):
+        k = int(np.argmax(Z[i] < 0)) if (Z[i] < 0).any() else rz
+        total += int(valid_len[Z[i, :k]].sum())
+    return 2.0 * total * z * r
+
+
+def bench_pbvi(args):
+    """PBVI at n=20,000, m=16, z=32, ns=8, r=4,096, rz=64, 50 iterations (fast mode)."""
+    from nova_amd import synth
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, set_mode
+    set_mode(FAST)
+    P = synth.workload("pbvi_n20000")
+    r, iters = P["Z"].shape[0], P["horizon"]
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    s = POMDPPBVIGPU(p, GammaInitial=P["Gamma0"])
+    stages = s.profile()                      # warm-up iteration, per-stage times
+    ms = s.run(iters)                         # one full solve's iterations
+    t0 = time.perf_counter()
+    s.run(iters)
+    wall = time.perf_counter() - t0
+    flops = pbvi_flops_per_iteration(P)
+    score_ms = stages[1]
+    achieved = flops / (score_ms * 1e-3) / 1e12
+    out = {"workload": "pbvi_n20000", "backups_per_s": r * iters / wall,
+           "ms_per_iteration": ms / iters, "stage_ms": dict(zip(
+               ["transpose", "score", "action", "materialize"], stages)),
+           "roofline": {"bound": "valu_fp32", "kernel": "pbvi_score", "achieved": achieved,
+                        "peak": 157.3, "unit": "TFLOP/s", "frac": achieved / 157.3,
+                        "peak_nonpacked_issue": 78.6,
+                        "algorithmic_flops_per_launch": flops}}
+    del s
+    p.uninitialize_gpu()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        o = Oracle()
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            k = args.cpu_beliefs
+            t = time.perf_counter()
+            o.pbvi_backup(P, P["Gamma0"], 0, k, diag=False)
+            dt = time.perf_counter() - t
+        finally:
+            os.sched_setaffinity(0, old)
+        out["cpu_baseline"] = {"value": k / dt, "unit": "backups/s", "cores": 1, "kind": "port",
+                               "sample": "%d beliefs backed up against all %d alphas (one "
+                                         "iteration's per-belief work), 1 pinned core" % (k, r)}
+    return out
+
+
+def bench_batch(args):
+    """256 uneven POMDPs: batched graph vs 256 sequential plain GPU solves."""
+    from nova_amd import synth
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, batch_execute, set_mode
+    set_mode(FAST)
+    batch = synth.batch_instances(count=args.batch_count)
+    models = []
+    for P in batch:
+        p = POMDP.from_dict(P)
+        p.initialize_gpu()
+        models.append(p)
+    solvers = [POMDPPBVIGPU(p, GammaInitial=P["Gamma0"]) for p, P in zip(models, batch)]
+    t0 = time.perf_counter()
+    plain = []
+    for s in solvers:
+        pol = s.solve()
+        plain.append((pol.Gamma_array(), pol.pi_array()))
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res, dev_ms = batch_execute(solvers)
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
+               np.array_equal(a[1], b[1]) for a, b in zip(plain, res))
+    count = len(batch)
+    out = {"workload": "batch_%d_uneven" % count, "plain_solves_per_s": count / t_plain,
+           "batched_solves_per_s": count / t_batch, "ratio": t_plain / t_batch,
+           "batched_graph_ms": dev_ms, "batched_equals_plain_bitwise": bool(same)}
+    del solvers
+    for p in models:
+        p.uninitialize_gpu()
     return out
 
 
@@ -149,12 +246,25 @@ def main():
                     help="sweeps per step (default: the workload's 200; fewer only for profiling)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MB first (counter scaling)")
+    ap.add_argument("--no-pbvi", action="store_true", help="skip the PBVI leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve leg")
+    ap.add_argument("--cpu-beliefs", type=int, default=4)
+    ap.add_argument("--batch-count", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path is single-GPU; see the "
               "sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     out = bench_vi(args)
+    extras = {}
+    if not args.no_pbvi:
+        extras["pbvi"] = bench_pbvi(args)
+        print("pbvi leg done", file=sys.stderr, flush=True)
+    if not args.no_batch:
+        extras["batch"] = bench_batch(args)
+        print("batch leg done", file=sys.stderr, flush=True)
+    if extras:
+        out["extras"] = extras
     print(json.dumps(out))
 
 

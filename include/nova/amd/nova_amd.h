@@ -65,6 +65,32 @@ extern "C" int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps,
         POMDPPBVIGPU *const *pbvis, unsigned int count, POMDPAlphaVectors *policies,
         float *elapsedMs);
 
+// One PBVI iteration with HIP events between its four kernels (transpose, score, action,
+// materialize) on the library stream; stageMs[4] receives their device times in ms.
+// Advances currentHorizon like pomdp_pbvi_update_gpu.
+extern "C" int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageMs);
+
+// ---- PBVI argmax replay (gradients of L = sum_b max_alpha alpha . b with respect to R)
+
+// Runs `iterations` iterations like pomdp_pbvi_run_gpu and records each one's choices:
+// d_recJ[(k*r + i)*z + o] = alpha index chosen for alpha i, observation o at iteration k, and
+// d_recA[k*r + i] = action chosen (-1: none; then d_recJ holds -1). Device arrays of
+// iterations*r*z and iterations*r ints.
+extern "C" int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi,
+        unsigned int iterations, int *d_recJ, int *d_recA);
+
+// d_values[i] = max_j Gamma[j] . b_i over belief i's sparse support (first maximiser from
+// -1e10) and d_arg[i] = that j, for the rGamma alphas of d_Gamma (rGamma x n, device).
+extern "C" int pomdp_pbvi_belief_values_gpu(const POMDP *pomdp, const float *d_Gamma,
+        unsigned int rGamma, float *d_values, int *d_arg);
+
+// dL/dR (n*m floats, device) of L = sum_i max_j Gamma_H[j] . b_i by replaying the records of
+// `iterations` iterations from GammaInitial (which does not depend on R), d_finalArg being the
+// maximisers of the final Gamma (pomdp_pbvi_belief_values_gpu). Uses the host S/T and the
+// device O/Z/B of pomdp.
+extern "C" int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations,
+        const int *d_recJ, const int *d_recA, const int *d_finalArg, float *d_dR);
+
 // Copies the argmax record of the last iteration to the host: jstar[(i*m + a)*z + o] = the
 // alpha chosen for belief i, action a, observation o (r*m*z ints, nullable) and actions[i]
 // = the action chosen for belief i, or -1 when none beat -1e10 (r ints, nullable).

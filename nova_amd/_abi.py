@@ -192,6 +192,11 @@ EXTRA_SIGNATURES = {
     "nova_amd_pbvi_mode": [ct.c_int],
     "pomdp_pbvi_run_gpu": [_POMDP, _PBG, U, _P(ct.c_float)],
     "pomdp_pbvi_argmax_gpu": [_POMDP, _PBG, c_int_p, c_uint_p],
+    "pomdp_pbvi_profile_gpu": [_POMDP, _PBG, _P(ct.c_float)],
+    "pomdp_pbvi_run_recorded_gpu": [_POMDP, _PBG, U, ct.c_void_p, ct.c_void_p],
+    "pomdp_pbvi_belief_values_gpu": [_POMDP, ct.c_void_p, U, ct.c_void_p, ct.c_void_p],
+    "pomdp_pbvi_replay_grad_gpu": [_POMDP, U, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                   ct.c_void_p],
     "pomdp_pbvi_batch_execute_gpu": [ct.c_void_p, ct.c_void_p, U, ct.c_void_p, _P(ct.c_float)],
     "pomdp_expand_random_seeded_gpu": [_POMDP, U, U, _P(ct.c_uint), _P(c_float_p)],
     "nova_amd_free_floats": [c_float_p],

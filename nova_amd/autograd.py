@@ -96,3 +96,73 @@ class BellmanFixedPoint(torch.autograd.Function):
         R, T, S, pi, V64 = ctx.saved_tensors
         _, dR, dT, _, _ = adjoint(R, T, S, ctx.gamma, pi, V64, g)
         return dR, dT, None, None, None
+
+
+# ---------------------------------------------------------------- PBVI argmax replay
+
+def pbvi_objective(P, R, horizon=None, mode=None):
+    """Runs PBVI on the device with recorded choices. Returns a dict with L = sum_b max_alpha
+    alpha . b (float64 of the float32 values), the per-belief values and maximisers, the final
+    Gamma, and the (device) records, plus the POMDP object that holds the device model."""
+    from .pomdp import POMDP
+    from .pomdp_pbvi import POMDPPBVIGPU, set_mode
+    lib = _abi.load()
+    Rn = R.detach().to("cpu", torch.float32).numpy() if torch.is_tensor(R) else np.asarray(R, np.float32)
+    H = int(P["horizon"] if horizon is None else horizon)
+    p = POMDP.from_arrays(P["S"], P["T"], P["O"], Rn, P["Z"], P["B"], P["gamma"], H)
+    p.initialize_gpu()
+    old = set_mode(mode) if mode is not None else None
+    try:
+        s = POMDPPBVIGPU(p, GammaInitial=P["Gamma0"])
+        r, z = p.r, p.z
+        recJ = torch.empty(H * r * z, dtype=torch.int32, device="cuda")
+        recA = torch.empty(H * r, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        rc = lib.pomdp_pbvi_run_recorded_gpu(ct.byref(p), ct.byref(s), H, recJ.data_ptr(),
+                                             recA.data_ptr())
+        if rc != 0:
+            raise RuntimeError("pomdp_pbvi_run_recorded_gpu failed with code %d" % rc)
+        G = torch.from_numpy(s.get_policy().Gamma_array()).cuda()
+    finally:
+        if old is not None:
+            set_mode(old)
+    vals = torch.empty(p.r, dtype=torch.float32, device="cuda")
+    arg = torch.empty(p.r, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    rc = lib.pomdp_pbvi_belief_values_gpu(ct.byref(p), G.data_ptr(), p.r, vals.data_ptr(),
+                                          arg.data_ptr())
+    if rc != 0:
+        raise RuntimeError("pomdp_pbvi_belief_values_gpu failed with code %d" % rc)
+    return dict(L=float(vals.double().sum()), values=vals, arg=arg, Gamma=G, recJ=recJ,
+                recA=recA, pomdp=p, solver=s, horizon=H)
+
+
+def pbvi_replay_grad(res):
+    """dL/dR (n x m float32 CUDA tensor) for a result of pbvi_objective."""
+    lib = _abi.load()
+    p = res["pomdp"]
+    dR = torch.empty((p.n, p.m), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    rc = lib.pomdp_pbvi_replay_grad_gpu(ct.byref(p), res["horizon"], res["recJ"].data_ptr(),
+                                        res["recA"].data_ptr(), res["arg"].data_ptr(),
+                                        dR.data_ptr())
+    if rc != 0:
+        raise RuntimeError("pomdp_pbvi_replay_grad_gpu failed with code %d" % rc)
+    return dR
+
+
+class PBVIBeliefValue(torch.autograd.Function):
+    """L(R) = sum over the belief set of max_alpha alpha . b after `horizon` PBVI iterations
+    from GammaInitial, differentiable in R by argmax replay. P holds the other model arrays
+    (S, T, O, Z, B, gamma, horizon, Gamma0) as numpy arrays."""
+
+    @staticmethod
+    def forward(ctx, R, P):
+        res = pbvi_objective(P, R)
+        ctx.res = res
+        return torch.tensor(res["L"], dtype=torch.float64, device=R.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        dR = pbvi_replay_grad(ctx.res)
+        return (g.to(torch.float32) * dR).to(ctx.res["Gamma"].device), None

@@ -66,6 +66,8 @@ struct PbviInst {
     int *partJ, *jstar;
     float *aval;
     int *act;
+    int *recJ;             // optional per-iteration record: recJ[(iter*r + i)*z + o] = j*
+    int *recA;             // and recA[iter*r + i] = chosen action (-1: none)
 };
 
 namespace {
@@ -464,6 +466,19 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
     }
     if (t == 0)
         P.act[i] = bestA;
+    if (P.recJ != nullptr) {
+        // the argmax checkpoint of this iteration: the chosen action's alpha indices
+        __shared__ int chosenA;
+        if (t == 0) {
+            chosenA = bestA;
+            P.recA[(size_t)iter * r + i] = bestA;
+        }
+        __syncthreads();
+        const int ca = chosenA;
+        for (unsigned o = t; o < z; o += kPB)
+            P.recJ[((size_t)iter * r + i) * z + o] =
+                (ca >= 0) ? P.jstar[((size_t)i * m + ca) * z + o] : -1;
+    }
 }
 
 // --------------------------------------------------------------------- 4. materialize
@@ -853,20 +868,22 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
                      g_pgraph.gamma == p->gamma && g_pgraph.mode == g_pbvi_mode &&
                      g_pgraph.jpt == jpt && std::memcmp(g_pgraph.key, key, sizeof(key)) == 0 &&
                      std::memcmp(g_pgraph.dims, dims, sizeof(dims)) == 0;
+    // The graph's kernels read the descriptor from plan memory when they run, and other calls
+    // (the tuner, profiling, recorded runs) reuse that memory: upload it on every run.
+    PbviPlan pl;
+    pl.count = 1;
+    pl.host.resize(1);
+    fill_inst(p, pl.host[0], pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
+              reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
+    pl.host[0].jpt = jpt;
+    int rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
+    if (rc != NOVA_SUCCESS)
+        return rc;
     if (!hit) {
         if (g_pgraph.exec) {
             hipGraphExecDestroy(g_pgraph.exec);
             g_pgraph.exec = nullptr;
         }
-        PbviPlan pl;
-        pl.count = 1;
-        pl.host.resize(1);
-        fill_inst(p, pl.host[0], pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
-                  reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
-        pl.host[0].jpt = jpt;
-        int rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
-        if (rc != NOVA_SUCCESS)
-            return rc;
         rc = plan_graph(pl, iters, flip, g_pbvi_mode, st, &g_pgraph.exec, fn);
         if (rc != NOVA_SUCCESS)
             return rc;
@@ -1095,6 +1112,103 @@ int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi, int *jst
         e = hipMemcpy(actions, I.act, sizeof(int) * pomdp->r, hipMemcpyDeviceToHost);
     return hip_fail(e, fn, "Failed to copy the argmax record to the host.",
                     NOVA_ERROR_MEMCPY_TO_HOST);
+}
+
+// One plain iteration with HIP events between its four stages (transpose, score, action,
+// materialize); stageMs[4] receives their device times. Advances currentHorizon like
+// pomdp_pbvi_update_gpu.
+int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageMs)
+{
+    const char *fn = "pomdp_pbvi_profile_gpu";
+    if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->d_alphaBA == nullptr ||
+            stageMs == nullptr) {
+        NOVA_ERR(fn, "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    hipStream_t st = stream();
+    PbviPlan pl;
+    pl.count = 1;
+    pl.host.resize(1);
+    fill_inst(pomdp, pl.host[0], pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
+              reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
+    pl.host[0].jpt = plain_jpt(pbvi->d_alphaBA);
+    int rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    const unsigned flip = pbvi->currentHorizon % 2;
+    const int mode = g_pbvi_mode;
+    const unsigned c = 1;
+    const unsigned *tr = pl.dOff, *be = pl.dOff + 4 * (c + 1);
+    hipEvent_t ev[5];
+    for (auto &e : ev)
+        hipEventCreate(&e);
+    hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, 0u, flip);
+    hipEventRecord(ev[1], st);
+    const int v = jpt_slot(pl.host[0].jpt);
+    const unsigned *sc = pl.dOff + (1 + v) * (c + 1);
+    dim3 g(pl.totSc[v]);
+    if (mode == 1) {
+        if (v == 0) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+        if (v == 1) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+        if (v == 2) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+    } else {
+        if (v == 0) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+        if (v == 1) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+        if (v == 2) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+    }
+    hipEventRecord(ev[2], st);
+    hipLaunchKernelGGL(pbvi_action, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
+    hipEventRecord(ev[3], st);
+    auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
+    hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, 0u, flip);
+    hipEventRecord(ev[4], st);
+    hipError_t e = hipStreamSynchronize(st);
+    for (int k = 0; k < 4; k++)
+        hipEventElapsedTime(&stageMs[k], ev[k], ev[k + 1]);
+    for (auto &x : ev)
+        hipEventDestroy(x);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Failed to run the profiled iteration.", NOVA_ERROR_KERNEL_EXECUTION);
+    pbvi->currentHorizon++;
+    return NOVA_SUCCESS;
+}
+
+// Runs `iterations` plain iterations (from the current state) recording each one's argmax
+// choices: d_recJ[(k*r + i)*z + o] and d_recA[k*r + i] for k < iterations (device arrays).
+int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned int iterations,
+        int *d_recJ, int *d_recA)
+{
+    const char *fn = "pomdp_pbvi_run_recorded_gpu";
+    if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->d_alphaBA == nullptr ||
+            iterations == 0 || d_recJ == nullptr || d_recA == nullptr) {
+        NOVA_ERR(fn, "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    hipStream_t st = stream();
+    PbviPlan pl;
+    pl.count = 1;
+    pl.host.resize(1);
+    fill_inst(pomdp, pl.host[0], pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
+              reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
+    pl.host[0].jpt = plain_jpt(pbvi->d_alphaBA);
+    pl.host[0].recJ = d_recJ;
+    pl.host[0].recA = d_recA;
+    int rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    hipGraphExec_t exec = nullptr;
+    rc = plan_graph(pl, iterations, pbvi->currentHorizon % 2, g_pbvi_mode, st, &exec, fn);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    hipError_t e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    hipGraphExecDestroy(exec);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Failed to run the recorded iterations.", NOVA_ERROR_KERNEL_EXECUTION);
+    pbvi->currentHorizon += iterations;
+    return NOVA_SUCCESS;
 }
 
 // Batched PBVI: every POMDP must be resident on the device (pomdp_initialize_gpu) and each

@@ -97,6 +97,13 @@ class POMDPPBVIGPU(_abi.NovaPOMDPPBVIGPU):
             raise Exception("pomdp_pbvi_run_gpu failed with code %d" % rc)
         return ms.value
 
+    def profile(self):
+        """One iteration; returns the device ms of its (transpose, score, action, materialize)."""
+        ms = (ct.c_float * 4)()
+        if self._lib.pomdp_pbvi_profile_gpu(self.pomdpPtr, self, ms) != 0:
+            raise Exception("pomdp_pbvi_profile_gpu failed")
+        return list(ms)
+
     def get_policy(self):
         policy = POMDPAlphaVectors()
         if self._lib.pomdp_pbvi_get_policy_gpu(self.pomdpPtr, self, policy) != 0:

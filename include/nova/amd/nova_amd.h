@@ -43,10 +43,12 @@ extern "C" int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi,
 
 // ---- PBVI (include/nova/pomdp/algorithms/pomdp_pbvi_gpu.h)
 
-// Selects the arithmetic of the GPU PBVI backup for later calls in this process and returns
-// the previous mode: 0 = fast (default; FMA accumulation, results within float rounding of
-// nova's), 1 = exact (every product and sum rounded in nova's order: Gamma and pi equal
-// nova's CPU results bit for bit). Other values only query.
+// Selects the arithmetic of the GPU PBVI backup for solvers initialized afterwards in this
+// process and returns the previous mode: 0 = fast (default; FMA accumulation, results within
+// float rounding of nova's), 1 = exact (every product and sum rounded in nova's order: Gamma
+// and pi equal nova's CPU results bit for bit), 2 = dense beliefs scored on the matrix cores
+// in bf16 with fp32 accumulation (any support length; n <= 36864; host S/T required),
+// 3 = the same with fp32 matrix-core arithmetic. Other values only query.
 extern "C" int nova_amd_pbvi_mode(int mode);
 
 // Runs `iterations` PBVI iterations on the device continuing from pbvi's state (the

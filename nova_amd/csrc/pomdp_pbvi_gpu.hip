@@ -554,10 +554,278 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
         P.pi[i] = (unsigned)a;
 }
 
+// ------------------------------------------------------------ dense beliefs (MFMA)
+//
+// With dense beliefs (support = all n states) the scoring of observation o under action a is
+//   C[i][j] = sum_s' U_a[i][s'] * (gamma O[a,s',o]) * Gamma[j][s'],   U_a = B T_a (r x n),
+// a GEMM per (a, o) of r x n by n x r. U_a does not depend on Gamma and is built once per
+// initialize. The kernel stages 128 x 32 tiles of U_a and of Gamma (the latter scaled by
+// gamma O[a,.,o] as it is staged) in LDS and accumulates 128 x 128 tiles of C on the matrix
+// cores: bf16 operands with fp32 accumulation (v_mfma_f32_32x32x16_bf16), or fp32 operands
+// (v_mfma_f32_32x32x2_f32) for the full-precision comparison. The epilogue reduces each row to
+// its maximum over the tile's 128 alphas (first maximiser), as the sparse kernel's partials.
+
+constexpr int kDT = 128;            // tile rows (beliefs) and columns (alphas)
+constexpr int kDK = 32;             // k (state) step per LDS tile
+constexpr int kDPad = 8;            // LDS row padding (elements)
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+template <typename IN> struct DenseIn;
+template <> struct DenseIn<__bf16> {
+    static __device__ __forceinline__ __bf16 cvt(float x) { return (__bf16)x; }
+};
+template <> struct DenseIn<float> {
+    static __device__ __forceinline__ float cvt(float x) { return x; }
+};
+
+// Gamma (float, r x n) -> GI (IN, r x n)
+template <typename IN>
+__global__ __launch_bounds__(kPB) void dense_convert(size_t count, const float *__restrict__ G,
+                                                     IN *__restrict__ GI)
+{
+    for (size_t x = blockIdx.x * (size_t)kPB + threadIdx.x; x < count; x += (size_t)gridDim.x * kPB)
+        GI[x] = DenseIn<IN>::cvt(G[x]);
+}
+
+// U[a][i][s'] = sum over predecessors (s, l) of s' under a of T(s,a,l) b_i(s), with the dense
+// belief row b_i staged in LDS from its sparse (Z, B) representation.
+template <typename IN>
+__global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, unsigned r,
+        const int *__restrict__ Z, const float *__restrict__ B,
+        const unsigned *__restrict__ pstart, const int *__restrict__ pS,
+        const float *__restrict__ pT, IN *__restrict__ U)
+{
+    extern __shared__ float brow[];
+    const unsigned a = blockIdx.y, i = blockIdx.x, t = threadIdx.x;
+    for (unsigned s = t; s < n; s += kPB)
+        brow[s] = 0.0f;
+    __syncthreads();
+    // the valid prefix of the support holds distinct states
+    __shared__ int kEnd;
+    if (t == 0)
+        kEnd = (int)rz;
+    __syncthreads();
+    for (unsigned k = t; k < rz; k += kPB)
+        if (Z[(size_t)i * rz + k] < 0)
+            atomicMin(&kEnd, (int)k);
+    __syncthreads();
+    for (int k = t; k < kEnd; k += kPB)
+        brow[Z[(size_t)i * rz + k]] = B[(size_t)i * rz + k];
+    __syncthreads();
+    const unsigned *ps = pstart + (size_t)a * (n + 1);
+    IN *out = U + ((size_t)a * r + i) * n;
+    for (unsigned sp = t; sp < n; sp += kPB) {
+        float acc = 0.0f;
+        for (unsigned q = ps[sp]; q < ps[sp + 1]; q++)
+            acc += pT[q] * brow[pS[q]];
+        out[sp] = DenseIn<IN>::cvt(acc);
+    }
+}
+
+// Os[(a*z + o)*n + s'] = gamma O[a,s',o]; Rb[i*m + a] = sum_k B[i,k] R[Z[i,k], a]
+__global__ __launch_bounds__(kPB) void dense_build_scales(unsigned n, unsigned m, unsigned z,
+        unsigned r, unsigned rz, float gamma, const float *__restrict__ O,
+        const float *__restrict__ R, const int *__restrict__ Z, const float *__restrict__ B,
+        float *__restrict__ Os, float *__restrict__ Rb)
+{
+    const size_t total = (size_t)m * z * n;
+    for (size_t x = blockIdx.x * (size_t)kPB + threadIdx.x; x < total; x += (size_t)gridDim.x * kPB) {
+        const unsigned sp = x % n;
+        const unsigned ao = x / n, a = ao / z, o = ao % z;
+        Os[x] = gamma * O[((size_t)a * n + sp) * z + o];
+    }
+    for (size_t x = blockIdx.x * (size_t)kPB + threadIdx.x; x < (size_t)r * m; x += (size_t)gridDim.x * kPB) {
+        const unsigned i = x / m, a = x % m;
+        float v = 0.0f;
+        for (unsigned k = 0; k < rz; k++) {
+            const int s = Z[(size_t)i * rz + k];
+            if (s < 0)
+                break;
+            v += B[(size_t)i * rz + k] * R[(size_t)s * m + a];
+        }
+        Rb[x] = v;
+    }
+}
+
+template <typename IN>
+__global__ __launch_bounds__(kPB) void dense_score(unsigned n, unsigned r, unsigned m,
+        unsigned z, unsigned itiles, const IN *__restrict__ U, const IN *__restrict__ GI,
+        const float *__restrict__ Os, float *__restrict__ partVal, int *__restrict__ partJ)
+{
+    __shared__ __attribute__((aligned(16))) IN As[kDT][kDK + kDPad];
+    __shared__ __attribute__((aligned(16))) IN Bs[kDT][kDK + kDPad];
+    __shared__ float redV[2][kDT];
+    __shared__ int redJ[2][kDT];
+
+    const unsigned ao = blockIdx.y, a = ao / z;
+    const unsigned itile = blockIdx.x % itiles, jtile = blockIdx.x / itiles;
+    const unsigned i0 = itile * kDT, j0 = jtile * kDT;
+    const unsigned t = threadIdx.x, l = t % 64, w = t / 64;
+    const unsigned wr = w / 2, wc = w % 2;
+    const IN *Ua = U + (size_t)a * r * n;
+    const float *s = Os + (size_t)ao * n;
+
+    floatx16_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int v = 0; v < 16; v++)
+                acc[x][y][v] = 0.0f;
+
+    // staging map: each lane moves 16 consecutive k of one row (two lanes per row)
+    const unsigned srow = t / 2, skh = (t % 2) * 16;
+    for (unsigned k0 = 0; k0 < n; k0 += kDK) {
+        {
+            const unsigned gi = i0 + srow, gj = j0 + srow;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const unsigned k = k0 + skh + e;
+                const bool kin = k < n;
+                As[srow][skh + e] = (gi < r && kin) ? Ua[(size_t)gi * n + k] : DenseIn<IN>::cvt(0.0f);
+                const float g = (gj < r && kin) ? (float)GI[(size_t)gj * n + k] * s[k] : 0.0f;
+                Bs[srow][skh + e] = DenseIn<IN>::cvt(g);
+            }
+        }
+        __syncthreads();
+        if constexpr (sizeof(IN) == 2) {
+#pragma unroll
+            for (int kk = 0; kk < kDK; kk += 16) {
+                bf16x8_t fa[2], fb[2];
+#pragma unroll
+                for (int x = 0; x < 2; x++) {
+                    const unsigned row = wr * 64 + x * 32 + l % 32;
+                    fa[x] = *reinterpret_cast<const bf16x8_t *>(&As[row][kk + 8 * (l / 32)]);
+                    const unsigned col = wc * 64 + x * 32 + l % 32;
+                    fb[x] = *reinterpret_cast<const bf16x8_t *>(&Bs[col][kk + 8 * (l / 32)]);
+                }
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[x], fb[y], acc[x][y], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < kDK; kk += 2) {
+                float fa[2], fb[2];
+#pragma unroll
+                for (int x = 0; x < 2; x++) {
+                    fa[x] = (float)As[wr * 64 + x * 32 + l % 32][kk + l / 32];
+                    fb[x] = (float)Bs[wc * 64 + x * 32 + l % 32][kk + l / 32];
+                }
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[x], fb[y], acc[x][y], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: per row, the first maximiser over this tile's alphas
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            float bv = kLow;
+            int bj = INT_MAX;
+#pragma unroll
+            for (int y = 0; y < 2; y++) {
+                const unsigned j = j0 + wc * 64 + y * 32 + l % 32;
+                const float val = acc[x][y][v];
+                if (j < r && (val > bv || (val == bv && (int)j < bj))) {
+                    bv = val;
+                    bj = (int)j;
+                }
+            }
+            for (int off = 16; off >= 1; off >>= 1) {
+                const float ov = __shfl_xor(bv, off, 32);
+                const int oj = __shfl_xor(bj, off, 32);
+                if (better(ov, oj, bv, bj)) {
+                    bv = ov;
+                    bj = oj;
+                }
+            }
+            if (l % 32 == 0) {
+                const unsigned row = wr * 64 + x * 32 + (v % 4) + 4 * (l / 32) + 8 * (v / 4);
+                redV[wc][row] = bv;
+                redJ[wc][row] = bj;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < kDT) {
+        const unsigned i = i0 + t;
+        float v0 = redV[0][t];
+        int j0v = redJ[0][t];
+        if (better(redV[1][t], redJ[1][t], v0, j0v)) {
+            v0 = redV[1][t];
+            j0v = redJ[1][t];
+        }
+        if (i < r) {
+            const unsigned o = ao % z;
+            const size_t idx = ((size_t)jtile * r * m + (size_t)i * m + a) * z + o;
+            partVal[idx] = v0;
+            partJ[idx] = j0v;
+        }
+    }
+}
+
+// Dense action choice: merges the alpha tiles per (a, o) (earlier tile wins ties), and picks
+// the first action maximising Rb[i,a] + sum_o best score (the scores are b . alpha terms).
+__global__ __launch_bounds__(kPB) void dense_action(unsigned m, unsigned z, unsigned r,
+        unsigned tiles, const float *__restrict__ Rb, const float *__restrict__ partVal,
+        const int *__restrict__ partJ, int *__restrict__ jstar, int *__restrict__ act)
+{
+    __shared__ float best[kZC * 8];
+    const unsigned i = blockIdx.x, t = threadIdx.x;
+    float bestValue = kLow;
+    int bestA = -1;
+    for (unsigned a = 0; a < m; a++) {
+        const size_t ia = (size_t)i * m + a;
+        float total = Rb[ia];
+        for (unsigned o0 = 0; o0 < z; o0 += kZC * 8) {
+            const unsigned oc = min((unsigned)(kZC * 8), z - o0);
+            for (unsigned oo = t; oo < oc; oo += kPB) {
+                float v = kLow;
+                int j = INT_MAX;
+                for (unsigned tl = 0; tl < tiles; tl++) {
+                    const size_t idx = ((size_t)tl * r * m + ia) * z + o0 + oo;
+                    if (partVal[idx] > v) {
+                        v = partVal[idx];
+                        j = partJ[idx];
+                    }
+                }
+                best[oo] = v;
+                jstar[ia * z + o0 + oo] = (j == INT_MAX) ? 0 : j;
+            }
+            __syncthreads();
+            if (t == 0)
+                for (unsigned oo = 0; oo < oc; oo++)
+                    total += best[oo];
+            __syncthreads();
+        }
+        if (t == 0 && total > bestValue) {
+            bestValue = total;
+            bestA = (int)a;
+        }
+    }
+    if (t == 0)
+        act[i] = bestA;
+}
+
 } // namespace
 
-// Scoring and materialisation arithmetic: 0 = fast (default), 1 = exact reference order.
+// Backup arithmetic: 0 = fast (default), 1 = exact reference order, 2 = dense beliefs with
+// bf16 matrix-core scoring, 3 = dense beliefs with fp32 matrix-core scoring.
 static int g_pbvi_mode = 0;
+
+static bool dense_mode(int mode) { return mode == 2 || mode == 3; }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -635,7 +903,10 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
 }
 
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
+struct DenseWs;
+
 struct PbviPlan {
+    const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
     unsigned count = 0;
     PbviInst *dInst = nullptr;       // device copy of the descriptors
     unsigned *dOff = nullptr;        // [tr | sc(jpt=1) | sc(2) | sc(4) | be], count+1 each
@@ -691,10 +962,76 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
     return hip_fail(e, fn, "Failed to upload the PBVI launch plan.", NOVA_ERROR_MEMCPY_TO_DEVICE);
 }
 
+// The extra device buffers of a dense-belief solve (modes 2 and 3), per solver workspace.
+struct DenseWs {
+    const void *key = nullptr;       // POMDPPBVIGPU::d_alphaBA it belongs to
+    int mode = 0;
+    void *mem = nullptr;
+    void *U = nullptr;               // [m][r][n] in the scoring precision
+    void *GI = nullptr;              // [r][n] Gamma in the scoring precision
+    float *Os = nullptr;             // [m][z][n] gamma*O
+    float *Rb = nullptr;             // [r][m] b . R[:,a]
+    float *partVal = nullptr;        // [tiles][r][m][z]
+    int *partJ = nullptr;
+    unsigned tiles = 0, n = 0, m = 0, z = 0, r = 0;
+};
+static std::vector<DenseWs> g_dense;
+
+static const DenseWs *dense_for(const void *key)
+{
+    for (auto &d : g_dense)
+        if (d.key == key)
+            return &d;
+    return nullptr;
+}
+
+static void dense_release(const void *key, const char *fn)
+{
+    for (size_t x = 0; x < g_dense.size(); x++)
+        if (g_dense[x].key == key) {
+            device_free(g_dense[x].mem, fn, "the dense-belief workspace");
+            g_dense.erase(g_dense.begin() + x);
+            return;
+        }
+}
+
+static void dense_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, hipStream_t st)
+{
+    const DenseWs &D = *pl.dense;
+    const PbviInst &I = pl.host[0];
+    const float *G = ((iter + flip) % 2 == 0) ? I.G0 : I.G1;
+    const size_t rn = (size_t)D.r * D.n;
+    const unsigned cg = (unsigned)std::min<size_t>((rn + kPB - 1) / kPB, 65536);
+    const unsigned itiles = (D.r + kDT - 1) / kDT;
+    const dim3 sg(itiles * D.tiles, D.m * D.z);
+    if (D.mode == 2) {
+        hipLaunchKernelGGL(dense_convert<__bf16>, dim3(cg), dim3(kPB), 0, st, rn, G,
+                           static_cast<__bf16 *>(D.GI));
+        hipLaunchKernelGGL(dense_score<__bf16>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z, itiles,
+                           static_cast<const __bf16 *>(D.U), static_cast<const __bf16 *>(D.GI),
+                           D.Os, D.partVal, D.partJ);
+    } else {
+        hipLaunchKernelGGL(dense_convert<float>, dim3(cg), dim3(kPB), 0, st, rn, G,
+                           static_cast<float *>(D.GI));
+        hipLaunchKernelGGL(dense_score<float>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z, itiles,
+                           static_cast<const float *>(D.U), static_cast<const float *>(D.GI),
+                           D.Os, D.partVal, D.partJ);
+    }
+    hipLaunchKernelGGL(dense_action, dim3(D.r), dim3(kPB), 0, st, D.m, D.z, D.r, D.tiles, D.Rb,
+                       D.partVal, D.partJ, I.jstar, I.act);
+    const unsigned *be = pl.dOff + 4 * (pl.count + 1);
+    hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st,
+                       pl.dInst, be, pl.count, iter, flip);
+}
+
 // Enqueues iteration `iter` of every instance of the plan.
 static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int mode,
                          hipStream_t st)
 {
+    if (pl.dense) {
+        dense_enqueue(pl, iter, flip, st);
+        return;
+    }
     const unsigned c = pl.count;
     const unsigned *tr = pl.dOff, *be = pl.dOff + 4 * (c + 1);
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
@@ -821,6 +1158,126 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     return best;
 }
 
+// Builds the dense-belief workspace of a plain solver: U_a = B T_a for every action (belief
+// rows staged in LDS, predecessor lists from the host S/T), gamma*O planes and b . R.
+static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStream_t st,
+                       const char *fn)
+{
+    dense_release(key, fn);
+    const unsigned n = p->n, m = p->m, z = p->z, r = p->r, ns = p->ns;
+    if (p->S == nullptr || p->T == nullptr || n > (unsigned)kLdsRowFloats) {
+        NOVA_ERR(fn, "Dense-belief scoring needs the host S/T arrays and n <= 36864.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const size_t esz = (mode == 2) ? 2 : 4;
+    DenseWs D;
+    D.key = key;
+    D.mode = mode;
+    D.n = n; D.m = m; D.z = z; D.r = r;
+    D.tiles = (r + kDT - 1) / kDT;
+    const size_t rmz = (size_t)r * m * z;
+    const size_t bU = align256(esz * (size_t)m * r * n), bG = align256(esz * (size_t)r * n),
+                 bO = align256(sizeof(float) * (size_t)m * z * n),
+                 bR = align256(sizeof(float) * (size_t)r * m),
+                 bP = align256(sizeof(float) * D.tiles * rmz);
+    int rc = device_alloc(&D.mem, bU + bG + bO + bR + 2 * bP, fn, "the dense-belief workspace");
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    char *c = static_cast<char *>(D.mem);
+    D.U = c; c += bU;
+    D.GI = c; c += bG;
+    D.Os = reinterpret_cast<float *>(c); c += bO;
+    D.Rb = reinterpret_cast<float *>(c); c += bR;
+    D.partVal = reinterpret_cast<float *>(c); c += bP;
+    D.partJ = reinterpret_cast<int *>(c);
+    g_dense.push_back(D);
+
+    // predecessor lists of every action (host), then the device builds
+    std::vector<unsigned> pstart((size_t)m * (n + 1), 0);
+    for (unsigned s2 = 0; s2 < n; s2++)
+        for (unsigned a = 0; a < m; a++)
+            for (unsigned l = 0; l < ns; l++) {
+                const int sp = p->S[idx3(s2, a, l, m, ns)];
+                if (sp < 0)
+                    break;
+                pstart[(size_t)a * (n + 1) + sp + 1]++;
+            }
+    size_t total = 0;
+    for (unsigned a = 0; a < m; a++) {
+        unsigned *row = &pstart[(size_t)a * (n + 1)];
+        for (unsigned sp = 0; sp < n; sp++)
+            row[sp + 1] += row[sp];
+        for (unsigned sp = 0; sp <= n; sp++)
+            row[sp] += (unsigned)total;
+        total = row[n];
+    }
+    std::vector<int> pS(std::max<size_t>(total, 1));
+    std::vector<float> pT(std::max<size_t>(total, 1));
+    {
+        std::vector<unsigned> fillPos(pstart);
+        for (unsigned s2 = 0; s2 < n; s2++)
+            for (unsigned a = 0; a < m; a++)
+                for (unsigned l = 0; l < ns; l++) {
+                    const size_t x = idx3(s2, a, l, m, ns);
+                    const int sp = p->S[x];
+                    if (sp < 0)
+                        break;
+                    const unsigned q = fillPos[(size_t)a * (n + 1) + sp]++;
+                    pS[q] = (int)s2;
+                    pT[q] = p->T[x];
+                }
+    }
+    void *dps = nullptr, *dpS = nullptr, *dpT = nullptr;
+    rc = device_alloc(&dps, sizeof(unsigned) * pstart.size(), fn, "predecessor lists");
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc(&dpS, sizeof(int) * pS.size(), fn, "predecessor lists");
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc(&dpT, sizeof(float) * pT.size(), fn, "predecessor lists");
+    hipError_t e = hipSuccess;
+    if (rc == NOVA_SUCCESS) {
+        e = hipMemcpy(dps, pstart.data(), sizeof(unsigned) * pstart.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(dpS, pS.data(), sizeof(int) * pS.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(dpT, pT.data(), sizeof(float) * pT.size(), hipMemcpyHostToDevice);
+    }
+    if (rc == NOVA_SUCCESS && e == hipSuccess) {
+        const int ldsBytes = (int)(sizeof(float) * n);
+        const size_t lds = sizeof(float) * n;
+        if (mode == 2) {
+            hipFuncSetAttribute((const void *)dense_build_u<__bf16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, ldsBytes);
+            hipLaunchKernelGGL(dense_build_u<__bf16>, dim3(r, m), dim3(kPB), lds, st, n, p->rz, r,
+                               p->d_Z, p->d_B, static_cast<unsigned *>(dps),
+                               static_cast<int *>(dpS), static_cast<float *>(dpT),
+                               static_cast<__bf16 *>(g_dense.back().U));
+        } else {
+            hipFuncSetAttribute((const void *)dense_build_u<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, ldsBytes);
+            hipLaunchKernelGGL(dense_build_u<float>, dim3(r, m), dim3(kPB), lds, st, n, p->rz, r,
+                               p->d_Z, p->d_B, static_cast<unsigned *>(dps),
+                               static_cast<int *>(dpS), static_cast<float *>(dpT),
+                               static_cast<float *>(g_dense.back().U));
+        }
+        const size_t tot = std::max((size_t)m * z * n, (size_t)r * m);
+        const unsigned grid = (unsigned)std::min<size_t>((tot + kPB - 1) / kPB, 65536);
+        hipLaunchKernelGGL(dense_build_scales, dim3(grid), dim3(kPB), 0, st, n, m, z, r, p->rz,
+                           p->gamma, p->d_O, p->d_R, p->d_Z, p->d_B, g_dense.back().Os,
+                           g_dense.back().Rb);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+    }
+    if (dps) device_free(dps, fn, "predecessor lists");
+    if (dpS) device_free(dpS, fn, "predecessor lists");
+    if (dpT) device_free(dpT, fn, "predecessor lists");
+    if (rc == NOVA_SUCCESS && e != hipSuccess)
+        rc = hip_fail(e, fn, "Failed to build the dense-belief workspace.", NOVA_ERROR_KERNEL_EXECUTION);
+    if (rc != NOVA_SUCCESS)
+        dense_release(key, fn);
+    return rc;
+}
+
 static bool pomdp_device_ready(const nova::POMDP *p)
 {
     return p != nullptr && p->n != 0 && p->ns != 0 && p->m != 0 && p->z != 0 && p->r != 0 &&
@@ -876,6 +1333,9 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
     fill_inst(p, pl.host[0], pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
               reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
     pl.host[0].jpt = jpt;
+    pl.dense = dense_for(pb->d_alphaBA);
+    if (pl.dense)
+        pl.host[0].tiles = pl.dense->tiles;
     int rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
@@ -926,7 +1386,7 @@ using namespace nova_amd;
 int nova_amd_pbvi_mode(int mode)
 {
     const int old = g_pbvi_mode;
-    if (mode == 0 || mode == 1)
+    if (mode >= 0 && mode <= 3)
         g_pbvi_mode = mode;
     return old;
 }
@@ -939,11 +1399,13 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
                      "GammaInitial set).");
         return NOVA_ERROR_INVALID_DATA;
     }
-    if (!pbvi_shape_ok(pomdp)) {
-        NOVA_ERR(fn, "Belief supports too long for this implementation (rz <= 512 and "
-                     "rz*ns <= 512 required).");
+    if (!dense_mode(g_pbvi_mode) && !pbvi_shape_ok(pomdp)) {
+        NOVA_ERR(fn, "Belief supports too long for the sparse backup (rz <= 512 and "
+                     "rz*ns <= 512 required); dense beliefs use modes 2/3.");
         return NOVA_ERROR_INVALID_DATA;
     }
+    if (pbvi->d_alphaBA != nullptr)
+        dense_release(pbvi->d_alphaBA, fn);
     device_free_if_owned(pbvi->d_Gamma, fn, "Gamma");
     device_free_if_owned(pbvi->d_GammaPrime, fn, "Gamma (prime)");
     device_free_if_owned(pbvi->d_pi, fn, "pi");
@@ -982,6 +1444,8 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     rc = build_ot(pomdp, I.OT, st);
     if (rc != NOVA_SUCCESS)
         return hip_fail(hipGetLastError(), fn, "Failed to build the O*T products.", rc);
+    if (dense_mode(g_pbvi_mode))
+        return dense_setup(pomdp, pbvi->d_alphaBA, g_pbvi_mode, st, fn);
     const unsigned jpt = tune_jpt(I, plain_plan_mem(pomdp, pbvi), st, g_pbvi_mode);
     bool found = false;
     for (auto &kv : g_plain_jpt)
@@ -1009,6 +1473,8 @@ int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
         hipGraphExecDestroy(g_pgraph.exec);
         g_pgraph.exec = nullptr;
     }
+    if (pbvi->d_alphaBA != nullptr)
+        dense_release(pbvi->d_alphaBA, fn);
     if (device_free_if_owned(pbvi->d_Gamma, fn, "Gamma") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(pbvi->d_GammaPrime, fn, "Gamma (prime)") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(pbvi->d_pi, fn, "pi") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
@@ -1142,6 +1608,48 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
     hipEvent_t ev[5];
     for (auto &e : ev)
         hipEventCreate(&e);
+    if (const DenseWs *Dp = dense_for(pbvi->d_alphaBA)) {
+        // dense beliefs: convert, score (matrix cores), action, materialize
+        const DenseWs &D = *Dp;
+        const PbviInst &I = pl.host[0];
+        const float *G = (flip == 0) ? I.G0 : I.G1;
+        const size_t rn = (size_t)D.r * D.n;
+        const unsigned cg = (unsigned)std::min<size_t>((rn + kPB - 1) / kPB, 65536);
+        const unsigned itiles = (D.r + kDT - 1) / kDT;
+        const dim3 sg(itiles * D.tiles, D.m * D.z);
+        hipEventRecord(ev[0], st);
+        if (D.mode == 2)
+            hipLaunchKernelGGL(dense_convert<__bf16>, dim3(cg), dim3(kPB), 0, st, rn, G,
+                               static_cast<__bf16 *>(D.GI));
+        else
+            hipLaunchKernelGGL(dense_convert<float>, dim3(cg), dim3(kPB), 0, st, rn, G,
+                               static_cast<float *>(D.GI));
+        hipEventRecord(ev[1], st);
+        if (D.mode == 2)
+            hipLaunchKernelGGL(dense_score<__bf16>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z,
+                               itiles, static_cast<const __bf16 *>(D.U),
+                               static_cast<const __bf16 *>(D.GI), D.Os, D.partVal, D.partJ);
+        else
+            hipLaunchKernelGGL(dense_score<float>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z,
+                               itiles, static_cast<const float *>(D.U),
+                               static_cast<const float *>(D.GI), D.Os, D.partVal, D.partJ);
+        hipEventRecord(ev[2], st);
+        hipLaunchKernelGGL(dense_action, dim3(D.r), dim3(kPB), 0, st, D.m, D.z, D.r, D.tiles,
+                           D.Rb, D.partVal, D.partJ, I.jstar, I.act);
+        hipEventRecord(ev[3], st);
+        hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st,
+                           pl.dInst, be, c, 0u, flip);
+        hipEventRecord(ev[4], st);
+        hipError_t e = hipStreamSynchronize(st);
+        for (int k = 0; k < 4; k++)
+            hipEventElapsedTime(&stageMs[k], ev[k], ev[k + 1]);
+        for (auto &x : ev)
+            hipEventDestroy(x);
+        if (e != hipSuccess)
+            return hip_fail(e, fn, "Failed to run the profiled iteration.", NOVA_ERROR_KERNEL_EXECUTION);
+        pbvi->currentHorizon++;
+        return NOVA_SUCCESS;
+    }
     hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, 0u, flip);
     hipEventRecord(ev[1], st);
@@ -1181,8 +1689,9 @@ int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned
 {
     const char *fn = "pomdp_pbvi_run_recorded_gpu";
     if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->d_alphaBA == nullptr ||
-            iterations == 0 || d_recJ == nullptr || d_recA == nullptr) {
-        NOVA_ERR(fn, "Invalid arguments.");
+            iterations == 0 || d_recJ == nullptr || d_recA == nullptr ||
+            dense_for(pbvi->d_alphaBA) != nullptr) {
+        NOVA_ERR(fn, "Invalid arguments (recording needs a sparse-belief solver).");
         return NOVA_ERROR_INVALID_DATA;
     }
     hipStream_t st = stream();
@@ -1218,8 +1727,8 @@ int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps, POMDPPBVIGPU *const
         unsigned int count, POMDPAlphaVectors *policies, float *elapsedMs)
 {
     const char *fn = "pomdp_pbvi_batch_execute_gpu";
-    if (pomdps == nullptr || pbvis == nullptr || count == 0) {
-        NOVA_ERR(fn, "Invalid arguments.");
+    if (pomdps == nullptr || pbvis == nullptr || count == 0 || dense_mode(g_pbvi_mode)) {
+        NOVA_ERR(fn, "Invalid arguments (batches use the sparse modes 0/1).");
         return NOVA_ERROR_INVALID_DATA;
     }
     for (unsigned b = 0; b < count; b++) {

@@ -106,6 +106,11 @@ extern "C" int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbv
 extern "C" int pomdp_expand_random_seeded_gpu(const POMDP *pomdp, unsigned int seed,
         unsigned int numDesiredBeliefPoints, unsigned int *maxNonZeroValues, float **Bnew);
 
+// pomdp_expand_distinct_beliefs_cpu with the scoring of every (belief, action, observation)
+// against every belief on the GPU; the appended beliefs equal the host function's bit for bit
+// (src/pomdp/utilities/pomdp_expand_cpu.cpp:166-266). Needs rz*ns <= 1024.
+extern "C" int pomdp_expand_distinct_beliefs_gpu(POMDP *pomdp);
+
 // delete[] of a float array the library allocated for the caller (Bnew of the expansions,
 // bp of pomdp_belief_update_cpu), for callers that cannot call delete[] themselves.
 extern "C" int nova_amd_free_floats(float *p);

@@ -200,6 +200,7 @@ EXTRA_SIGNATURES = {
     "pomdp_pbvi_batch_execute_gpu": [ct.c_void_p, ct.c_void_p, U, ct.c_void_p, _P(ct.c_float)],
     "pomdp_expand_random_seeded_gpu": [_POMDP, U, U, _P(ct.c_uint), _P(c_float_p)],
     "nova_amd_free_floats": [c_float_p],
+    "pomdp_expand_distinct_beliefs_gpu": [_POMDP],
     "mdp_vi_solve_gpu": [_MDP, _VIG, ct.c_void_p, ct.c_double, U, U, _P(ct.c_uint),
                          _P(ct.c_double)],
     "mdp_vi_adjoint_gpu": [_MDP, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,

@@ -87,3 +87,19 @@ def test_unseeded_expansion_api(lib):
     assert np.array_equal(got[0], b0)
     assert np.allclose(got.sum(axis=1), 1.0, atol=1e-5)
     assert lib.pomdp_expand_random_gpu(ct.byref(p), 256, 1, ct.byref(maxnz), ct.byref(Bp)) == 1
+
+
+@pytest.mark.parametrize("n,m,z,ns,r,rz,seed", [(20, 3, 3, 4, 6, 5, 8), (60, 4, 5, 6, 40, 9, 9)])
+def test_distinct_beliefs_gpu_bitwise(reflib, lib, n, m, z, ns, r, rz, seed):
+    """pomdp_expand_distinct_beliefs_gpu appends exactly the beliefs nova's host function does."""
+    P = synth.pomdp_arrays(n=n, m=m, z=z, ns=ns, r=r, rz=rz, seed=seed)
+    P["S"] = P["S"].copy()
+    P["S"][1, 0, 1] = -1          # a ragged successor list
+    P["Z"] = P["Z"].copy()
+    P["Z"][2, 3] = -1             # a ragged support
+    p = POMDP.from_dict(P)
+    assert lib.pomdp_expand_distinct_beliefs_gpu(ct.byref(p)) == 0
+    Z, B = p.beliefs()
+    Z2, B2 = reflib.expand_distinct(P)
+    assert np.array_equal(Z, Z2)
+    assert np.array_equal(B.view(np.uint32), B2.view(np.uint32))

@@ -7,6 +7,7 @@
 #include <nova/mdp/algorithms/mdp_vi_gpu.h>
 #include <nova/pomdp/pomdp.h>
 #include <nova/pomdp/algorithms/pomdp_pbvi_gpu.h>
+#include <nova/pomdp/algorithms/pomdp_perseus_cpu.h>
 
 namespace nova {
 
@@ -98,6 +99,15 @@ extern "C" int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int itera
 // = the action chosen for belief i, or -1 when none beat -1e10 (r ints, nullable).
 extern "C" int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi,
         int *jstar, unsigned int *actions);
+
+// ---- Perseus (include/nova/pomdp/algorithms/pomdp_perseus_cpu.h)
+
+// pomdp_perseus_execute_cpu with its backups and value updates on the GPU: same arguments,
+// same randomized loop (beliefs drawn with libc rand() by nova's formula), same result
+// (bit for bit in PBVI exact mode under the same srand seed; within float rounding otherwise).
+// The POMDP must be resident on the device; per is left uninitialized on return.
+extern "C" int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per,
+        POMDPAlphaVectors *policy);
 
 // ---- belief expansion (include/nova/pomdp/utilities/pomdp_expand_gpu.h)
 

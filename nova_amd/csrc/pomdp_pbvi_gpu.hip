@@ -52,6 +52,7 @@ namespace nova_amd {
 // One problem of a (possibly single-element) batch, as the kernels see it.
 struct PbviInst {
     unsigned n, ns, m, z, r, rz, tiles, horizon;
+    unsigned rb;           // beliefs backed up (PBVI: r; Perseus: 1). r counts the alphas.
     unsigned jpt;          // alphas per lane of the scoring variant chosen for this instance
     int useLds;            // materialisation stages n floats in LDS
     float gamma;
@@ -245,8 +246,9 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
     const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r;
     const float gamma = P.gamma;
     const unsigned local = blockIdx.x - off[b];
-    const unsigned ia = local % (r * m);         // belief * m + action
-    const unsigned tile = local / (r * m);
+    const unsigned rb = P.rb;
+    const unsigned ia = local % (rb * m);        // belief * m + action
+    const unsigned tile = local / (rb * m);
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
                     v = redV[wv][t];
                     j = redJ[wv][t];
                 }
-            const size_t idx = ((size_t)tile * r * m + ia) * z + o0 + t;
+            const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
             P.partVal[idx] = v;
             P.partJ[idx] = j;
         }
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
                 float v = kLow;
                 int j = INT_MAX;
                 for (unsigned tl = 0; tl < P.tiles; tl++) {
-                    const size_t idx = ((size_t)tl * r * m + ia) * z + o0 + oo;
+                    const size_t idx = ((size_t)tl * P.rb * m + ia) * z + o0 + oo;
                     if (P.partVal[idx] > v) {
                         v = P.partVal[idx];
                         j = P.partJ[idx];
@@ -892,6 +894,7 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
 {
     std::memset(&I, 0, sizeof(I));
     I.n = p->n; I.ns = p->ns; I.m = p->m; I.z = p->z; I.r = p->r; I.rz = p->rz;
+    I.rb = p->r;
     I.horizon = horizon;
     I.gamma = p->gamma;
     I.jpt = 2;
@@ -938,8 +941,8 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
         I.tiles = tiles_for(I.r, I.jpt);
         tr[b + 1] = tr[b] + ((I.n + 63) / 64) * ((I.r + 63) / 64);
         for (int v = 0; v < 3; v++)
-            scs[v][b + 1] = scs[v][b] + ((jpt_slot(I.jpt) == v) ? I.r * I.m * I.tiles : 0u);
-        be[b + 1] = be[b] + I.r;
+            scs[v][b + 1] = scs[v][b] + ((jpt_slot(I.jpt) == v) ? I.rb * I.m * I.tiles : 0u);
+        be[b + 1] = be[b] + I.rb;
         if (I.useLds)
             pl.ldsBytes = std::max(pl.ldsBytes, sizeof(float) * (size_t)I.n);
         if (I.horizon != kNoHorizon)
@@ -1797,6 +1800,285 @@ int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps, POMDPPBVIGPU *const
         pomdp_pbvi_uninitialize_gpu(pomdps[b], pbvis[b]);
     if (planMem)
         device_free(planMem, fn, "the batch plan");
+    return rc;
+}
+
+} // namespace nova
+
+// ------------------------------------------------------------------------- Perseus (GPU)
+//
+// nova's randomized Perseus loop (src/pomdp/algorithms/pomdp_perseus_cpu.cpp:312-429) with its
+// device work on the GPU: the backup of the sampled belief against the current alpha set runs
+// on the PBVI kernels (one belief, rGamma alphas), and V(b) over the whole belief set is kept
+// on the device (the new set only grows by one alpha per step, so V_{n+1}(b) is updated by one
+// sparse dot per belief). The belief is drawn on the host with libc rand() by nova's formula,
+// so with the same seed, exact mode reproduces nova's CPU results bit for bit.
+
+namespace nova_amd {
+namespace {
+
+// v[i] = max(v[i], b_i . alpha) (first = initialise), b . alpha summed B*alpha in support order
+__global__ __launch_bounds__(kPB) void perseus_vb_update(unsigned n, unsigned r, unsigned rz,
+        const int *__restrict__ Z, const float *__restrict__ B, const float *__restrict__ alpha,
+        int first, float *__restrict__ v)
+{
+    const unsigned i = blockIdx.x * kPB + threadIdx.x;
+    if (i >= r)
+        return;
+    float d = 0.0f;
+    for (unsigned k = 0; k < rz; k++) {
+        const int s = Z[(size_t)i * rz + k];
+        if (s < 0)
+            break;
+        d = __fadd_rn(d, __fmul_rn(B[(size_t)i * rz + k], alpha[s]));
+    }
+    if (first || v[i] < d)
+        v[i] = d;
+}
+
+// V(b) over the rG alphas of G for every belief (first maximiser index in arg), -1e10 when empty
+__global__ __launch_bounds__(kPB) void perseus_vb_full(unsigned n, unsigned r, unsigned rz,
+        unsigned rG, const int *__restrict__ Z, const float *__restrict__ B,
+        const float *__restrict__ G, float *__restrict__ v, int *__restrict__ arg)
+{
+    const unsigned i = blockIdx.x * kPB + threadIdx.x;
+    if (i >= r)
+        return;
+    float best = kLow;
+    int bj = 0;
+    for (unsigned g = 0; g < rG; g++) {
+        float d = 0.0f;
+        for (unsigned k = 0; k < rz; k++) {
+            const int s = Z[(size_t)i * rz + k];
+            if (s < 0)
+                break;
+            d = __fadd_rn(d, __fmul_rn(B[(size_t)i * rz + k], G[(size_t)g * n + s]));
+        }
+        if (best < d) {
+            best = d;
+            bj = (int)g;
+        }
+    }
+    v[i] = best;
+    arg[i] = bj;
+}
+
+__global__ void fill_zero(size_t count, float *__restrict__ x, unsigned *__restrict__ a)
+{
+    const size_t q = blockIdx.x * (size_t)kPB + threadIdx.x;
+    if (q < count)
+        x[q] = 0.0f;
+    if (q == 0)
+        *a = 0;
+}
+
+} // namespace
+} // namespace nova_amd
+
+namespace nova {
+
+using namespace nova_amd;
+
+int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlphaVectors *policy)
+{
+    const char *fn = "pomdp_perseus_execute_gpu";
+    if (!pomdp_device_ready(pomdp) || pomdp->gamma < 0.0f || pomdp->gamma > 1.0f ||
+            pomdp->horizon < 1 || per == nullptr || per->GammaInitial == nullptr ||
+            policy == nullptr || dense_mode(g_pbvi_mode) || !pbvi_shape_ok(pomdp)) {
+        NOVA_ERR(fn, "Invalid arguments (device-resident POMDP, GammaInitial, sparse mode).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const unsigned n = pomdp->n, r = pomdp->r, rz = pomdp->rz, m = pomdp->m;
+    const size_t rn = (size_t)r * n;
+    hipStream_t st = stream();
+    // device state: the two alpha sets, the new alpha and its action, V(b) arrays, workspace
+    float *GA = nullptr, *GB = nullptr, *alphaNew = nullptr, *vOld = nullptr, *vNew = nullptr;
+    float *vTmp = nullptr;
+    unsigned *actNew = nullptr;
+    int *argOld = nullptr;
+    void *ws = nullptr;
+    int rc = device_alloc((void **)&GA, rn * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&GB, rn * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&alphaNew, (size_t)n * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&actNew, 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&vOld, (size_t)r * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&vNew, (size_t)r * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&argOld, (size_t)r * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&vTmp, (size_t)r * 4, fn, "Perseus buffers");
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc(&ws, align256(inst_ws_bytes(pomdp)) + plan_bytes(1), fn, "Perseus buffers");
+    hipError_t e = hipSuccess;
+    if (rc == NOVA_SUCCESS) {
+        e = hipMemcpy(GA, per->GammaInitial, rn * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(GB, per->GammaInitial, rn * 4, hipMemcpyHostToDevice);
+    }
+    PbviInst I;
+    if (rc == NOVA_SUCCESS && e == hipSuccess) {
+        fill_inst(pomdp, I, GA, alphaNew, actNew, static_cast<char *>(ws), kNoHorizon);
+        rc = build_ot(pomdp, I.OT, st);
+        if (rc == NOVA_SUCCESS)
+            I.jpt = tune_jpt(I, static_cast<char *>(ws) + align256(inst_ws_bytes(pomdp)), st,
+                             g_pbvi_mode);
+    }
+    if (rc == NOVA_SUCCESS && e != hipSuccess)
+        rc = hip_fail(e, fn, "Failed to upload GammaInitial.", NOVA_ERROR_MEMCPY_TO_DEVICE);
+
+    // host state, as POMDPPerseusCPU holds it (pomdp_perseus_cpu.cpp:183-221)
+    unsigned horizon = 0, rGam = 0, rGamP = 0;
+    std::vector<unsigned> piA(r, 0), piB(r, 0), BTilde(r);
+    for (unsigned i = 0; i < r; i++)
+        BTilde[i] = i;
+    unsigned rTilde = r;
+    std::vector<float> hOld(r), hNew(r);
+    std::vector<int> hArg(r);
+    void *planMem = static_cast<char *>(ws) + align256(inst_ws_bytes(pomdp));
+    const unsigned gridB = (r + kPB - 1) / kPB;
+    bool horizonStart = true;
+    float *G = GA, *Gp = GB;
+    unsigned *rG = &rGam, *rGp = &rGamP;
+    std::vector<unsigned> *pi = &piA, *piP = &piB;
+    while (rc == NOVA_SUCCESS && horizon < pomdp->horizon) {
+        const bool even = (horizon % 2 == 0);
+        G = even ? GA : GB;
+        Gp = even ? GB : GA;
+        rG = even ? &rGam : &rGamP;
+        rGp = even ? &rGamP : &rGam;
+        pi = even ? &piA : &piB;
+        piP = even ? &piB : &piA;
+        if (horizonStart) {
+            // V_n(b) for every belief, fixed during this horizon; GT of the current set
+            hipLaunchKernelGGL(perseus_vb_full, dim3(gridB), dim3(kPB), 0, st, n, r, rz, *rG,
+                               pomdp->d_Z, pomdp->d_B, G, vOld, argOld);
+            I.r = std::max(*rG, 1u);
+            I.rb = 1;
+            I.G0 = G;
+            PbviPlan pl;
+            pl.count = 1;
+            pl.host.assign(1, I);
+            rc = plan_upload(pl, planMem, st, fn);
+            if (rc != NOVA_SUCCESS)
+                break;
+            hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst,
+                               pl.dOff, 1u, 0u, 0u);
+            e = hipMemcpy(hOld.data(), vOld, r * 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess)
+                e = hipMemcpy(hArg.data(), argOld, r * 4, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, fn, "Failed to read V(b).", NOVA_ERROR_MEMCPY_TO_HOST);
+                break;
+            }
+            horizonStart = false;
+        }
+        // draw a belief still to improve (nova's formula, index kept below rTilde)
+        unsigned tIdx = (unsigned)((float)std::rand() / (float)RAND_MAX * (float)rTilde);
+        if (tIdx >= rTilde)
+            tIdx = rTilde - 1;
+        const unsigned bIndex = BTilde[tIdx];
+        // backup of belief bIndex against the current set: a one-belief PBVI iteration
+        PbviPlan pl;
+        pl.count = 1;
+        PbviInst J = I;
+        J.r = std::max(*rG, 1u);
+        J.rb = 1;
+        J.G0 = G;
+        J.G1 = alphaNew;
+        J.pi = actNew;
+        J.Z = pomdp->d_Z + (size_t)bIndex * rz;
+        J.B = pomdp->d_B + (size_t)bIndex * rz;
+        pl.host.assign(1, J);
+        rc = plan_upload(pl, planMem, st, fn);
+        if (rc != NOVA_SUCCESS)
+            break;
+        hipLaunchKernelGGL(fill_zero, dim3((n + kPB - 1) / kPB), dim3(kPB), 0, st, (size_t)n,
+                           alphaNew, actNew);
+        {
+            const unsigned c = 1;
+            const int v = jpt_slot(J.jpt);
+            const unsigned *sc = pl.dOff + (1 + v) * (c + 1), *be = pl.dOff + 4 * (c + 1);
+            dim3 g(pl.totSc[v]);
+            if (g_pbvi_mode == 1) {
+                if (v == 0) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+                if (v == 1) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+                if (v == 2) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+            } else {
+                if (v == 0) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+                if (v == 1) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+                if (v == 2) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
+            }
+            hipLaunchKernelGGL(pbvi_action, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
+            auto matFn = (g_pbvi_mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
+            hipLaunchKernelGGL(matFn, dim3(1), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, 0u, 0u);
+        }
+        // b . alpha_new (support order), read for bIndex only
+        hipLaunchKernelGGL(perseus_vb_update, dim3(gridB), dim3(kPB), 0, st, n, r, rz,
+                           pomdp->d_Z, pomdp->d_B, alphaNew, 1, vTmp);
+        float bDot = 0.0f;
+        unsigned aNew = 0;
+        e = hipMemcpy(&bDot, vTmp + bIndex, 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(&aNew, actNew, 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, fn, "Failed to run the backup.", NOVA_ERROR_KERNEL_EXECUTION);
+            break;
+        }
+        const float Vnb = (*rG > 0) ? hOld[bIndex] : kLow;
+        const unsigned oldArg = (*rG > 0) ? (unsigned)hArg[bIndex] : 0u;
+        float *dst = Gp + (size_t)(*rGp) * n;
+        if (bDot >= Vnb) {
+            e = hipMemcpyAsync(dst, alphaNew, (size_t)n * 4, hipMemcpyDeviceToDevice, st);
+            (*piP)[*rGp] = aNew;
+        } else {
+            e = hipMemcpyAsync(dst, G + (size_t)oldArg * n, (size_t)n * 4, hipMemcpyDeviceToDevice, st);
+            (*piP)[*rGp] = (*pi)[oldArg];
+        }
+        (*rGp)++;
+        if (*rGp > r) {
+            NOVA_ERR(fn, "Out of memory. Too many alpha-vectors added.");
+            rc = NOVA_ERROR_OUT_OF_MEMORY;
+            break;
+        }
+        // V_{n+1}(b) grows by the appended alpha: a running maximum per belief
+        hipLaunchKernelGGL(perseus_vb_update, dim3(gridB), dim3(kPB), 0, st, n, r, rz,
+                           pomdp->d_Z, pomdp->d_B, dst, (*rGp == 1) ? 1 : 0, vNew);
+        e = hipMemcpy(hNew.data(), vNew, r * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, fn, "Failed to read V(b).", NOVA_ERROR_MEMCPY_TO_HOST);
+            break;
+        }
+        rTilde = 0;
+        for (unsigned i = 0; i < r; i++) {
+            const float vo = (*rG > 0) ? hOld[i] : kLow;
+            if (hNew[i] < vo)
+                BTilde[rTilde++] = i;
+        }
+        if (rTilde == 0) {
+            horizon++;
+            *rG = 0;
+            rTilde = r;
+            for (unsigned i = 0; i < r; i++)
+                BTilde[i] = i;
+            horizonStart = true;
+        }
+    }
+    if (rc == NOVA_SUCCESS) {
+        const bool even = (horizon % 2 == 0);
+        const unsigned rOut = even ? rGam : rGamP;
+        if (pomdp_alpha_vectors_initialize(policy, n, m, rOut) != NOVA_SUCCESS) {
+            NOVA_ERR(fn, "Could not create the policy.");
+            rc = NOVA_ERROR_POLICY_CREATION;
+        } else {
+            e = hipMemcpy(policy->Gamma, even ? GA : GB, sizeof(float) * (size_t)rOut * n,
+                          hipMemcpyDeviceToHost);
+            std::memcpy(policy->pi, (even ? piA : piB).data(), sizeof(unsigned) * rOut);
+            if (e != hipSuccess)
+                rc = hip_fail(e, fn, "Failed to copy the policy.", NOVA_ERROR_MEMCPY_TO_HOST);
+        }
+    }
+    void *bufs[] = {GA, GB, alphaNew, actNew, vOld, vNew, vTmp, argOld, ws};
+    for (void *q : bufs)
+        if (q)
+            device_free(q, fn, "Perseus buffers");
     return rc;
 }
 

@@ -201,6 +201,63 @@ def bench_pbvi(args):
     return out
 
 
+def bench_dense(args):
+    """Dense-belief scoring on the matrix cores: n=20,000, m=16, z=32, ns=8, r=4,096 beliefs
+    with full support (rz = n), one backup from the same Gamma with bf16 operands and with
+    fp32 operands (fp32 accumulation in both); the bf16 choices are compared with fp32's."""
+    from nova_amd import synth
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, set_mode
+    base = synth.workload("pbvi_n20000", rz=4)
+    P = synth.dense_beliefs(base, seed=8)
+    del base
+    r, n = P["Gamma0"].shape
+    m, z = P["R"].shape[1], P["O"].shape[2]
+    rng = np.random.default_rng(9)
+    G = (P["Gamma0"] + rng.uniform(-1, 1, P["Gamma0"].shape)).astype(np.float32)
+    flops = 2.0 * m * z * r * r * n
+    peaks = {2: 2500.0, 3: 157.3}
+    res = {}
+    for mode, label in ((2, "bf16"), (3, "fp32")):
+        set_mode(mode)
+        try:
+            p = POMDP.from_dict(P)
+            p.initialize_gpu()
+            s = POMDPPBVIGPU(p, GammaInitial=G)
+            s.profile()                                   # warm-up
+            s2 = POMDPPBVIGPU(p, GammaInitial=G)
+            del s
+            stages = s2.profile()
+            js, act = s2.argmax_record()
+            G1 = s2.get_policy().Gamma_array()
+            del s2
+            p.uninitialize_gpu()
+        finally:
+            set_mode(FAST)
+        tf = flops / (stages[1] * 1e-3) / 1e12
+        res[label] = dict(stage_ms=dict(zip(["convert", "score", "action", "materialize"],
+                                            stages)),
+                          achieved=tf, peak=peaks[mode], frac=tf / peaks[mode],
+                          js=js, act=act, G1=G1)
+        print("dense %s leg done" % label, file=sys.stderr, flush=True)
+    b, f = res["bf16"], res["fp32"]
+    Bd = P["B"].astype(np.float64)
+    vb = (b["G1"].astype(np.float64) @ Bd.T).max(axis=0)
+    vf = (f["G1"].astype(np.float64) @ Bd.T).max(axis=0)
+    out = {"workload": "pbvi_n20000_dense (r=4096, rz=n)", "algorithmic_flops_per_launch": flops,
+           "alpha_choice_disagreement_bf16_vs_fp32": float(np.mean(b["js"] != f["js"])),
+           "action_disagreement_bf16_vs_fp32": float(np.mean(b["act"] != f["act"])),
+           "value_rel_error_bf16_vs_fp32": float(np.max(np.abs(vb - vf)) /
+                                                 max(1.0, np.max(np.abs(vf))))}
+    for label in ("bf16", "fp32"):
+        x = res[label]
+        out[label] = {"stage_ms": x["stage_ms"],
+                      "roofline": {"bound": "mfma", "kernel": "dense_score<%s>" % label,
+                                   "achieved": x["achieved"], "peak": x["peak"],
+                                   "unit": "TFLOP/s", "frac": x["achieved"] / x["peak"]}}
+    return out
+
+
 def bench_batch(args):
     """256 uneven POMDPs: batched graph vs 256 sequential plain GPU solves."""
     from nova_amd import synth
@@ -246,8 +303,11 @@ def main():
                     help="sweeps per step (default: the workload's 200; fewer only for profiling)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one plain device copy of this many MB first (counter scaling)")
+    ap.add_argument("--no-vi", action="store_true",
+                    help="skip the VI leg (the line then carries only the other legs; profiling)")
     ap.add_argument("--no-pbvi", action="store_true", help="skip the PBVI leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve leg")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense-belief MFMA leg")
     ap.add_argument("--cpu-beliefs", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=256)
     args = ap.parse_args()
@@ -255,11 +315,14 @@ def main():
         print("bench.py: this build measures one GPU only (the path is single-GPU; see the "
               "sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    out = bench_vi(args)
+    out = {"metric": METRIC, "config": {"workload": "none (VI leg skipped)"}} if args.no_vi \
+        else bench_vi(args)
     extras = {}
     if not args.no_pbvi:
         extras["pbvi"] = bench_pbvi(args)
         print("pbvi leg done", file=sys.stderr, flush=True)
+    if not args.no_dense:
+        extras["dense"] = bench_dense(args)
     if not args.no_batch:
         extras["batch"] = bench_batch(args)
         print("batch leg done", file=sys.stderr, flush=True)

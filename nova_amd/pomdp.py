@@ -86,10 +86,33 @@ class POMDP(_abi.NovaPOMDP):
         return (view(self.Z, self.r * self.rz, np.int32).reshape(self.r, self.rz),
                 view(self.B, self.r * self.rz, np.float32).reshape(self.r, self.rz))
 
-    def expand(self, method="random", numBeliefsToAdd=1000, pemaPolicy=None):
-        """Expands the belief set (python/nova/pomdp.py:202-229). Host algorithms; call
+    def expand(self, method="random", numBeliefsToAdd=1000, pemaPolicy=None, gpu=False,
+               seed=None):
+        """Expands the belief set (python/nova/pomdp.py:202-229). With gpu=True the random
+        trajectories (pomdp_expand_random_gpu, or its seeded variant when seed is given) or the
+        distinct-belief scoring run on the device; the POMDP must then be resident. Call
         initialize_gpu() again afterwards to refresh the device copy of Z/B."""
-        if method == "random":
+        if gpu and method == "random":
+            nnz = ct.c_uint(0)
+            Bnew = _abi.c_float_p()
+            if seed is None:
+                rc = self._lib.pomdp_expand_random_gpu(self, 1024, numBeliefsToAdd,
+                                                       ct.byref(nnz), ct.byref(Bnew))
+            else:
+                rc = self._lib.pomdp_expand_random_seeded_gpu(self, seed, numBeliefsToAdd,
+                                                              ct.byref(nnz), ct.byref(Bnew))
+            if rc == 0:
+                rc = self._lib.pomdp_add_new_raw_beliefs_cpu(self, numBeliefsToAdd, Bnew)
+            if Bnew:
+                self._lib.nova_amd_free_floats(Bnew)
+        elif gpu and method == "distinct_beliefs":
+            rc = self._lib.pomdp_expand_distinct_beliefs_gpu(self)
+        elif gpu:
+            raise Exception("Expansion '%s' has no GPU variant." % method)
+        elif seed is not None:
+            raise Exception("A seed applies to the GPU random expansion only (the host one "
+                            "seeds with srand(time) as nova does).")
+        elif method == "random":
             rc = self._lib.pomdp_expand_random_cpu(self, numBeliefsToAdd)
         elif method == "distinct_beliefs":
             rc = self._lib.pomdp_expand_distinct_beliefs_cpu(self)

@@ -130,8 +130,8 @@ def batch_execute(solvers, policies=True):
     count = len(solvers)
     PP = ct.POINTER(_abi.NovaPOMDP) * count
     SP = ct.POINTER(_abi.NovaPOMDPPBVIGPU) * count
-    parr = PP(*[ct.pointer(s.pomdp) for s in solvers])
-    sarr = SP(*[ct.pointer(s) for s in solvers])
+    parr = PP(*[ct.cast(ct.pointer(s.pomdp), ct.POINTER(_abi.NovaPOMDP)) for s in solvers])
+    sarr = SP(*[ct.cast(ct.pointer(s), ct.POINTER(_abi.NovaPOMDPPBVIGPU)) for s in solvers])
     pols = (_abi.NovaPOMDPAlphaVectors * count)() if policies else None
     ms = ct.c_float(0.0)
     rc = lib.pomdp_pbvi_batch_execute_gpu(ct.cast(parr, ct.c_void_p), ct.cast(sarr, ct.c_void_p),

@@ -30,3 +30,11 @@ class POMDPPerseusCPU(_abi.NovaPOMDPPerseusCPU):
         if self._lib.pomdp_perseus_execute_cpu(self.pomdpPtr, self, policy) != 0:
             raise Exception("Failed to execute the CPU Perseus solver.")
         return policy
+
+    def solve_gpu(self):
+        """pomdp_perseus_execute_gpu: the same randomized loop with its backups and V(b)
+        updates on the device (the POMDP must be resident)."""
+        policy = POMDPAlphaVectors()
+        if self._lib.pomdp_perseus_execute_gpu(self.pomdpPtr, self, policy) != 0:
+            raise Exception("Failed to execute the GPU Perseus solver.")
+        return policy

@@ -82,6 +82,17 @@ def pomdp_arrays(n, m, z, ns, r, rz, seed, gamma=0.95, horizon=1, **_):
     return dict(S=S, T=T, O=O, R=R, Z=Z, B=B, gamma=g, horizon=int(horizon), Gamma0=Gamma0)
 
 
+def dense_beliefs(P, seed):
+    """Replaces P's belief set by r dense beliefs: Z[i] = 0..n-1, B[i] ~ Dirichlet(1) over all n
+    states (the dense-belief variant of the PBVI workloads)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    r, n = P["Gamma0"].shape
+    P = dict(P)
+    P["Z"] = np.tile(np.arange(n, dtype=np.int32), (r, 1))
+    P["B"] = dirichlet1(rng, (r, n))
+    return P
+
+
 def workload(name, **overrides):
     p = dict(WORKLOADS[name])
     p.update(overrides)

@@ -43,7 +43,7 @@ def test_pbvi_cpu_bitwise(reflib):
 def test_perseus_cpu_bitwise_seeded(reflib):
     """Both libraries draw beliefs with the process's libc rand(); seeding it identically
     before each run makes the randomized Perseus comparable bit for bit."""
-    P = synth.pomdp_arrays(n=25, m=3, z=3, ns=4, r=20, rz=5, seed=43, gamma=0.9, horizon=4)
+    P = synth.pomdp_arrays(n=80, m=4, z=6, ns=5, r=60, rz=8, seed=44, gamma=0.9, horizon=30)
     p = POMDP.from_dict(P)
     libc.srand(1234)
     pol = POMDPPerseusCPU(p, GammaInitial=P["Gamma0"]).solve()

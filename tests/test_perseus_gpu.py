@@ -49,18 +49,22 @@ def run_gpu(lib, P, seed, mode):
         set_mode(FAST)
 
 
-@pytest.mark.parametrize("params", [dict(n=25, m=3, z=3, ns=4, r=20, rz=5, seed=43, horizon=4),
-                                    dict(n=80, m=4, z=6, ns=5, r=60, rz=8, seed=44, horizon=3)])
+@pytest.mark.parametrize("params", [dict(n=25, m=3, z=3, ns=4, r=20, rz=5, seed=43, horizon=40),
+                                    dict(n=80, m=4, z=6, ns=5, r=60, rz=8, seed=44, horizon=30),
+                                    dict(n=300, m=5, z=8, ns=6, r=200, rz=12, seed=45,
+                                         horizon=25)])
 def test_perseus_gpu_exact_bitwise(reflib, lib, params):
     P = synth.pomdp_arrays(gamma=0.9, **params)
     G, pi = run_ref(reflib, P, 99)
     Gg, pig = run_gpu(lib, P, 99, EXACT)
+    print("perseus exact mode n=%d: %d alphas, bitwise %s" % (params["n"], G.shape[0],
+          Gg.shape == G.shape and np.array_equal(Gg.view(np.uint32), G.view(np.uint32))))
     assert Gg.shape == G.shape
     assert np.array_equal(Gg.view(np.uint32), G.view(np.uint32)) and np.array_equal(pig, pi)
 
 
 def test_perseus_gpu_fast_values(reflib, lib):
-    P = synth.pomdp_arrays(n=80, m=4, z=6, ns=5, r=60, rz=8, seed=45, gamma=0.9, horizon=3)
+    P = synth.pomdp_arrays(n=80, m=4, z=6, ns=5, r=60, rz=8, seed=45, gamma=0.9, horizon=30)
     G, _ = run_ref(reflib, P, 7)
     Gg, _ = run_gpu(lib, P, 7, FAST)
     Z, B = P["Z"], P["B"]

@@ -383,12 +383,146 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
     }
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+// Fast-mode scoring on the matrix cores. For belief i, action a and a tile of kMTile alphas it
+// forms C[o][j] = sum_u W[u][o] * GT[sp_u][j], W = gamma * b_k * O[a,sp_u,o] * T[s_k,a,l] over
+// the belief's valid (k, l) pairs u, as v_mfma_f32_32x32x2_f32 products: rows are 32
+// observations, columns alphas, the reduction runs over the pairs two at a time. W is staged in
+// LDS in chunks of kWChunk pairs; each wave owns 128 alphas as four 32 x 32 blocks and loads
+// its B operand straight from GT (two 128-byte row segments per instruction). The epilogue
+// writes the same per-tile argmax partials as pbvi_score.
+constexpr int kMTile = 512;          // alphas per workgroup
+constexpr int kWChunk = 256;         // support pairs per staged W chunk
+
+__global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter)
+{
+    __shared__ SupportLds L;
+    __shared__ float w[kWChunk * 32];         // [u][o] for the current chunk and 32 observations
+    __shared__ int scratch[kKMax + 1];
+    __shared__ float redV[kPB / 64][32];
+    __shared__ int redJ[kPB / 64][32];
+
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r;
+    const float gamma = P.gamma;
+    const unsigned local = blockIdx.x - off[b];
+    const unsigned rb = P.rb;
+    const unsigned ia = local % (rb * m);        // belief * m + action
+    const unsigned tile = local / (rb * m);
+    const unsigned i = ia / m, a = ia % m;
+    const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
+
+    build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);
+    const int kEnd = L.kEnd, uEnd = L.uEnd;
+    const float *__restrict__ GT = P.GT;
+    const float *__restrict__ OT = P.OT;
+    const unsigned jw = tile * kMTile + wave * 128;     // the wave's first alpha
+
+    for (unsigned o0 = 0; o0 < z; o0 += 32) {
+        const int zc = min(32, (int)(z - o0));
+        floatx16_t acc[4];
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int v = 0; v < 16; v++)
+                acc[y][v] = 0.0f;
+        for (int c0 = 0; c0 < uEnd; c0 += kWChunk) {
+            const int cn = min(kWChunk, uEnd - c0);
+            const int cPad = (cn + 1) & ~1;
+            __syncthreads();
+            for (int x = t; x < cPad * 32; x += kPB) {
+                const int u = c0 + x / 32, oo = x % 32;
+                float v = 0.0f;
+                if (u < uEnd && oo < zc) {
+                    // the support entry k of pair u: startK[k] <= u < startK[k+1]
+                    int lo = 0, hi = kEnd - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) / 2;
+                        if (L.startK[mid] <= u) lo = mid; else hi = mid - 1;
+                    }
+                    v = gamma * L.bK[lo] * OT[((size_t)(a * z + o0 + oo) * n) * ns + L.otU[u]];
+                }
+                w[x] = v;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int u0 = 0; u0 < cPad; u0 += 2) {
+                const int uu = u0 + (int)(lane / 32);
+                const float fa = w[uu * 32 + lane % 32];
+                const int sp = (c0 + uu < uEnd) ? L.spU[c0 + uu] : 0;
+                const float *row = GT + (size_t)sp * r;
+                float fb[4];
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    const unsigned j = jw + y * 32 + lane % 32;
+                    fb[y] = (j < r) ? row[j] : 0.0f;
+                }
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb[y], acc[y], 0, 0, 0);
+            }
+        }
+        // per row (observation): first maximiser over the wave's 128 alphas, then the 4 waves
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            float bv = kLow;
+            int bj = INT_MAX;
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                const unsigned j = jw + y * 32 + lane % 32;
+                if (j < r && acc[y][v] > bv) {
+                    bv = acc[y][v];
+                    bj = (int)j;
+                }
+            }
+            for (int off2 = 16; off2 >= 1; off2 >>= 1) {
+                const float ov = __shfl_xor(bv, off2, 32);
+                const int oj = __shfl_xor(bj, off2, 32);
+                if (better(ov, oj, bv, bj)) {
+                    bv = ov;
+                    bj = oj;
+                }
+            }
+            if (lane % 32 == 0) {
+                const int o = (v % 4) + 4 * (int)(lane / 32) + 8 * (v / 4);
+                redV[wave][o] = bv;
+                redJ[wave][o] = bj;
+            }
+        }
+        __syncthreads();
+        if (t < (unsigned)zc) {
+            float v = redV[0][t];
+            int j = redJ[0][t];
+            for (int wv = 1; wv < kPB / 64; wv++)
+                if (better(redV[wv][t], redJ[wv][t], v, j)) {
+                    v = redV[wv][t];
+                    j = redJ[wv][t];
+                }
+            const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
+            P.partVal[idx] = v;
+            P.partJ[idx] = j;
+        }
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------------------- 3. action
 
+// EXACT: each action's alpha is formed on the belief's support in the reference's rounding order
+// and dotted with b, as nova chooses. Fast: the action value is b . R[:,a] plus the observations'
+// best scores (each score is already the b-weighted term of that observation).
+template <bool EXACT>
 __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     __shared__ int js[kZC * 8];
+    __shared__ float bestO[kZC * 8];
     __shared__ float alphaK[kKMax];
     __shared__ float bS[kKMax];
     __shared__ int kEndS;
@@ -430,9 +564,23 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
                 if (j == INT_MAX)
                     j = 0;   // no alpha above -1e10: the reference's index is undefined
                 js[oo] = j;
+                bestO[oo] = v;
                 P.jstar[ia * z + o0 + oo] = j;
             }
             __syncthreads();
+            if (!EXACT) {
+                if (t == 0) {
+                    float v = (o0 == 0) ? 0.0f : alphaK[0];
+                    if (o0 == 0)
+                        for (int k = 0; k < kEnd; k++)
+                            v += bS[k] * P.R[(size_t)P.Z[(size_t)i * rz + k] * m + a];
+                    for (unsigned oo = 0; oo < oc; oo++)
+                        v += bestO[oo];
+                    alphaK[0] = v;                 // running action value
+                }
+                __syncthreads();
+                continue;
+            }
             // alpha_a on the support, observation-major accumulation as the reference
             for (int k = t; k < kEnd; k += kPB) {
                 const int s = P.Z[(size_t)i * rz + k];
@@ -456,8 +604,12 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
         }
         if (t == 0) {
             float v = 0.0f;
-            for (int k = 0; k < kEnd; k++)
-                v = __fadd_rn(v, __fmul_rn(alphaK[k], bS[k]));
+            if (EXACT) {
+                for (int k = 0; k < kEnd; k++)
+                    v = __fadd_rn(v, __fmul_rn(alphaK[k], bS[k]));
+            } else {
+                v = alphaK[0];
+            }
             P.aval[ia] = v;
             if (v > bestValue) {
                 bestValue = v;
@@ -561,18 +713,22 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
 // With dense beliefs (support = all n states) the scoring of observation o under action a is
 //   C[i][j] = sum_s' U_a[i][s'] * (gamma O[a,s',o]) * Gamma[j][s'],   U_a = B T_a (r x n),
 // a GEMM per (a, o) of r x n by n x r. U_a does not depend on Gamma and is built once per
-// initialize. The kernel stages 128 x 32 tiles of U_a and of Gamma (the latter scaled by
-// gamma O[a,.,o] as it is staged) in LDS and accumulates 128 x 128 tiles of C on the matrix
-// cores: bf16 operands with fp32 accumulation (v_mfma_f32_32x32x16_bf16), or fp32 operands
-// (v_mfma_f32_32x32x2_f32) for the full-precision comparison. The epilogue reduces each row to
-// its maximum over the tile's 128 alphas (first maximiser), as the sparse kernel's partials.
+// initialize. Operands live in HBM zero-padded to rpad = 256-multiple rows and kpad =
+// 64-multiple columns, so the GEMM loop needs no bounds checks. A workgroup of 8 waves owns a
+// 256 x 256 tile of C: each k-step stages 256 x 128-byte slices of U_a and of Gamma (the latter
+// scaled by gamma O[a,.,o] in registers on the way) into one of two LDS buffers while the matrix
+// cores work on the other; each wave accumulates a 128 x 64 sub-tile as 4 x 2 blocks of 32 x 32
+// (v_mfma_f32_32x32x16_bf16 on bf16 operands, or v_mfma_f32_32x32x2_f32 on fp32 operands, fp32
+// accumulation in both). The epilogue reduces each row to its maximum over the tile's 256
+// alphas (first maximiser), the same partials the sparse kernel writes.
 
-constexpr int kDT = 128;            // tile rows (beliefs) and columns (alphas)
-constexpr int kDK = 32;             // k (state) step per LDS tile
-constexpr int kDPad = 8;            // LDS row padding (elements)
-
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float floatx16_t __attribute__((ext_vector_type(16)));
+constexpr int kGT = 256;                        // tile rows (beliefs) and columns (alphas)
+constexpr int kGThreads = 512;                  // 8 waves: 2 row halves x 4 column quarters
+constexpr int kGRowBytes = 128;                 // k-slice per LDS row: 64 bf16 or 32 fp32
+constexpr int kGStride = kGRowBytes + 16;       // padded LDS row (bytes)
+constexpr int kGOperandBytes = kGT * kGStride;  // one operand slice
+constexpr size_t kGLdsBytes = 4 * (size_t)kGOperandBytes;   // 2 buffers x 2 operands = 147,456
+constexpr unsigned kGKPad = 64;                 // column padding of the dense operands
 
 template <typename IN> struct DenseIn;
 template <> struct DenseIn<__bf16> {
@@ -582,20 +738,22 @@ template <> struct DenseIn<float> {
     static __device__ __forceinline__ float cvt(float x) { return x; }
 };
 
-// Gamma (float, r x n) -> GI (IN, r x n)
+// Gamma (float, r x n) -> GI (IN, rows of kpad; the padding keeps its zeros)
 template <typename IN>
-__global__ __launch_bounds__(kPB) void dense_convert(size_t count, const float *__restrict__ G,
+__global__ __launch_bounds__(kPB) void dense_convert(unsigned n, unsigned kpad, size_t count,
+                                                     const float *__restrict__ G,
                                                      IN *__restrict__ GI)
 {
     for (size_t x = blockIdx.x * (size_t)kPB + threadIdx.x; x < count; x += (size_t)gridDim.x * kPB)
-        GI[x] = DenseIn<IN>::cvt(G[x]);
+        GI[(x / n) * kpad + x % n] = DenseIn<IN>::cvt(G[x]);
 }
 
 // U[a][i][s'] = sum over predecessors (s, l) of s' under a of T(s,a,l) b_i(s), with the dense
-// belief row b_i staged in LDS from its sparse (Z, B) representation.
+// belief row b_i staged in LDS from its sparse (Z, B) representation. Rows of kpad columns,
+// rpad rows per action; the padding keeps its zeros.
 template <typename IN>
-__global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, unsigned r,
-        const int *__restrict__ Z, const float *__restrict__ B,
+__global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, unsigned rpad,
+        unsigned kpad, const int *__restrict__ Z, const float *__restrict__ B,
         const unsigned *__restrict__ pstart, const int *__restrict__ pS,
         const float *__restrict__ pT, IN *__restrict__ U)
 {
@@ -617,7 +775,7 @@ __global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, un
         brow[Z[(size_t)i * rz + k]] = B[(size_t)i * rz + k];
     __syncthreads();
     const unsigned *ps = pstart + (size_t)a * (n + 1);
-    IN *out = U + ((size_t)a * r + i) * n;
+    IN *out = U + ((size_t)a * rpad + i) * kpad;
     for (unsigned sp = t; sp < n; sp += kPB) {
         float acc = 0.0f;
         for (unsigned q = ps[sp]; q < ps[sp + 1]; q++)
@@ -626,9 +784,9 @@ __global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, un
     }
 }
 
-// Os[(a*z + o)*n + s'] = gamma O[a,s',o]; Rb[i*m + a] = sum_k B[i,k] R[Z[i,k], a]
-__global__ __launch_bounds__(kPB) void dense_build_scales(unsigned n, unsigned m, unsigned z,
-        unsigned r, unsigned rz, float gamma, const float *__restrict__ O,
+// Os[(a*z + o)*kpad + s'] = gamma O[a,s',o]; Rb[i*m + a] = sum_k B[i,k] R[Z[i,k], a]
+__global__ __launch_bounds__(kPB) void dense_build_scales(unsigned n, unsigned kpad, unsigned m,
+        unsigned z, unsigned r, unsigned rz, float gamma, const float *__restrict__ O,
         const float *__restrict__ R, const int *__restrict__ Z, const float *__restrict__ B,
         float *__restrict__ Os, float *__restrict__ Rb)
 {
@@ -636,7 +794,7 @@ __global__ __launch_bounds__(kPB) void dense_build_scales(unsigned n, unsigned m
     for (size_t x = blockIdx.x * (size_t)kPB + threadIdx.x; x < total; x += (size_t)gridDim.x * kPB) {
         const unsigned sp = x % n;
         const unsigned ao = x / n, a = ao / z, o = ao % z;
-        Os[x] = gamma * O[((size_t)a * n + sp) * z + o];
+        Os[(size_t)ao * kpad + sp] = gamma * O[((size_t)a * n + sp) * z + o];
     }
     for (size_t x = blockIdx.x * (size_t)kPB + threadIdx.x; x < (size_t)r * m; x += (size_t)gridDim.x * kPB) {
         const unsigned i = x / m, a = x % m;
@@ -651,87 +809,149 @@ __global__ __launch_bounds__(kPB) void dense_build_scales(unsigned n, unsigned m
     }
 }
 
+// 16 bytes of IN scaled elementwise by sc (4 or 8 floats)
+template <typename IN> __device__ __forceinline__ uint4 scale16(uint4 v, const float *sc);
+template <> __device__ __forceinline__ uint4 scale16<float>(uint4 v, const float *sc)
+{
+    uint4 o;
+    o.x = __float_as_uint(__uint_as_float(v.x) * sc[0]);
+    o.y = __float_as_uint(__uint_as_float(v.y) * sc[1]);
+    o.z = __float_as_uint(__uint_as_float(v.z) * sc[2]);
+    o.w = __float_as_uint(__uint_as_float(v.w) * sc[3]);
+    return o;
+}
+template <> __device__ __forceinline__ uint4 scale16<__bf16>(uint4 v, const float *sc)
+{
+    const unsigned in[4] = {v.x, v.y, v.z, v.w};
+    unsigned out[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float lo = __uint_as_float(in[q] << 16) * sc[2 * q];
+        const float hi = __uint_as_float(in[q] & 0xffff0000u) * sc[2 * q + 1];
+        const __bf16 bl = (__bf16)lo, bh = (__bf16)hi;
+        out[q] = (unsigned)__builtin_bit_cast(unsigned short, bl) |
+                 ((unsigned)__builtin_bit_cast(unsigned short, bh) << 16);
+    }
+    return make_uint4(out[0], out[1], out[2], out[3]);
+}
+
 template <typename IN>
-__global__ __launch_bounds__(kPB) void dense_score(unsigned n, unsigned r, unsigned m,
-        unsigned z, unsigned itiles, const IN *__restrict__ U, const IN *__restrict__ GI,
+__global__ __launch_bounds__(kGThreads) void dense_score(unsigned r, unsigned kpad, unsigned m,
+        unsigned z, unsigned tilesPerSide, const IN *__restrict__ U, const IN *__restrict__ GI,
         const float *__restrict__ Os, float *__restrict__ partVal, int *__restrict__ partJ)
 {
-    __shared__ __attribute__((aligned(16))) IN As[kDT][kDK + kDPad];
-    __shared__ __attribute__((aligned(16))) IN Bs[kDT][kDK + kDPad];
-    __shared__ float redV[2][kDT];
-    __shared__ int redJ[2][kDT];
-
+    extern __shared__ __attribute__((aligned(16))) unsigned char glds[];
+    constexpr int EPC = 16 / (int)sizeof(IN);             // elements per 16-byte chunk
+    constexpr int KS = kGRowBytes / (int)sizeof(IN);      // k per step
     const unsigned ao = blockIdx.y, a = ao / z;
-    const unsigned itile = blockIdx.x % itiles, jtile = blockIdx.x / itiles;
-    const unsigned i0 = itile * kDT, j0 = jtile * kDT;
-    const unsigned t = threadIdx.x, l = t % 64, w = t / 64;
-    const unsigned wr = w / 2, wc = w % 2;
-    const IN *Ua = U + (size_t)a * r * n;
-    const float *s = Os + (size_t)ao * n;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, so XCD x gets a contiguous
+    // run of tiles (whole tile rows), sharing the U_a and Gamma panels in its L2
+    const unsigned T = tilesPerSide * tilesPerSide;
+    const unsigned tileIdx = (T % 8 == 0) ? (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8 : blockIdx.x;
+    const unsigned itile = tileIdx / tilesPerSide, jtile = tileIdx % tilesPerSide;
+    const unsigned i0 = itile * kGT, j0 = jtile * kGT;
+    const unsigned t = threadIdx.x, l = t % 64, w = t / 64, wr = w / 4, wc = w % 4;
+    const size_t rpad = (size_t)tilesPerSide * kGT;
+    const IN *Ua = U + ((size_t)a * rpad + i0) * kpad;
+    const IN *Gj = GI + (size_t)j0 * kpad;
+    const float *sv = Os + (size_t)ao * kpad;
 
-    floatx16_t acc[2][2];
+    // staging: 16-byte chunk ch of rows row0 + 64 q (q < 4), for both operands
+    const unsigned ch = t % 8, row0 = t / 8;
+    uint4 ra[4], rb[4];
+    auto gload = [&](unsigned k0) {
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+        for (int q = 0; q < 4; q++) {
+            const size_t off = (size_t)(row0 + 64 * q) * kpad + k0 + ch * EPC;
+            ra[q] = *reinterpret_cast<const uint4 *>(Ua + off);
+            rb[q] = *reinterpret_cast<const uint4 *>(Gj + off);
+        }
+    };
+    auto sstore = [&](unsigned k0, unsigned buf) {
+        float sc[EPC];
+#pragma unroll
+        for (int e = 0; e < EPC; e += 4) {
+            const float4 f = *reinterpret_cast<const float4 *>(sv + k0 + ch * EPC + e);
+            sc[e] = f.x; sc[e + 1] = f.y; sc[e + 2] = f.z; sc[e + 3] = f.w;
+        }
+        unsigned char *A = glds + (size_t)buf * 2 * kGOperandBytes;
+        unsigned char *Bm = A + kGOperandBytes;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const unsigned o = (row0 + 64 * q) * kGStride + ch * 16;
+            *reinterpret_cast<uint4 *>(A + o) = ra[q];
+            *reinterpret_cast<uint4 *>(Bm + o) = scale16<IN>(rb[q], sc);
+        }
+    };
+
+    floatx16_t acc[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
 #pragma unroll
         for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int v = 0; v < 16; v++)
                 acc[x][y][v] = 0.0f;
 
-    // staging map: each lane moves 16 consecutive k of one row (two lanes per row)
-    const unsigned srow = t / 2, skh = (t % 2) * 16;
-    for (unsigned k0 = 0; k0 < n; k0 += kDK) {
-        {
-            const unsigned gi = i0 + srow, gj = j0 + srow;
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const unsigned k = k0 + skh + e;
-                const bool kin = k < n;
-                As[srow][skh + e] = (gi < r && kin) ? Ua[(size_t)gi * n + k] : DenseIn<IN>::cvt(0.0f);
-                const float g = (gj < r && kin) ? (float)GI[(size_t)gj * n + k] * s[k] : 0.0f;
-                Bs[srow][skh + e] = DenseIn<IN>::cvt(g);
-            }
-        }
-        __syncthreads();
+    const unsigned nk = kpad / KS;
+    gload(0);
+    sstore(0, 0);
+    __syncthreads();
+    for (unsigned ks = 0; ks < nk; ks++) {
+        const unsigned buf = ks % 2;
+        if (ks + 1 < nk)
+            gload((ks + 1) * KS);
+        const unsigned char *A = glds + (size_t)buf * 2 * kGOperandBytes;
+        const unsigned char *Bm = A + kGOperandBytes;
         if constexpr (sizeof(IN) == 2) {
 #pragma unroll
-            for (int kk = 0; kk < kDK; kk += 16) {
-                bf16x8_t fa[2], fb[2];
+            for (int kk = 0; kk < KS; kk += 16) {
+                bf16x8_t fa[4], fb[2];
+                const unsigned kb = (kk + 8 * (l / 32)) * 2;
 #pragma unroll
-                for (int x = 0; x < 2; x++) {
-                    const unsigned row = wr * 64 + x * 32 + l % 32;
-                    fa[x] = *reinterpret_cast<const bf16x8_t *>(&As[row][kk + 8 * (l / 32)]);
-                    const unsigned col = wc * 64 + x * 32 + l % 32;
-                    fb[x] = *reinterpret_cast<const bf16x8_t *>(&Bs[col][kk + 8 * (l / 32)]);
-                }
+                for (int x = 0; x < 4; x++)
+                    fa[x] = *reinterpret_cast<const bf16x8_t *>(
+                        A + (wr * 128 + x * 32 + l % 32) * kGStride + kb);
 #pragma unroll
-                for (int x = 0; x < 2; x++)
+                for (int y = 0; y < 2; y++)
+                    fb[y] = *reinterpret_cast<const bf16x8_t *>(
+                        Bm + (wc * 64 + y * 32 + l % 32) * kGStride + kb);
+#pragma unroll
+                for (int x = 0; x < 4; x++)
 #pragma unroll
                     for (int y = 0; y < 2; y++)
                         acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[x], fb[y], acc[x][y], 0, 0, 0);
             }
         } else {
 #pragma unroll
-            for (int kk = 0; kk < kDK; kk += 2) {
-                float fa[2], fb[2];
+            for (int kk = 0; kk < KS; kk += 2) {
+                float fa[4], fb[2];
+                const unsigned kb = (kk + l / 32) * 4;
 #pragma unroll
-                for (int x = 0; x < 2; x++) {
-                    fa[x] = (float)As[wr * 64 + x * 32 + l % 32][kk + l / 32];
-                    fb[x] = (float)Bs[wc * 64 + x * 32 + l % 32][kk + l / 32];
-                }
+                for (int x = 0; x < 4; x++)
+                    fa[x] = *reinterpret_cast<const float *>(
+                        A + (wr * 128 + x * 32 + l % 32) * kGStride + kb);
 #pragma unroll
-                for (int x = 0; x < 2; x++)
+                for (int y = 0; y < 2; y++)
+                    fb[y] = *reinterpret_cast<const float *>(
+                        Bm + (wc * 64 + y * 32 + l % 32) * kGStride + kb);
+#pragma unroll
+                for (int x = 0; x < 4; x++)
 #pragma unroll
                     for (int y = 0; y < 2; y++)
                         acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[x], fb[y], acc[x][y], 0, 0, 0);
             }
         }
+        if (ks + 1 < nk)
+            sstore((ks + 1) * KS, buf ^ 1);
         __syncthreads();
     }
 
-    // epilogue: per row, the first maximiser over this tile's alphas
+    // epilogue: per row, the first maximiser over this tile's 256 alphas
+    float *redV = reinterpret_cast<float *>(glds);             // [4][256]
+    int *redJ = reinterpret_cast<int *>(glds + 4 * kGT * sizeof(float));
 #pragma unroll
-    for (int x = 0; x < 2; x++) {
+    for (int x = 0; x < 4; x++) {
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             float bv = kLow;
@@ -754,26 +974,28 @@ __global__ __launch_bounds__(kPB) void dense_score(unsigned n, unsigned r, unsig
                 }
             }
             if (l % 32 == 0) {
-                const unsigned row = wr * 64 + x * 32 + (v % 4) + 4 * (l / 32) + 8 * (v / 4);
-                redV[wc][row] = bv;
-                redJ[wc][row] = bj;
+                const unsigned row = wr * 128 + x * 32 + (v % 4) + 4 * (l / 32) + 8 * (v / 4);
+                redV[wc * kGT + row] = bv;
+                redJ[wc * kGT + row] = bj;
             }
         }
     }
     __syncthreads();
-    if (t < kDT) {
+    if (t < kGT) {
         const unsigned i = i0 + t;
-        float v0 = redV[0][t];
-        int j0v = redJ[0][t];
-        if (better(redV[1][t], redJ[1][t], v0, j0v)) {
-            v0 = redV[1][t];
-            j0v = redJ[1][t];
-        }
+        float v0 = redV[t];
+        int jv = redJ[t];
+#pragma unroll
+        for (int q = 1; q < 4; q++)
+            if (better(redV[q * kGT + t], redJ[q * kGT + t], v0, jv)) {
+                v0 = redV[q * kGT + t];
+                jv = redJ[q * kGT + t];
+            }
         if (i < r) {
             const unsigned o = ao % z;
             const size_t idx = ((size_t)jtile * r * m + (size_t)i * m + a) * z + o;
             partVal[idx] = v0;
-            partJ[idx] = j0v;
+            partJ[idx] = jv;
         }
     }
 }
@@ -831,7 +1053,10 @@ static bool dense_mode(int mode) { return mode == 2 || mode == 3; }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + kPB * jpt - 1) / (kPB * jpt); }
+// Scoring variants: jpt = 1, 2, 4 alphas per lane on the vector ALUs (tiles of 256 * jpt alphas),
+// jpt = 8 the matrix-core kernel of fast mode (tiles of kMTile alphas).
+static unsigned tile_width(unsigned jpt) { return jpt == 8 ? (unsigned)kMTile : kPB * jpt; }
+static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
 // Bytes of one instance's workspace (tiles sized for the widest variant, jpt = 1).
 static size_t inst_ws_bytes(const nova::POMDP *p)
@@ -913,34 +1138,54 @@ struct PbviPlan {
     unsigned count = 0;
     PbviInst *dInst = nullptr;       // device copy of the descriptors
     unsigned *dOff = nullptr;        // [tr | sc(jpt=1) | sc(2) | sc(4) | be], count+1 each
-    unsigned totTr = 0, totSc[3] = {0, 0, 0}, totBe = 0;
+    unsigned totTr = 0, totSc[4] = {0, 0, 0, 0}, totBe = 0;
     size_t ldsBytes = 0;
     unsigned maxHorizon = 0;
     std::vector<PbviInst> host;
 };
 
-static int jpt_slot(unsigned jpt) { return jpt == 1 ? 0 : (jpt == 2 ? 1 : 2); }
+static int jpt_slot(unsigned jpt) { return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : 3)); }
+constexpr int kVariants = 4;
+
+// Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
+static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
+                         const unsigned *sc, unsigned c, unsigned iter, hipStream_t st)
+{
+    if (mode == 1) {
+        if (jpt == 1) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
+        if (jpt == 2) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
+        if (jpt == 4) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
+    } else {
+        if (jpt == 1) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
+        if (jpt == 2) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
+        if (jpt == 4) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
+        if (jpt == 8) hipLaunchKernelGGL(pbvi_score_mfma, g, dim3(kPB), 0, st, dInst, sc, c, iter);
+    }
+}
 
 // Writes descriptors and offsets into dst (device memory of plan_bytes(count)).
 static size_t plan_bytes(unsigned count)
 {
-    return align256(sizeof(PbviInst) * count) + align256(sizeof(unsigned) * 5 * (count + 1));
+    return align256(sizeof(PbviInst) * count) +
+           align256(sizeof(unsigned) * (kVariants + 2) * (count + 1));
 }
 
 static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
 {
     const unsigned c = pl.count;
-    std::vector<unsigned> off(5 * (size_t)(c + 1), 0);
-    unsigned *tr = &off[0], *sc1 = &off[c + 1], *sc2 = &off[2 * (c + 1)],
-             *sc4 = &off[3 * (c + 1)], *be = &off[4 * (c + 1)];
-    unsigned *scs[3] = {sc1, sc2, sc4};
+    // [tr | score offsets of each variant | be], c + 1 entries each
+    std::vector<unsigned> off((kVariants + 2) * (size_t)(c + 1), 0);
+    unsigned *tr = &off[0], *be = &off[(kVariants + 1) * (c + 1)];
+    unsigned *scs[kVariants];
+    for (int v = 0; v < kVariants; v++)
+        scs[v] = &off[(1 + v) * (c + 1)];
     pl.ldsBytes = 0;
     pl.maxHorizon = 0;
     for (unsigned b = 0; b < c; b++) {
         PbviInst &I = pl.host[b];
         I.tiles = tiles_for(I.r, I.jpt);
         tr[b + 1] = tr[b] + ((I.n + 63) / 64) * ((I.r + 63) / 64);
-        for (int v = 0; v < 3; v++)
+        for (int v = 0; v < kVariants; v++)
             scs[v][b + 1] = scs[v][b] + ((jpt_slot(I.jpt) == v) ? I.rb * I.m * I.tiles : 0u);
         be[b + 1] = be[b] + I.rb;
         if (I.useLds)
@@ -949,7 +1194,7 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
             pl.maxHorizon = std::max(pl.maxHorizon, I.horizon);
     }
     pl.totTr = tr[c];
-    for (int v = 0; v < 3; v++)
+    for (int v = 0; v < kVariants; v++)
         pl.totSc[v] = scs[v][c];
     pl.totBe = be[c];
     pl.dInst = static_cast<PbviInst *>(dst);
@@ -970,13 +1215,13 @@ struct DenseWs {
     const void *key = nullptr;       // POMDPPBVIGPU::d_alphaBA it belongs to
     int mode = 0;
     void *mem = nullptr;
-    void *U = nullptr;               // [m][r][n] in the scoring precision
-    void *GI = nullptr;              // [r][n] Gamma in the scoring precision
-    float *Os = nullptr;             // [m][z][n] gamma*O
+    void *U = nullptr;               // [m][rpad][kpad] in the scoring precision
+    void *GI = nullptr;              // [rpad][kpad] Gamma in the scoring precision
+    float *Os = nullptr;             // [m][z][kpad] gamma*O
     float *Rb = nullptr;             // [r][m] b . R[:,a]
     float *partVal = nullptr;        // [tiles][r][m][z]
     int *partJ = nullptr;
-    unsigned tiles = 0, n = 0, m = 0, z = 0, r = 0;
+    unsigned tiles = 0, n = 0, m = 0, z = 0, r = 0, rpad = 0, kpad = 0;
 };
 static std::vector<DenseWs> g_dense;
 
@@ -998,31 +1243,41 @@ static void dense_release(const void *key, const char *fn)
         }
 }
 
+// Gamma conversion and the matrix-core scoring of one dense iteration; ev (nullable) gets an
+// event recorded between the two.
+static void dense_launch_score(const DenseWs &D, const float *G, hipStream_t st, hipEvent_t ev)
+{
+    const size_t rn = (size_t)D.r * D.n;
+    const unsigned cg = (unsigned)std::min<size_t>((rn + kPB - 1) / kPB, 65536);
+    const dim3 sg(D.tiles * D.tiles, D.m * D.z);
+    if (D.mode == 2) {
+        hipLaunchKernelGGL(dense_convert<__bf16>, dim3(cg), dim3(kPB), 0, st, D.n, D.kpad, rn, G,
+                           static_cast<__bf16 *>(D.GI));
+        if (ev)
+            hipEventRecord(ev, st);
+        hipLaunchKernelGGL(dense_score<__bf16>, sg, dim3(kGThreads), kGLdsBytes, st, D.r, D.kpad,
+                           D.m, D.z, D.tiles, static_cast<const __bf16 *>(D.U),
+                           static_cast<const __bf16 *>(D.GI), D.Os, D.partVal, D.partJ);
+    } else {
+        hipLaunchKernelGGL(dense_convert<float>, dim3(cg), dim3(kPB), 0, st, D.n, D.kpad, rn, G,
+                           static_cast<float *>(D.GI));
+        if (ev)
+            hipEventRecord(ev, st);
+        hipLaunchKernelGGL(dense_score<float>, sg, dim3(kGThreads), kGLdsBytes, st, D.r, D.kpad,
+                           D.m, D.z, D.tiles, static_cast<const float *>(D.U),
+                           static_cast<const float *>(D.GI), D.Os, D.partVal, D.partJ);
+    }
+}
+
 static void dense_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, hipStream_t st)
 {
     const DenseWs &D = *pl.dense;
     const PbviInst &I = pl.host[0];
     const float *G = ((iter + flip) % 2 == 0) ? I.G0 : I.G1;
-    const size_t rn = (size_t)D.r * D.n;
-    const unsigned cg = (unsigned)std::min<size_t>((rn + kPB - 1) / kPB, 65536);
-    const unsigned itiles = (D.r + kDT - 1) / kDT;
-    const dim3 sg(itiles * D.tiles, D.m * D.z);
-    if (D.mode == 2) {
-        hipLaunchKernelGGL(dense_convert<__bf16>, dim3(cg), dim3(kPB), 0, st, rn, G,
-                           static_cast<__bf16 *>(D.GI));
-        hipLaunchKernelGGL(dense_score<__bf16>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z, itiles,
-                           static_cast<const __bf16 *>(D.U), static_cast<const __bf16 *>(D.GI),
-                           D.Os, D.partVal, D.partJ);
-    } else {
-        hipLaunchKernelGGL(dense_convert<float>, dim3(cg), dim3(kPB), 0, st, rn, G,
-                           static_cast<float *>(D.GI));
-        hipLaunchKernelGGL(dense_score<float>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z, itiles,
-                           static_cast<const float *>(D.U), static_cast<const float *>(D.GI),
-                           D.Os, D.partVal, D.partJ);
-    }
+    dense_launch_score(D, G, st, nullptr);
     hipLaunchKernelGGL(dense_action, dim3(D.r), dim3(kPB), 0, st, D.m, D.z, D.r, D.tiles, D.Rb,
                        D.partVal, D.partJ, I.jstar, I.act);
-    const unsigned *be = pl.dOff + 4 * (pl.count + 1);
+    const unsigned *be = pl.dOff + (kVariants + 1) * (pl.count + 1);
     hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st,
                        pl.dInst, be, pl.count, iter, flip);
 }
@@ -1036,24 +1291,19 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
         return;
     }
     const unsigned c = pl.count;
-    const unsigned *tr = pl.dOff, *be = pl.dOff + 4 * (c + 1);
+    const unsigned *tr = pl.dOff, *be = pl.dOff + (kVariants + 1) * (c + 1);
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    for (int v = 0; v < 3; v++) {
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8};
+    for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
-        const unsigned *sc = pl.dOff + (1 + v) * (c + 1);
-        dim3 g(pl.totSc[v]);
-        if (mode == 1) {
-            if (v == 0) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
-            if (v == 1) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
-            if (v == 2) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
-        } else {
-            if (v == 0) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
-            if (v == 1) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
-            if (v == 2) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, iter);
-        }
+        launch_score(kJpt[v], mode, dim3(pl.totSc[v]), pl.dInst, pl.dOff + (1 + v) * (c + 1), c,
+                     iter, st);
     }
-    hipLaunchKernelGGL(pbvi_action, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
+    if (mode == 1)
+        hipLaunchKernelGGL(pbvi_action<true>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
+    else
+        hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
     hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, iter,
                        flip);
@@ -1086,12 +1336,17 @@ static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mod
 // The scoring kernel's variants (1, 2 or 4 alphas per lane) trade occupancy against reuse of
 // the staged O*T factors. For each size class (r, belief-support pairs rz*ns and z, each
 // bucketed to a power of two) the first instance seen is timed with every variant on one
-// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4
-// forces a variant; NOVA_AMD_AUTOTUNE=0 disables timing (default variant 2).
+// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4|8
+// forces a variant (8: the matrix-core kernel, fast mode only); NOVA_AMD_AUTOTUNE=0 disables
+// timing (default variant 2).
 
 struct TuneKey {
     unsigned r, u, z;
-    bool operator==(const TuneKey &o) const { return r == o.r && u == o.u && z == o.z; }
+    int exact;
+    bool operator==(const TuneKey &o) const
+    {
+        return r == o.r && u == o.u && z == o.z && exact == o.exact;
+    }
 };
 static std::vector<std::pair<TuneKey, unsigned>> g_tuned;
 
@@ -1107,13 +1362,13 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
 {
     if (const char *f = std::getenv("NOVA_AMD_PBVI_JPT")) {
         const int v = std::atoi(f);
-        if (v == 1 || v == 2 || v == 4)
+        if (v == 1 || v == 2 || v == 4 || (v == 8 && mode != 1))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
         if (std::atoi(f) == 0)
             return 2;
-    const TuneKey key{bucket(I.r), bucket(I.rz * I.ns), bucket(I.z)};
+    const TuneKey key{bucket(I.r), bucket(I.rz * I.ns), bucket(I.z), mode == 1 ? 1 : 0};
     for (auto &kv : g_tuned)
         if (kv.first == key)
             return kv.second;
@@ -1122,7 +1377,9 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u}) {
+    for (unsigned jpt : {1u, 2u, 4u, 8u}) {
+        if (jpt == 8 && mode == 1)
+            continue;                          // the matrix-core kernel is fast mode only
         PbviPlan pl;
         pl.count = 1;
         I.jpt = jpt;
@@ -1137,15 +1394,7 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
         float ms = 0.0f;
         for (int rep = 0; rep < 2; rep++) {   // first launch warms up
             hipEventRecord(e0, st);
-            if (mode == 1) {
-                if (jpt == 1) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
-                if (jpt == 2) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
-                if (jpt == 4) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
-            } else {
-                if (jpt == 1) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
-                if (jpt == 2) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
-                if (jpt == 4) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, 1u, 0u);
-            }
+            launch_score(jpt, mode, g, pl.dInst, sc, 1u, 0u, st);
             hipEventRecord(e1, st);
             hipEventSynchronize(e1);
             hipEventElapsedTime(&ms, e0, e1);
@@ -1177,10 +1426,13 @@ static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStrea
     D.key = key;
     D.mode = mode;
     D.n = n; D.m = m; D.z = z; D.r = r;
-    D.tiles = (r + kDT - 1) / kDT;
+    D.tiles = (r + kGT - 1) / kGT;
+    D.rpad = D.tiles * kGT;
+    D.kpad = (n + kGKPad - 1) / kGKPad * kGKPad;
     const size_t rmz = (size_t)r * m * z;
-    const size_t bU = align256(esz * (size_t)m * r * n), bG = align256(esz * (size_t)r * n),
-                 bO = align256(sizeof(float) * (size_t)m * z * n),
+    const size_t bU = align256(esz * (size_t)m * D.rpad * D.kpad),
+                 bG = align256(esz * (size_t)D.rpad * D.kpad),
+                 bO = align256(sizeof(float) * (size_t)m * z * D.kpad),
                  bR = align256(sizeof(float) * (size_t)r * m),
                  bP = align256(sizeof(float) * D.tiles * rmz);
     int rc = device_alloc(&D.mem, bU + bG + bO + bR + 2 * bP, fn, "the dense-belief workspace");
@@ -1245,29 +1497,36 @@ static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStrea
             e = hipMemcpy(dpT, pT.data(), sizeof(float) * pT.size(), hipMemcpyHostToDevice);
     }
     if (rc == NOVA_SUCCESS && e == hipSuccess) {
+        // the zero padding of U, GI and Os is never written again
+        const DenseWs &Dn = g_dense.back();
+        e = hipMemsetAsync(Dn.mem, 0, bU + bG + bO, st);
         const int ldsBytes = (int)(sizeof(float) * n);
         const size_t lds = sizeof(float) * n;
         if (mode == 2) {
             hipFuncSetAttribute((const void *)dense_build_u<__bf16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, ldsBytes);
-            hipLaunchKernelGGL(dense_build_u<__bf16>, dim3(r, m), dim3(kPB), lds, st, n, p->rz, r,
-                               p->d_Z, p->d_B, static_cast<unsigned *>(dps),
+            hipFuncSetAttribute((const void *)dense_score<__bf16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
+            hipLaunchKernelGGL(dense_build_u<__bf16>, dim3(r, m), dim3(kPB), lds, st, n, p->rz,
+                               Dn.rpad, Dn.kpad, p->d_Z, p->d_B, static_cast<unsigned *>(dps),
                                static_cast<int *>(dpS), static_cast<float *>(dpT),
-                               static_cast<__bf16 *>(g_dense.back().U));
+                               static_cast<__bf16 *>(Dn.U));
         } else {
             hipFuncSetAttribute((const void *)dense_build_u<float>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, ldsBytes);
-            hipLaunchKernelGGL(dense_build_u<float>, dim3(r, m), dim3(kPB), lds, st, n, p->rz, r,
-                               p->d_Z, p->d_B, static_cast<unsigned *>(dps),
+            hipFuncSetAttribute((const void *)dense_score<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
+            hipLaunchKernelGGL(dense_build_u<float>, dim3(r, m), dim3(kPB), lds, st, n, p->rz,
+                               Dn.rpad, Dn.kpad, p->d_Z, p->d_B, static_cast<unsigned *>(dps),
                                static_cast<int *>(dpS), static_cast<float *>(dpT),
-                               static_cast<float *>(g_dense.back().U));
+                               static_cast<float *>(Dn.U));
         }
         const size_t tot = std::max((size_t)m * z * n, (size_t)r * m);
         const unsigned grid = (unsigned)std::min<size_t>((tot + kPB - 1) / kPB, 65536);
-        hipLaunchKernelGGL(dense_build_scales, dim3(grid), dim3(kPB), 0, st, n, m, z, r, p->rz,
-                           p->gamma, p->d_O, p->d_R, p->d_Z, p->d_B, g_dense.back().Os,
-                           g_dense.back().Rb);
-        e = hipGetLastError();
+        hipLaunchKernelGGL(dense_build_scales, dim3(grid), dim3(kPB), 0, st, n, Dn.kpad, m, z, r,
+                           p->rz, p->gamma, p->d_O, p->d_R, p->d_Z, p->d_B, Dn.Os, Dn.Rb);
+        if (e == hipSuccess)
+            e = hipGetLastError();
         if (e == hipSuccess)
             e = hipStreamSynchronize(st);
     }
@@ -1607,7 +1866,7 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
     const unsigned flip = pbvi->currentHorizon % 2;
     const int mode = g_pbvi_mode;
     const unsigned c = 1;
-    const unsigned *tr = pl.dOff, *be = pl.dOff + 4 * (c + 1);
+    const unsigned *tr = pl.dOff, *be = pl.dOff + (kVariants + 1) * (c + 1);
     hipEvent_t ev[5];
     for (auto &e : ev)
         hipEventCreate(&e);
@@ -1616,26 +1875,8 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
         const DenseWs &D = *Dp;
         const PbviInst &I = pl.host[0];
         const float *G = (flip == 0) ? I.G0 : I.G1;
-        const size_t rn = (size_t)D.r * D.n;
-        const unsigned cg = (unsigned)std::min<size_t>((rn + kPB - 1) / kPB, 65536);
-        const unsigned itiles = (D.r + kDT - 1) / kDT;
-        const dim3 sg(itiles * D.tiles, D.m * D.z);
         hipEventRecord(ev[0], st);
-        if (D.mode == 2)
-            hipLaunchKernelGGL(dense_convert<__bf16>, dim3(cg), dim3(kPB), 0, st, rn, G,
-                               static_cast<__bf16 *>(D.GI));
-        else
-            hipLaunchKernelGGL(dense_convert<float>, dim3(cg), dim3(kPB), 0, st, rn, G,
-                               static_cast<float *>(D.GI));
-        hipEventRecord(ev[1], st);
-        if (D.mode == 2)
-            hipLaunchKernelGGL(dense_score<__bf16>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z,
-                               itiles, static_cast<const __bf16 *>(D.U),
-                               static_cast<const __bf16 *>(D.GI), D.Os, D.partVal, D.partJ);
-        else
-            hipLaunchKernelGGL(dense_score<float>, sg, dim3(kPB), 0, st, D.n, D.r, D.m, D.z,
-                               itiles, static_cast<const float *>(D.U),
-                               static_cast<const float *>(D.GI), D.Os, D.partVal, D.partJ);
+        dense_launch_score(D, G, st, ev[1]);
         hipEventRecord(ev[2], st);
         hipLaunchKernelGGL(dense_action, dim3(D.r), dim3(kPB), 0, st, D.m, D.z, D.r, D.tiles,
                            D.Rb, D.partVal, D.partJ, I.jstar, I.act);
@@ -1658,18 +1899,12 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
     hipEventRecord(ev[1], st);
     const int v = jpt_slot(pl.host[0].jpt);
     const unsigned *sc = pl.dOff + (1 + v) * (c + 1);
-    dim3 g(pl.totSc[v]);
-    if (mode == 1) {
-        if (v == 0) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-        if (v == 1) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-        if (v == 2) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-    } else {
-        if (v == 0) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-        if (v == 1) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-        if (v == 2) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-    }
+    launch_score(pl.host[0].jpt, mode, dim3(pl.totSc[v]), pl.dInst, sc, c, 0u, st);
     hipEventRecord(ev[2], st);
-    hipLaunchKernelGGL(pbvi_action, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
+    if (mode == 1)
+        hipLaunchKernelGGL(pbvi_action<true>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
+    else
+        hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
     hipEventRecord(ev[3], st);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
     hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, 0u, flip);
@@ -1995,18 +2230,12 @@ int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlp
         {
             const unsigned c = 1;
             const int v = jpt_slot(J.jpt);
-            const unsigned *sc = pl.dOff + (1 + v) * (c + 1), *be = pl.dOff + 4 * (c + 1);
-            dim3 g(pl.totSc[v]);
-            if (g_pbvi_mode == 1) {
-                if (v == 0) hipLaunchKernelGGL((pbvi_score<true, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-                if (v == 1) hipLaunchKernelGGL((pbvi_score<true, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-                if (v == 2) hipLaunchKernelGGL((pbvi_score<true, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-            } else {
-                if (v == 0) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-                if (v == 1) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-                if (v == 2) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, pl.dInst, sc, c, 0u);
-            }
-            hipLaunchKernelGGL(pbvi_action, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
+            const unsigned *sc = pl.dOff + (1 + v) * (c + 1), *be = pl.dOff + (kVariants + 1) * (c + 1);
+            launch_score(J.jpt, g_pbvi_mode, dim3(pl.totSc[v]), pl.dInst, sc, c, 0u, st);
+            if (g_pbvi_mode == 1)
+                hipLaunchKernelGGL(pbvi_action<true>, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
+            else
+                hipLaunchKernelGGL(pbvi_action<false>, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
             auto matFn = (g_pbvi_mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
             hipLaunchKernelGGL(matFn, dim3(1), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, 0u, 0u);
         }

@@ -138,10 +138,14 @@ def sparse_values(P, G):
     return V
 
 
+@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES, ids=[s[0] for s in SHAPES])
-def test_fast_step_vs_oracle(oracle, name, params, horizon, rag):
+def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, variant):
     """From the same Gamma (the oracle's after t iterations), one fast GPU iteration chooses
-    the oracle's alphas except at near-ties."""
+    the oracle's alphas except at near-ties; for the tuned scoring kernel and for each forced
+    one (1 and 4 alphas per lane on the vector ALUs, 8 = the matrix-core kernel)."""
+    if variant != "tuned":
+        monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
     P = synth.pomdp_arrays(**_params(params))
     if rag:
         P = ragged(P, params["seed"])
@@ -171,11 +175,15 @@ def test_fast_step_vs_oracle(oracle, name, params, horizon, rag):
             b = ref["GammaPrime"][ok_rows].astype(np.float64)
             assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, np.max(np.abs(b)))
         del s
-    print("%s: fast mode, %d of %d alpha choices differ (all near-ties)" % (name, mismatch, total))
+    print("%s: fast mode (scoring %s), %d of %d alpha choices differ (all near-ties)"
+          % (name, variant, mismatch, total))
 
 
+@pytest.mark.parametrize("variant", ["tuned", "8"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES[:2], ids=[s[0] for s in SHAPES[:2]])
-def test_fast_run_values_vs_fp64(oracle, name, params, horizon, rag):
+def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag, variant):
+    if variant != "tuned":
+        monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
     P = synth.pomdp_arrays(**_params(params))
     if rag:
         P = ragged(P, params["seed"])
@@ -183,5 +191,6 @@ def test_fast_run_values_vs_fp64(oracle, name, params, horizon, rag):
     G64, _ = oracle.pbvi_run(P, horizon=horizon, dtype=np.float64)
     v, v64 = sparse_values(P, G), sparse_values(P, G64)
     err = np.max(np.abs(v - v64)) / max(1.0, np.max(np.abs(v64)))
-    print("%s: fast mode, max_b |V(b) - V_fp64(b)| / ||V|| = %.3g" % (name, err))
+    print("%s: fast mode (scoring %s), max_b |V(b) - V_fp64(b)| / ||V|| = %.3g"
+          % (name, variant, err))
     assert err < 1e-4

@@ -64,6 +64,8 @@ struct PbviInst {
     float *G0, *G1;        // Gamma / GammaPrime
     unsigned *pi;
     float *OT, *GT, *partVal;
+    int *SA;               // per-action successor rows SA[(a*n + s)*ns + l] = S[s,a,l]
+    float *TA, *RT, *OTr;  // TA likewise for T; RT[a*n + s] = R[s,a]; OTr[(a*z + o)*n + s'] = O[a,s',o]
     int *partJ, *jstar;
     float *aval;
     int *act;
@@ -134,6 +136,35 @@ __global__ __launch_bounds__(kPB) void pbvi_build_ot(unsigned n, unsigned ns, un
             v = __fmul_rn(O[((size_t)a * n + sp) * z + o], T[row + l]);
         }
         OT[x] = v;
+    }
+}
+
+// Action-major copies for the fast materialisation: one action's successor rows, rewards and
+// observation planes are then contiguous over states.
+__global__ __launch_bounds__(kPB) void pbvi_build_aux(unsigned n, unsigned ns, unsigned m,
+        unsigned z, const int *__restrict__ S, const float *__restrict__ T,
+        const float *__restrict__ R, const float *__restrict__ O, int *__restrict__ SA,
+        float *__restrict__ TA, float *__restrict__ RT, float *__restrict__ OTr)
+{
+    const size_t stride = (size_t)gridDim.x * kPB, x0 = blockIdx.x * (size_t)kPB + threadIdx.x;
+    const size_t rows = (size_t)m * n * ns;
+    for (size_t x = x0; x < rows; x += stride) {
+        const unsigned l = x % ns;
+        const size_t y = x / ns;
+        const unsigned s = y % n, a = y / n;
+        const size_t src = ((size_t)s * m + a) * ns + l;
+        SA[x] = S[src];
+        TA[x] = T[src];
+    }
+    for (size_t x = x0; x < (size_t)m * n; x += stride) {
+        const unsigned s = x % n, a = x / n;
+        RT[x] = R[(size_t)s * m + a];
+    }
+    for (size_t x = x0; x < (size_t)m * z * n; x += stride) {
+        const unsigned sp = x % n;
+        const size_t y = x / n;
+        const unsigned o = y % z, a = y / z;
+        OTr[x] = O[((size_t)a * n + sp) * z + o];
     }
 }
 
@@ -683,25 +714,29 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
             }
         }
     } else {
-        // h(sp) = sum_o O[a,sp,o] * Gamma[j*_o, sp] in LDS, then R + gamma*sum_l T*h(S)
+        // h(sp) = sum_o O[a,sp,o] * Gamma[j*_o, sp] in LDS, then R + gamma*sum_l T*h(S), all
+        // reads contiguous over states through the action-major copies
+        const float *Oa = P.OTr + (size_t)a * z * n;
         for (unsigned sp = t; sp < n; sp += kPB) {
-            const float *Os = P.O + ((size_t)a * n + sp) * z;
             float h = 0.0f;
             for (unsigned o = 0; o < z; o++)
-                h = __builtin_fmaf(Os[o], G[(size_t)js[o] * n + sp], h);
+                h = __builtin_fmaf(Oa[(size_t)o * n + sp], G[(size_t)js[o] * n + sp], h);
             row[sp] = h;
         }
         __syncthreads();
+        const int *Sa = P.SA + (size_t)a * n * ns;
+        const float *Ta = P.TA + (size_t)a * n * ns;
+        const float *Ra = P.RT + (size_t)a * n;
         for (unsigned s = t; s < n; s += kPB) {
-            const size_t rw = ((size_t)s * m + a) * ns;
+            const size_t rw = (size_t)s * ns;
             float acc = 0.0f;
             for (unsigned l = 0; l < ns; l++) {
-                const int sp = P.S[rw + l];
+                const int sp = Sa[rw + l];
                 if (sp < 0)
                     break;
-                acc = __builtin_fmaf(P.T[rw + l], row[sp], acc);
+                acc = __builtin_fmaf(Ta[rw + l], row[sp], acc);
             }
-            out[s] = __builtin_fmaf(gamma, acc, P.R[(size_t)s * m + a]);
+            out[s] = __builtin_fmaf(gamma, acc, Ra[s]);
         }
     }
     if (t == 0)
@@ -1064,6 +1099,9 @@ static size_t inst_ws_bytes(const nova::POMDP *p)
     const size_t rmz = (size_t)p->r * p->m * p->z;
     const unsigned tiles = tiles_for(p->r, 1);
     return align256(sizeof(float) * (size_t)p->m * p->z * p->n * p->ns) +
+           2 * align256(sizeof(float) * (size_t)p->m * p->n * p->ns) +
+           align256(sizeof(float) * (size_t)p->m * p->n) +
+           align256(sizeof(float) * (size_t)p->m * p->z * p->n) +
            align256(sizeof(float) * (size_t)p->n * p->r) +
            align256(sizeof(float) * tiles * rmz) + align256(sizeof(int) * tiles * rmz) +
            align256(sizeof(int) * rmz) + align256(sizeof(float) * (size_t)p->r * p->m) +
@@ -1078,6 +1116,14 @@ static void inst_ws(const nova::POMDP *p, char *base, PbviInst &I)
     char *c = base;
     I.OT = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->m * p->z * p->n * p->ns);
+    I.SA = reinterpret_cast<int *>(c);
+    c += align256(sizeof(float) * (size_t)p->m * p->n * p->ns);
+    I.TA = reinterpret_cast<float *>(c);
+    c += align256(sizeof(float) * (size_t)p->m * p->n * p->ns);
+    I.RT = reinterpret_cast<float *>(c);
+    c += align256(sizeof(float) * (size_t)p->m * p->n);
+    I.OTr = reinterpret_cast<float *>(c);
+    c += align256(sizeof(float) * (size_t)p->m * p->z * p->n);
     I.GT = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->n * p->r);
     I.partVal = reinterpret_cast<float *>(c);
@@ -1630,12 +1676,15 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
     return NOVA_SUCCESS;
 }
 
-static int build_ot(const nova::POMDP *p, float *OT, hipStream_t st)
+// Builds the Gamma-independent parts of an instance's workspace: OT and the action-major copies.
+static int build_ot(const nova::POMDP *p, const PbviInst &I, hipStream_t st)
 {
     const size_t total = (size_t)p->m * p->z * p->n * p->ns;
     const unsigned grid = (unsigned)std::min<size_t>((total + kPB - 1) / kPB, 65536);
     hipLaunchKernelGGL(pbvi_build_ot, dim3(grid), dim3(kPB), 0, st, p->n, p->ns, p->m, p->z,
-                       p->d_S, p->d_T, p->d_O, OT);
+                       p->d_S, p->d_T, p->d_O, I.OT);
+    hipLaunchKernelGGL(pbvi_build_aux, dim3(grid), dim3(kPB), 0, st, p->n, p->ns, p->m, p->z,
+                       p->d_S, p->d_T, p->d_R, p->d_O, I.SA, I.TA, I.RT, I.OTr);
     return hipGetLastError() == hipSuccess ? NOVA_SUCCESS : NOVA_ERROR_KERNEL_EXECUTION;
 }
 
@@ -1703,7 +1752,7 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     PbviInst I;
     fill_inst(pomdp, I, pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
               reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
-    rc = build_ot(pomdp, I.OT, st);
+    rc = build_ot(pomdp, I, st);
     if (rc != NOVA_SUCCESS)
         return hip_fail(hipGetLastError(), fn, "Failed to build the O*T products.", rc);
     if (dense_mode(g_pbvi_mode))
@@ -2151,7 +2200,7 @@ int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlp
     PbviInst I;
     if (rc == NOVA_SUCCESS && e == hipSuccess) {
         fill_inst(pomdp, I, GA, alphaNew, actNew, static_cast<char *>(ws), kNoHorizon);
-        rc = build_ot(pomdp, I.OT, st);
+        rc = build_ot(pomdp, I, st);
         if (rc == NOVA_SUCCESS)
             I.jpt = tune_jpt(I, static_cast<char *>(ws) + align256(inst_ws_bytes(pomdp)), st,
                              g_pbvi_mode);

@@ -106,12 +106,17 @@ def test_finite_differences_full_size():
     L0 = float((V64t * rho.double()).sum())
     skipped, kinked = 0, 0
     c = 0
+    attempts = 0
     while c < 64:
+        attempts += 1
+        assert attempts <= 400, "too many coordinates rejected (%d at action ties, %d kinked)" % (
+            skipped, kinked)
         # V* has a kink wherever two actions tie; a central difference whose step moves some
         # Q across a tie averages two one-sided slopes instead of measuring the derivative.
         # Coordinates whose step reaches a tie at s or at a predecessor of s are redrawn before
         # solving; ties further upstream show as one-sided slopes that disagree, and those
-        # coordinates are redrawn too (for a smooth L they agree to about h * L''/L' ~ 1e-7).
+        # coordinates are redrawn too (for a smooth L they agree to about h * L''/L' ~ 1e-7 relative,
+        # plus the solves' rounding floor).
         s = int(rng.integers(n))
         on = c % 2 == 0
         a = int(pic[s]) if on else int((pic[s] + 1 + rng.integers(m - 1)) % m)
@@ -139,13 +144,18 @@ def test_finite_differences_full_size():
             Lp, Lm = L(R, Tp), L(R, Tm)
             an = float(dT[s, a, i])
         fwd, bwd = (Lp - L0) / dp, (L0 - Lm) / dm
-        if abs(fwd - bwd) > 1e-4 * max(abs(fwd), abs(bwd), 1e-6):
+        # the FP64 solves stop at their rounding floor (L to ~1e-11), so one-sided slopes carry
+        # ~1e-7 of noise; a kink inside the step shows as a much larger gap
+        if abs(fwd - bwd) > 5e-4 * max(abs(fwd), abs(bwd)) + 2e-7:
             kinked += 1
+            print("  attempt %d: one-sided slopes apart (%.6g vs %.6g), redrawn" % (
+                attempts, fwd, bwd), flush=True)
             continue
         fd = (Lp - Lm) / (dp + dm)
         scale = max(abs(fd), abs(an), 1e-12)
         errs.append(abs(fd - an) / scale if max(abs(fd), abs(an)) > 1e-9 else abs(fd - an))
         rows.append((c, "R" if c < 32 else "T", s, a, on, fd, an, errs[-1]))
+        print("  coordinate %d done (error %.3g)" % (c, errs[-1]), flush=True)
         c += 1
     for row in rows:
         if row[-1] > 1e-3:

@@ -28,7 +28,8 @@
  * The reference has no residual-stopped VI and no gradients; these follow the mathematics:
  * V* = max_a R + gamma*P_a V*; at the greedy policy pi, L = rho.V* has
  * dL/dR[s,a] = lambda(s)[a = pi(s)] and dL/dT[s,a,i] = gamma*lambda(s)*V*(S[s,a,i])[a = pi(s)]
- * with (I - gamma P_pi)^T lambda = rho. Tests pin them with central finite differences. */
+ * with (I - gamma P_pi)^T lambda = rho. tests/test_oracle.py pins them against exact policy
+ * iteration, a dense solve of the adjoint system and central differences of the exact V*. */
 
 /* Sweeps V <- max_a(R + gamma*T.V) in FP64 (Jacobi, one buffer pair, same tie rule as the
  * sweep) until max|V' - V| <= tol or maxIter sweeps. Returns the sweeps done; pi is the greedy

@@ -161,6 +161,46 @@ class Oracle:
             n, r, _p(G), _p(pi), _p(b), ct.byref(Vb), ct.byref(a))
         return Vb.value, a.value, arg
 
+    # ---- Perseus (src/pomdp/algorithms/pomdp_perseus_cpu.cpp:183-429)
+    def perseus_run(self, P, rand, rand_max=2147483647, dtype=np.float32):
+        """nova's randomized Perseus loop over orc_perseus_step: `horizon` rounds, each stepping
+        until no belief lost value; beliefs drawn as (unsigned)((float)rand() / (float)RAND_MAX
+        * (float)rTilde) (:339-341), clamped below rTilde. rand() returns the next libc value.
+        Returns the final (Gamma, pi) as get_policy would (:432-458)."""
+        S, T, O, R = (_c(P["S"], np.int32), _c(P["T"], np.float32), _c(P["O"], np.float32),
+                      _c(P["R"], np.float32))
+        Z, B = _c(P["Z"], np.int32), _c(P["B"], np.float32)
+        n, m, ns = S.shape
+        z = O.shape[2]
+        r, rz = Z.shape
+        G = [_c(P["Gamma0"], dtype).copy(), _c(P["Gamma0"], dtype).copy()]
+        pis = [np.zeros(r, np.uint32), np.zeros(r, np.uint32)]
+        counts = [0, 0]
+        BTilde = np.arange(r, dtype=np.uint32)
+        rTilde = ct.c_uint(r)
+        alpha = np.empty(2 * n, dtype)
+        step = getattr(self.lib, "orc_perseus_step_" + self._sfx(dtype))
+        horizon = 0
+        while horizon < P["horizon"]:
+            cur, nxt = (0, 1) if horizon % 2 == 0 else (1, 0)
+            x = np.float32(rand()) / np.float32(rand_max) * np.float32(rTilde.value)
+            t = min(int(x), rTilde.value - 1)
+            rGp = ct.c_uint(counts[nxt])
+            rc = step(n, ns, m, z, r, rz, ct.c_float(P["gamma"]), _p(S), _p(T), _p(O), _p(R),
+                      _p(Z), _p(B), _p(G[cur]), counts[cur], _p(pis[cur]), _p(G[nxt]),
+                      ct.byref(rGp), _p(pis[nxt]), int(BTilde[t]), _p(BTilde), ct.byref(rTilde),
+                      _p(alpha))
+            if rc != 0:
+                raise RuntimeError("Perseus step failed with code %d" % rc)
+            counts[nxt] = rGp.value
+            if rTilde.value == 0:
+                horizon += 1
+                counts[cur] = 0
+                rTilde.value = r
+                BTilde[:] = np.arange(r, dtype=np.uint32)
+        out = 0 if horizon % 2 == 0 else 1
+        return G[out][:counts[out]].copy(), pis[out][:counts[out]].copy()
+
     # ---- FP64 references for the differentiable solve
     def vi_solve_f64(self, S, T, R, gamma, V0=None, max_iter=100000, tol=1e-12):
         n, m, ns = S.shape

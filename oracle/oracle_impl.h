@@ -10,6 +10,17 @@
  *
  * Every function cites the reference file:line it follows. Where the reference has undefined
  * behaviour the restatement picks a defined value and says so at that line.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   orc_vi_sweep, orc_vi_run      nova's known answers; bit-for-bit vs nova's build (seeded,
+ *                                 ragged); FP64 twin vs a dense numpy Bellman operator
+ *   orc_pbvi_backup, orc_pbvi_run bit-for-bit vs nova's build; nova's closed forms; FP64 twin
+ *                                 vs a dense enumeration of every argmax
+ *   orc_belief_update, orc_prob_obs  bit-for-bit vs nova's build; normalisation invariants
+ *   orc_expand_distinct           bit-for-bit vs nova's build
+ *   orc_alpha_value               nova's known answers
+ *   orc_perseus_step, orc_sparse_vb  driven by nova's loop on the same libc rand() stream,
+ *                                 bit-for-bit vs nova's build
  */
 
 #define ORC_SENT_LO ((REAL)-1e10)   /* include/nova/constants.h:34 (FLT_MIN = -1e+10) */

@@ -179,3 +179,102 @@ def test_alpha_value_known_answers(oracle):
         assert abs(Vb - v) < 1e-5 and act == a
     Vb, _, _ = oracle.alpha_value(G, pi, np.array([0.5, 0.5], np.float32))
     assert abs(Vb - 0.5) < 1e-5
+
+
+def test_perseus_oracle_bitwise_vs_reference_build(oracle, reflib):
+    """The oracle's Perseus step, driven by nova's loop over the same libc rand() stream, equals
+    nova's own Perseus (oracle/_ref) bit for bit."""
+    import ctypes as ct
+    from nova_amd import _abi
+    libc = ct.CDLL("libc.so.6")
+    for params in [dict(n=25, m=3, z=3, ns=4, r=20, rz=5, seed=43, horizon=40),
+                   dict(n=80, m=4, z=6, ns=5, r=60, rz=8, seed=44, horizon=30)]:
+        P = synth.pomdp_arrays(gamma=0.9, **params)
+        libc.srand(77)
+        G, pi = oracle.perseus_run(P, libc.rand)
+        pr, keep = reflib._pomdp(P)
+        g0 = np.ascontiguousarray(P["Gamma0"], np.float32)
+        per = _abi.NovaPOMDPPerseusCPU()
+        per.GammaInitial = ct.cast(g0.ctypes.data, _abi.c_float_p)
+        pol = _abi.NovaPOMDPAlphaVectors()
+        libc.srand(77)
+        assert reflib.lib.pomdp_perseus_execute_cpu(ct.byref(pr), ct.byref(per),
+                                                    ct.byref(pol)) == 0
+        n = P["S"].shape[0]
+        Gr = np.ctypeslib.as_array(pol.Gamma, (pol.r * n,)).reshape(pol.r, n).copy()
+        pir = np.ctypeslib.as_array(pol.pi, (pol.r,)).copy()
+        reflib.lib.pomdp_alpha_vectors_uninitialize(ct.byref(pol))
+        assert G.shape == Gr.shape and G.shape[0] > 3
+        assert np.array_equal(bits(G), bits(Gr)) and np.array_equal(pi, pir)
+
+
+def _dense_model(w):
+    S, T = w["S"], w["T"].astype(np.float64)
+    n, m, ns = S.shape
+    P = np.zeros((m, n, n))
+    for s in range(n):
+        for a in range(m):
+            for i in range(ns):
+                P[a, s, S[s, a, i]] += T[s, a, i]
+    return P
+
+
+def _policy_iteration(P, R, g):
+    """V* and pi* by exact policy iteration (dense linear solves)."""
+    m, n, _ = P.shape
+    pi = np.zeros(n, int)
+    for _ in range(100):
+        Ppi = P[pi, np.arange(n)]
+        V = np.linalg.solve(np.eye(n) - g * Ppi, R[np.arange(n), pi])
+        Q = R.T + g * P @ V
+        new = Q.argmax(axis=0)
+        if np.array_equal(new, pi):
+            return V, pi
+        pi = new
+    raise AssertionError("policy iteration did not settle")
+
+
+def test_fp64_solve_adjoint_and_gradients_vs_dense_algebra(oracle):
+    """orc_vi_solve_f64 against exact policy iteration, orc_adjoint_f64 against a dense solve
+    of (I - gamma P_pi)^T lambda = rho, and orc_vi_grads_f64 against central differences of
+    the exact (policy-iteration) V* on every R coordinate and 40 T coordinates."""
+    w = synth.mdp_arrays(n=60, m=3, ns=5, seed=21, gamma=0.9)
+    g = float(w["gamma"])
+    S, T, R = w["S"], w["T"], w["R"].astype(np.float64)
+    n, m, ns = S.shape
+    P = _dense_model(w)
+    Vx, pix = _policy_iteration(P, R, g)
+    V, pi, it, res = oracle.vi_solve_f64(S, T, w["R"], w["gamma"], tol=1e-13)
+    assert res <= 1e-13
+    np.testing.assert_allclose(V, Vx, rtol=0, atol=1e-11)
+    assert np.array_equal(pi.astype(int), pix)
+    rho = np.random.default_rng(3).uniform(0, 1, n)
+    lam = oracle.adjoint_f64(S, T, w["gamma"], pi, rho)
+    Ppi = P[pix, np.arange(n)]
+    lamx = np.linalg.solve((np.eye(n) - g * Ppi).T, rho)
+    np.testing.assert_allclose(lam, lamx, rtol=1e-11, atol=1e-11)
+    dR, dT = oracle.vi_grads_f64(S, w["gamma"], pi, lam, V)
+
+    def L(Rm, Tm):
+        wm = dict(w, T=Tm)
+        Vm, pim = _policy_iteration(_dense_model(wm), Rm, g)
+        assert np.array_equal(pim, pix)   # the step stays inside the policy's region
+        return rho @ Vm
+
+    h = 1e-6
+    fdR = np.empty((n, m))
+    for s in range(n):
+        for a in range(m):
+            Rp, Rm_ = R.copy(), R.copy()
+            Rp[s, a] += h
+            Rm_[s, a] -= h
+            fdR[s, a] = (L(Rp, T.astype(np.float64)) - L(Rm_, T.astype(np.float64))) / (2 * h)
+    np.testing.assert_allclose(dR, fdR, rtol=1e-6, atol=1e-7)
+    rng = np.random.default_rng(4)
+    for _ in range(40):
+        s, a, i = int(rng.integers(n)), int(rng.integers(m)), int(rng.integers(ns))
+        Tp, Tm = T.astype(np.float64), T.astype(np.float64)
+        Tp[s, a, i] += h
+        Tm[s, a, i] -= h
+        fd = (L(R, Tp) - L(R, Tm)) / (2 * h)
+        assert abs(dT[s, a, i] - fd) <= 1e-6 * max(1.0, abs(fd))

@@ -56,7 +56,6 @@ struct PbviInst {
     unsigned jpt;          // alphas per lane of the scoring variant chosen for this instance
     int useLds;            // materialisation stages n floats in LDS
     float gamma;
-    unsigned pad;
     const int *S;
     const float *T, *O, *R;
     const int *Z;
@@ -65,6 +64,10 @@ struct PbviInst {
     unsigned *pi;
     float *OT, *GT, *partVal;
     int *SA;               // per-action successor rows SA[(a*n + s)*ns + l] = S[s,a,l]
+    int *pairSp;           // fast mode: valid (k, l) pairs of belief i under action a,
+    float *pairW;          //   pairSp/pairW[(i*m + a)*kmax + u] = S[s_k,a,l], gamma*b_k*T[s_k,a,l]
+    unsigned *pairCnt;     //   pairCnt[i*m + a] = number of pairs
+    unsigned kmax;         //   rz*ns (the lists exist when it is at most kUMax)
     float *TA, *RT, *OTr;  // TA likewise for T; RT[a*n + s] = R[s,a]; OTr[(a*z + o)*n + s'] = O[a,s',o]
     int *partJ, *jstar;
     float *aval;
@@ -166,6 +169,37 @@ __global__ __launch_bounds__(kPB) void pbvi_build_aux(unsigned n, unsigned ns, u
         const unsigned o = y % z, a = y / z;
         OTr[x] = O[((size_t)a * n + sp) * z + o];
     }
+}
+
+// The valid (k, l) pairs of every (belief, action) with their Gamma-independent factor
+// gamma * b_k * T[s_k,a,l], in support order (fast-mode scoring reads these instead of
+// compacting the support in every workgroup).
+__global__ __launch_bounds__(kPB) void pbvi_build_pairs(unsigned r, unsigned m, unsigned ns,
+        unsigned rz, unsigned kmax, float gamma, const int *__restrict__ S,
+        const float *__restrict__ T, const int *__restrict__ Z, const float *__restrict__ B,
+        int *__restrict__ pairSp, float *__restrict__ pairW, unsigned *__restrict__ pairCnt)
+{
+    const size_t ia = blockIdx.x * (size_t)kPB + threadIdx.x;
+    if (ia >= (size_t)r * m)
+        return;
+    const unsigned i = ia / m, a = ia % m;
+    unsigned cnt = 0;
+    for (unsigned k = 0; k < rz; k++) {
+        const int s = Z[(size_t)i * rz + k];
+        if (s < 0)
+            break;
+        const float gb = gamma * B[(size_t)i * rz + k];
+        const size_t row = ((size_t)s * m + a) * ns;
+        for (unsigned l = 0; l < ns; l++) {
+            const int sp = S[row + l];
+            if (sp < 0)
+                break;
+            pairSp[ia * kmax + cnt] = sp;
+            pairW[ia * kmax + cnt] = gb * T[row + l];
+            cnt++;
+        }
+    }
+    pairCnt[ia] = cnt;
 }
 
 // ----------------------------------------------------------------------- 1. transpose
@@ -283,10 +317,23 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
-    build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);
-    const int kEnd = L.kEnd, uEnd = L.uEnd;
+    int kEnd, uEnd;
+    if (EXACT) {
+        build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);
+        kEnd = L.kEnd;
+        uEnd = L.uEnd;
+    } else {
+        // the precomputed pair list of (i, a): successors to LDS, factors read when staged
+        uEnd = (int)P.pairCnt[ia];
+        kEnd = 0;
+        for (int u = t; u < uEnd; u += kPB)
+            L.spU[u] = P.pairSp[(size_t)ia * P.kmax + u];
+        __syncthreads();
+    }
     const float *__restrict__ GT = P.GT;
     const float *__restrict__ OT = P.OT;
+    const float *__restrict__ pw = EXACT ? nullptr : P.pairW + (size_t)ia * P.kmax;
+    const float *__restrict__ Oa = P.O + (size_t)a * n * z;
 
     int jq[JPT];
     bool live[JPT];
@@ -298,21 +345,15 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
 
     for (unsigned o0 = 0; o0 < z; o0 += kZC) {
         const int zc = min((int)kZC, (int)(z - o0));
-        // stage the chunk's factors: exact -> O*T; fast -> gamma*B_k*O*T
+        // stage the chunk's factors: exact -> O*T; fast -> (gamma*b_k*T) * O
         for (int x = t; x < uEnd * kZC; x += kPB) {
             const int u = x / kZC, oo = x % kZC;
             float v = 0.0f;
             if (oo < zc) {
-                v = OT[((size_t)(a * z + o0 + oo) * n) * ns + L.otU[u]];
-                if (!EXACT) {
-                    // the support entry k of pair u: startK[k] <= u < startK[k+1]
-                    int lo = 0, hi = kEnd - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) / 2;
-                        if (L.startK[mid] <= u) lo = mid; else hi = mid - 1;
-                    }
-                    v = gamma * L.bK[lo] * v;
-                }
+                if (EXACT)
+                    v = OT[((size_t)(a * z + o0 + oo) * n) * ns + L.otU[u]];
+                else
+                    v = pw[u] * Oa[(size_t)L.spU[u] * z + o0 + oo];
             }
             w[u * kZC + oo] = v;
         }
@@ -326,7 +367,9 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
             for (int q = 0; q < JPT; q++)
                 val[oo][q] = 0.0f;
 
-        for (int k = 0; k < kEnd; k++) {
+        // exact: per support entry k, its pairs (rounded as the reference); fast: all pairs
+        const int kLoops = EXACT ? kEnd : (uEnd > 0 ? 1 : 0);
+        for (int k = 0; k < kLoops; k++) {
             if (EXACT) {
 #pragma unroll
                 for (int oo = 0; oo < kZC; oo++)
@@ -334,8 +377,8 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
                     for (int q = 0; q < JPT; q++)
                         vtk[oo % (EXACT ? kZC : 1)][q] = 0.0f;
             }
-            const int u1 = L.startK[k + 1];
-            for (int u = L.startK[k]; u < u1; u++) {
+            const int u1 = EXACT ? L.startK[k + 1] : uEnd;
+            for (int u = EXACT ? L.startK[k] : 0; u < u1; u++) {
                 const size_t rowOff = (size_t)L.spU[u] * r;
                 float g[JPT];
 #pragma unroll
@@ -430,9 +473,8 @@ constexpr int kWChunk = 256;         // support pairs per staged W chunk
 __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
-    __shared__ SupportLds L;
+    __shared__ int spU[kUMax];                // successor of each pair
     __shared__ float w[kWChunk * 32];         // [u][o] for the current chunk and 32 observations
-    __shared__ int scratch[kKMax + 1];
     __shared__ float redV[kPB / 64][32];
     __shared__ int redJ[kPB / 64][32];
 
@@ -449,10 +491,14 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
-    build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);
-    const int kEnd = L.kEnd, uEnd = L.uEnd;
+    // the precomputed pair list of (i, a): successors to LDS, factors read when staged
+    const int uEnd = (int)P.pairCnt[ia];
+    for (int u = t; u < uEnd; u += kPB)
+        spU[u] = P.pairSp[(size_t)ia * P.kmax + u];
+    __syncthreads();
     const float *__restrict__ GT = P.GT;
-    const float *__restrict__ OT = P.OT;
+    const float *__restrict__ pw = P.pairW + (size_t)ia * P.kmax;
+    const float *__restrict__ Oa = P.O + (size_t)a * n * z;
     const unsigned jw = tile * kMTile + wave * 128;     // the wave's first alpha
 
     for (unsigned o0 = 0; o0 < z; o0 += 32) {
@@ -470,15 +516,8 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
             for (int x = t; x < cPad * 32; x += kPB) {
                 const int u = c0 + x / 32, oo = x % 32;
                 float v = 0.0f;
-                if (u < uEnd && oo < zc) {
-                    // the support entry k of pair u: startK[k] <= u < startK[k+1]
-                    int lo = 0, hi = kEnd - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) / 2;
-                        if (L.startK[mid] <= u) lo = mid; else hi = mid - 1;
-                    }
-                    v = gamma * L.bK[lo] * OT[((size_t)(a * z + o0 + oo) * n) * ns + L.otU[u]];
-                }
+                if (u < uEnd && oo < zc)
+                    v = pw[u] * Oa[(size_t)spU[u] * z + o0 + oo];
                 w[x] = v;
             }
             __syncthreads();
@@ -486,7 +525,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
             for (int u0 = 0; u0 < cPad; u0 += 2) {
                 const int uu = u0 + (int)(lane / 32);
                 const float fa = w[uu * 32 + lane % 32];
-                const int sp = (c0 + uu < uEnd) ? L.spU[c0 + uu] : 0;
+                const int sp = (c0 + uu < uEnd) ? spU[c0 + uu] : 0;
                 const float *row = GT + (size_t)sp * r;
                 float fb[4];
 #pragma unroll
@@ -1093,12 +1132,26 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 static unsigned tile_width(unsigned jpt) { return jpt == 8 ? (unsigned)kMTile : kPB * jpt; }
 static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
+static bool pbvi_shape_ok(const nova::POMDP *p)
+{
+    return p->rz <= (unsigned)kKMax && (size_t)p->rz * p->ns <= (size_t)kUMax;
+}
+
+// Bytes of the fast-mode pair lists (none for shapes the sparse kernels do not take).
+static size_t pair_bytes(const nova::POMDP *p)
+{
+    if (!pbvi_shape_ok(p))
+        return 0;
+    const size_t lists = (size_t)p->r * p->m * p->rz * p->ns;
+    return 2 * align256(sizeof(float) * lists) + align256(sizeof(unsigned) * (size_t)p->r * p->m);
+}
+
 // Bytes of one instance's workspace (tiles sized for the widest variant, jpt = 1).
 static size_t inst_ws_bytes(const nova::POMDP *p)
 {
     const size_t rmz = (size_t)p->r * p->m * p->z;
     const unsigned tiles = tiles_for(p->r, 1);
-    return align256(sizeof(float) * (size_t)p->m * p->z * p->n * p->ns) +
+    return pair_bytes(p) + align256(sizeof(float) * (size_t)p->m * p->z * p->n * p->ns) +
            2 * align256(sizeof(float) * (size_t)p->m * p->n * p->ns) +
            align256(sizeof(float) * (size_t)p->m * p->n) +
            align256(sizeof(float) * (size_t)p->m * p->z * p->n) +
@@ -1126,6 +1179,16 @@ static void inst_ws(const nova::POMDP *p, char *base, PbviInst &I)
     c += align256(sizeof(float) * (size_t)p->m * p->z * p->n);
     I.GT = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->n * p->r);
+    I.kmax = p->rz * p->ns;
+    if (pbvi_shape_ok(p)) {
+        const size_t lists = (size_t)p->r * p->m * p->rz * p->ns;
+        I.pairSp = reinterpret_cast<int *>(c);
+        c += align256(sizeof(float) * lists);
+        I.pairW = reinterpret_cast<float *>(c);
+        c += align256(sizeof(float) * lists);
+        I.pairCnt = reinterpret_cast<unsigned *>(c);
+        c += align256(sizeof(unsigned) * (size_t)p->r * p->m);
+    }
     I.partVal = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * tiles * rmz);
     I.partJ = reinterpret_cast<int *>(c);
@@ -1135,11 +1198,6 @@ static void inst_ws(const nova::POMDP *p, char *base, PbviInst &I)
     I.aval = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->r * p->m);
     I.act = reinterpret_cast<int *>(c);
-}
-
-static bool pbvi_shape_ok(const nova::POMDP *p)
-{
-    return p->rz <= (unsigned)kKMax && (size_t)p->rz * p->ns <= (size_t)kUMax;
 }
 
 // Whether n floats can be staged in LDS by the materialize kernels (their dynamic LDS must be
@@ -1685,6 +1743,12 @@ static int build_ot(const nova::POMDP *p, const PbviInst &I, hipStream_t st)
                        p->d_S, p->d_T, p->d_O, I.OT);
     hipLaunchKernelGGL(pbvi_build_aux, dim3(grid), dim3(kPB), 0, st, p->n, p->ns, p->m, p->z,
                        p->d_S, p->d_T, p->d_R, p->d_O, I.SA, I.TA, I.RT, I.OTr);
+    if (I.pairSp != nullptr) {
+        const size_t rm = (size_t)p->r * p->m;
+        hipLaunchKernelGGL(pbvi_build_pairs, dim3((unsigned)((rm + kPB - 1) / kPB)), dim3(kPB), 0,
+                           st, p->r, p->m, p->ns, p->rz, I.kmax, p->gamma, p->d_S, p->d_T,
+                           p->d_Z, p->d_B, I.pairSp, I.pairW, I.pairCnt);
+    }
     return hipGetLastError() == hipSuccess ? NOVA_SUCCESS : NOVA_ERROR_KERNEL_EXECUTION;
 }
 
@@ -2270,6 +2334,11 @@ int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlp
         J.pi = actNew;
         J.Z = pomdp->d_Z + (size_t)bIndex * rz;
         J.B = pomdp->d_B + (size_t)bIndex * rz;
+        if (J.pairSp != nullptr) {
+            J.pairSp += (size_t)bIndex * m * J.kmax;
+            J.pairW += (size_t)bIndex * m * J.kmax;
+            J.pairCnt += (size_t)bIndex * m;
+        }
         pl.host.assign(1, J);
         rc = plan_upload(pl, planMem, st, fn);
         if (rc != NOVA_SUCCESS)

@@ -50,6 +50,11 @@
 namespace nova_amd {
 
 // One problem of a (possibly single-element) batch, as the kernels see it.
+// Pointers the kernels read out of descriptors are generic; casting them to the global address
+// space lets the compiler emit global_load/global_store instead of flat accesses (which also
+// wait on the LDS counter).
+#define NOVA_GLOBAL __attribute__((address_space(1)))
+
 struct PbviInst {
     unsigned n, ns, m, z, r, rz, tiles, horizon;
     unsigned rb;           // beliefs backed up (PBVI: r; Perseus: 1). r counts the alphas.
@@ -330,10 +335,10 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
             L.spU[u] = P.pairSp[(size_t)ia * P.kmax + u];
         __syncthreads();
     }
-    const float *__restrict__ GT = P.GT;
-    const float *__restrict__ OT = P.OT;
-    const float *__restrict__ pw = EXACT ? nullptr : P.pairW + (size_t)ia * P.kmax;
-    const float *__restrict__ Oa = P.O + (size_t)a * n * z;
+    const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
+    const NOVA_GLOBAL float *OT = (const NOVA_GLOBAL float *)P.OT;
+    const NOVA_GLOBAL float *pw = EXACT ? nullptr : (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
 
     int jq[JPT];
     bool live[JPT];
@@ -491,15 +496,24 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
-    // the precomputed pair list of (i, a): successors to LDS, factors read when staged
-    const int uEnd = (int)P.pairCnt[ia];
-    for (int u = t; u < uEnd; u += kPB)
-        spU[u] = P.pairSp[(size_t)ia * P.kmax + u];
+    // the precomputed pair list of (i, a): successors to LDS (padded with state 0 to a multiple
+    // of 16 pairs, whose factors are 0), factors read when staged
+    const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
+    const int uEnd16 = (uEnd + 15) & ~15;
+    const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
+    for (int u = t; u < uEnd16; u += kPB)
+        spU[u] = (u < uEnd) ? pSp[u] : 0;
     __syncthreads();
-    const float *__restrict__ GT = P.GT;
-    const float *__restrict__ pw = P.pairW + (size_t)ia * P.kmax;
-    const float *__restrict__ Oa = P.O + (size_t)a * n * z;
+    const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
+    const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
     const unsigned jw = tile * kMTile + wave * 128;     // the wave's first alpha
+    // this lane's four columns, clamped into the row: columns past r only feed C columns the
+    // epilogue ignores
+    unsigned jc[4];
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+        jc[y] = min(jw + y * 32 + lane % 32, r - 1);
 
     for (unsigned o0 = 0; o0 < z; o0 += 32) {
         const int zc = min(32, (int)(z - o0));
@@ -510,8 +524,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
             for (int v = 0; v < 16; v++)
                 acc[y][v] = 0.0f;
         for (int c0 = 0; c0 < uEnd; c0 += kWChunk) {
-            const int cn = min(kWChunk, uEnd - c0);
-            const int cPad = (cn + 1) & ~1;
+            const int cPad = min(kWChunk, uEnd16 - c0);     // a multiple of 16
             __syncthreads();
             for (int x = t; x < cPad * 32; x += kPB) {
                 const int u = c0 + x / 32, oo = x % 32;
@@ -521,21 +534,44 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
                 w[x] = v;
             }
             __syncthreads();
-#pragma unroll 4
-            for (int u0 = 0; u0 < cPad; u0 += 2) {
-                const int uu = u0 + (int)(lane / 32);
-                const float fa = w[uu * 32 + lane % 32];
-                const int sp = (c0 + uu < uEnd) ? spU[c0 + uu] : 0;
-                const float *row = GT + (size_t)sp * r;
-                float fb[4];
+            // four pair-steps (8 pairs) per trip; the B operands of the next trip are loaded
+            // before this trip's 16 MFMAs, so their latency hides behind the matrix work
+            float fb[4][4], nb[4][4];
+            auto loadB = [&](int u0, float (&dst)[4][4]) {
 #pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    const unsigned j = jw + y * 32 + lane % 32;
-                    fb[y] = (j < r) ? row[j] : 0.0f;
+                for (int q = 0; q < 4; q++) {
+                    const int uu = u0 + 2 * q + (int)(lane / 32);
+                    const NOVA_GLOBAL float *row = GT + (size_t)spU[c0 + uu] * r;
+#pragma unroll
+                    for (int y = 0; y < 4; y++)
+                        dst[q][y] = row[jc[y]];
                 }
+            };
+            auto mfmaTrip = [&](int u0, const float (&b)[4][4]) {
+                float fa[4];
 #pragma unroll
-                for (int y = 0; y < 4; y++)
-                    acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa, fb[y], acc[y], 0, 0, 0);
+                for (int q = 0; q < 4; q++)
+                    fa[q] = w[(u0 + 2 * q + (int)(lane / 32)) * 32 + lane % 32];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int y = 0; y < 4; y++)
+                        acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[q], b[q][y], acc[y], 0, 0, 0);
+            };
+            // ping-pong between two register sets: trip t's operands were requested one trip
+            // earlier. The loads carry no conditions (the last prefetch re-reads pairs already
+            // staged), so the compiler can count the outstanding ones exactly.
+            // sched_barrier keeps the scheduler from sinking the prefetch below the MFMAs.
+            loadB(0, fb);
+            for (int u0 = 0; u0 < cPad; u0 += 16) {
+                loadB(u0 + 8, nb);
+                __builtin_amdgcn_sched_barrier(0);
+                mfmaTrip(u0, fb);
+                __builtin_amdgcn_sched_barrier(0);
+                loadB(min(u0 + 16, cPad - 8), fb);
+                __builtin_amdgcn_sched_barrier(0);
+                mfmaTrip(u0 + 8, nb);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         // per row (observation): first maximiser over the wave's 128 alphas, then the 4 waves
@@ -755,17 +791,18 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
     } else {
         // h(sp) = sum_o O[a,sp,o] * Gamma[j*_o, sp] in LDS, then R + gamma*sum_l T*h(S), all
         // reads contiguous over states through the action-major copies
-        const float *Oa = P.OTr + (size_t)a * z * n;
+        const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.OTr + (size_t)a * z * n;
+        const NOVA_GLOBAL float *Gg = (const NOVA_GLOBAL float *)G;
         for (unsigned sp = t; sp < n; sp += kPB) {
             float h = 0.0f;
             for (unsigned o = 0; o < z; o++)
-                h = __builtin_fmaf(Oa[(size_t)o * n + sp], G[(size_t)js[o] * n + sp], h);
+                h = __builtin_fmaf(Oa[(size_t)o * n + sp], Gg[(size_t)js[o] * n + sp], h);
             row[sp] = h;
         }
         __syncthreads();
-        const int *Sa = P.SA + (size_t)a * n * ns;
-        const float *Ta = P.TA + (size_t)a * n * ns;
-        const float *Ra = P.RT + (size_t)a * n;
+        const NOVA_GLOBAL int *Sa = (const NOVA_GLOBAL int *)P.SA + (size_t)a * n * ns;
+        const NOVA_GLOBAL float *Ta = (const NOVA_GLOBAL float *)P.TA + (size_t)a * n * ns;
+        const NOVA_GLOBAL float *Ra = (const NOVA_GLOBAL float *)P.RT + (size_t)a * n;
         for (unsigned s = t; s < n; s += kPB) {
             const size_t rw = (size_t)s * ns;
             float acc = 0.0f;
@@ -775,7 +812,7 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
                     break;
                 acc = __builtin_fmaf(Ta[rw + l], row[sp], acc);
             }
-            out[s] = __builtin_fmaf(gamma, acc, Ra[s]);
+            ((NOVA_GLOBAL float *)out)[s] = __builtin_fmaf(gamma, acc, Ra[s]);
         }
     }
     if (t == 0)

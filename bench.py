@@ -166,6 +166,7 @@ def bench_pbvi(args):
     p.initialize_gpu()
     s = POMDPPBVIGPU(p, GammaInitial=P["Gamma0"])
     stages = s.profile()                      # warm-up iteration, per-stage times
+    variant = s.scoring_variant()
     ms = s.run(iters)                         # one full solve's iterations
     t0 = time.perf_counter()
     s.run(iters)
@@ -176,10 +177,11 @@ def bench_pbvi(args):
     out = {"workload": "pbvi_n20000", "backups_per_s": r * iters / wall,
            "ms_per_iteration": ms / iters, "stage_ms": dict(zip(
                ["transpose", "score", "action", "materialize"], stages)),
-           "roofline": {"bound": "valu_fp32", "kernel": "pbvi_score", "achieved": achieved,
-                        "peak": 157.3, "unit": "TFLOP/s", "frac": achieved / 157.3,
-                        "peak_nonpacked_issue": 78.6,
-                        "algorithmic_flops_per_launch": flops}}
+           "roofline": {"bound": "mfma_fp32" if variant == 8 else "valu_fp32",
+                        "kernel": "pbvi_score_mfma" if variant == 8 else
+                                  "pbvi_score<fast,%d>" % variant,
+                        "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": achieved / 157.3, "algorithmic_flops_per_launch": flops}}
     del s
     p.uninitialize_gpu()
     if not args.no_cpu:

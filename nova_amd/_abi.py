@@ -201,6 +201,7 @@ EXTRA_SIGNATURES = {
     "pomdp_expand_random_seeded_gpu": [_POMDP, U, U, _P(ct.c_uint), _P(c_float_p)],
     "nova_amd_free_floats": [c_float_p],
     "pomdp_expand_distinct_beliefs_gpu": [_POMDP],
+    "pomdp_pbvi_scoring_variant_gpu": [_PBG, _P(ct.c_uint)],
     "pomdp_perseus_execute_gpu": [_POMDP, _PER, _PAV],
     "mdp_vi_solve_gpu": [_MDP, _VIG, ct.c_void_p, ct.c_double, U, U, _P(ct.c_uint),
                          _P(ct.c_double)],

@@ -1995,6 +1995,16 @@ int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi, int *jst
 // One plain iteration with HIP events between its four stages (transpose, score, action,
 // materialize); stageMs[4] receives their device times. Advances currentHorizon like
 // pomdp_pbvi_update_gpu.
+int pomdp_pbvi_scoring_variant_gpu(const POMDPPBVIGPU *pbvi, unsigned int *variant)
+{
+    if (pbvi == nullptr || pbvi->d_alphaBA == nullptr || variant == nullptr) {
+        NOVA_ERR("pomdp_pbvi_scoring_variant_gpu", "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    *variant = dense_for(pbvi->d_alphaBA) ? 0u : plain_jpt(pbvi->d_alphaBA);
+    return NOVA_SUCCESS;
+}
+
 int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageMs)
 {
     const char *fn = "pomdp_pbvi_profile_gpu";

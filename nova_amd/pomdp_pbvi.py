@@ -97,6 +97,13 @@ class POMDPPBVIGPU(_abi.NovaPOMDPPBVIGPU):
             raise Exception("pomdp_pbvi_run_gpu failed with code %d" % rc)
         return ms.value
 
+    def scoring_variant(self):
+        """1, 2, 4: alphas per lane on the vector ALUs; 8: fp32 matrix cores; 0: dense GEMM."""
+        v = ct.c_uint(0)
+        if self._lib.pomdp_pbvi_scoring_variant_gpu(self, ct.byref(v)) != 0:
+            raise Exception("pomdp_pbvi_scoring_variant_gpu failed")
+        return v.value
+
     def profile(self):
         """One iteration; returns the device ms of its (transpose, score, action, materialize)."""
         ms = (ct.c_float * 4)()

@@ -82,13 +82,6 @@ def bench_vi(args):
     w = synth.workload("vi_n1e6")
     n, m, ns = p["n"], p["m"], p["ns"]
     sweeps = args.sweeps or p["horizon"]
-    if args.calibrate_copy_mb:
-        # a plain device copy of known size, for scaling HBM byte counters
-        a = torch.ones(args.calibrate_copy_mb << 18, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
     mdp = MDP.from_arrays(w["S"], w["T"], w["R"], gamma=w["gamma"], horizon=sweeps)
     mdp.initialize_gpu()
     vi = MDPValueIterationGPU(mdp, Vinitial=np.zeros(n, np.float32))
@@ -317,6 +310,14 @@ def main():
         print("bench.py: this build measures one GPU only (the path is single-GPU; see the "
               "sharding design); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy_mb:
+        # a plain device copy of known size, for scaling HBM byte counters
+        import torch
+        a = torch.ones(args.calibrate_copy_mb << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     out = {"metric": METRIC, "config": {"workload": "none (VI leg skipped)"}} if args.no_vi \
         else bench_vi(args)
     extras = {}

@@ -743,8 +743,12 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
 
 // --------------------------------------------------------------------- 4. materialize
 
+// Threads of the materialisation workgroup: 16 waves per belief keep enough loads in flight
+// while one or two workgroups per CU hold their n-float row in LDS.
+constexpr int kMB = 1024;
+
 template <bool EXACT>
-__global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restrict__ insts,
+__global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     extern __shared__ float row[];   // n floats when the instance stages rows in LDS
@@ -764,19 +768,19 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
     const int *js = P.jstar + ((size_t)i * m + a) * z;
     if (EXACT || !useLds) {
         // alpha(s) = R[s,a] + sum_o (gamma * sum_l (O*T) * Gamma[j*_o, sp]), o ascending
-        for (unsigned s = t; s < n; s += kPB)
+        for (unsigned s = t; s < n; s += kMB)
             out[s] = P.R[(size_t)s * m + a];
         for (unsigned o = 0; o < z; o++) {
             const float *Gj = G + (size_t)js[o] * n;
             if (useLds) {
                 __syncthreads();
-                for (unsigned s = t; s < n; s += kPB)
+                for (unsigned s = t; s < n; s += kMB)
                     row[s] = Gj[s];
                 __syncthreads();
             }
             const float *srcv = useLds ? row : Gj;
             const float *ot = P.OT + ((size_t)(a * z + o) * n) * ns;
-            for (unsigned s = t; s < n; s += kPB) {
+            for (unsigned s = t; s < n; s += kMB) {
                 const size_t rw = ((size_t)s * m + a) * ns;
                 float vt = 0.0f;
                 for (unsigned l = 0; l < ns; l++) {
@@ -793,8 +797,9 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
         // reads contiguous over states through the action-major copies
         const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.OTr + (size_t)a * z * n;
         const NOVA_GLOBAL float *Gg = (const NOVA_GLOBAL float *)G;
-        for (unsigned sp = t; sp < n; sp += kPB) {
+        for (unsigned sp = t; sp < n; sp += kMB) {
             float h = 0.0f;
+#pragma unroll 8
             for (unsigned o = 0; o < z; o++)
                 h = __builtin_fmaf(Oa[(size_t)o * n + sp], Gg[(size_t)js[o] * n + sp], h);
             row[sp] = h;
@@ -803,7 +808,7 @@ __global__ __launch_bounds__(kPB) void pbvi_materialize(const PbviInst *__restri
         const NOVA_GLOBAL int *Sa = (const NOVA_GLOBAL int *)P.SA + (size_t)a * n * ns;
         const NOVA_GLOBAL float *Ta = (const NOVA_GLOBAL float *)P.TA + (size_t)a * n * ns;
         const NOVA_GLOBAL float *Ra = (const NOVA_GLOBAL float *)P.RT + (size_t)a * n;
-        for (unsigned s = t; s < n; s += kPB) {
+        for (unsigned s = t; s < n; s += kMB) {
             const size_t rw = (size_t)s * ns;
             float acc = 0.0f;
             for (unsigned l = 0; l < ns; l++) {
@@ -1419,7 +1424,7 @@ static void dense_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, hipS
     hipLaunchKernelGGL(dense_action, dim3(D.r), dim3(kPB), 0, st, D.m, D.z, D.r, D.tiles, D.Rb,
                        D.partVal, D.partJ, I.jstar, I.act);
     const unsigned *be = pl.dOff + (kVariants + 1) * (pl.count + 1);
-    hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st,
+    hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kMB), pl.ldsBytes, st,
                        pl.dInst, be, pl.count, iter, flip);
 }
 
@@ -1446,7 +1451,7 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     else
         hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
-    hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, iter,
+    hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kMB), pl.ldsBytes, st, pl.dInst, be, c, iter,
                        flip);
 }
 
@@ -2041,7 +2046,7 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
         hipLaunchKernelGGL(dense_action, dim3(D.r), dim3(kPB), 0, st, D.m, D.z, D.r, D.tiles,
                            D.Rb, D.partVal, D.partJ, I.jstar, I.act);
         hipEventRecord(ev[3], st);
-        hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st,
+        hipLaunchKernelGGL(pbvi_materialize<false>, dim3(pl.totBe), dim3(kMB), pl.ldsBytes, st,
                            pl.dInst, be, c, 0u, flip);
         hipEventRecord(ev[4], st);
         hipError_t e = hipStreamSynchronize(st);
@@ -2067,7 +2072,7 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
         hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
     hipEventRecord(ev[3], st);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
-    hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, 0u, flip);
+    hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kMB), pl.ldsBytes, st, pl.dInst, be, c, 0u, flip);
     hipEventRecord(ev[4], st);
     hipError_t e = hipStreamSynchronize(st);
     for (int k = 0; k < 4; k++)
@@ -2402,7 +2407,7 @@ int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlp
             else
                 hipLaunchKernelGGL(pbvi_action<false>, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
             auto matFn = (g_pbvi_mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
-            hipLaunchKernelGGL(matFn, dim3(1), dim3(kPB), pl.ldsBytes, st, pl.dInst, be, c, 0u, 0u);
+            hipLaunchKernelGGL(matFn, dim3(1), dim3(kMB), pl.ldsBytes, st, pl.dInst, be, c, 0u, 0u);
         }
         // b . alpha_new (support order), read for bIndex only
         hipLaunchKernelGGL(perseus_vb_update, dim3(gridB), dim3(kPB), 0, st, n, r, rz,

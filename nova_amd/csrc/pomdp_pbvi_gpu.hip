@@ -73,6 +73,7 @@ struct PbviInst {
     float *pairW;          //   pairSp/pairW[(i*m + a)*kmax + u] = S[s_k,a,l], gamma*b_k*T[s_k,a,l]
     unsigned *pairCnt;     //   pairCnt[i*m + a] = number of pairs
     unsigned kmax;         //   rz*ns (the lists exist when it is at most kUMax)
+    unsigned rS;           // row stride of GT: r rounded up to a whole matrix-core tile
     float *TA, *RT, *OTr;  // TA likewise for T; RT[a*n + s] = R[s,a]; OTr[(a*z + o)*n + s'] = O[a,s',o]
     int *partJ, *jstar;
     float *aval;
@@ -227,10 +228,11 @@ __global__ __launch_bounds__(kPB) void pbvi_transpose(const PbviInst *__restrict
         tile[q][tx] = (j < P.r && s < P.n) ? G[(size_t)j * P.n + s] : 0.0f;
     }
     __syncthreads();
+    // columns r..rS-1 of each row are written as zeros
     for (unsigned q = ty; q < 64; q += 4) {
         const unsigned s = s0 + q, j = j0 + tx;
-        if (s < P.n && j < P.r)
-            P.GT[(size_t)s * P.r + j] = tile[tx][q];
+        if (s < P.n && j < P.rS)
+            ((NOVA_GLOBAL float *)P.GT)[(size_t)s * P.rS + j] = tile[tx][q];
     }
 }
 
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
             }
             const int u1 = EXACT ? L.startK[k + 1] : uEnd;
             for (int u = EXACT ? L.startK[k] : 0; u < u1; u++) {
-                const size_t rowOff = (size_t)L.spU[u] * r;
+                const size_t rowOff = (size_t)L.spU[u] * P.rS;
                 float g[JPT];
 #pragma unroll
                 for (int q = 0; q < JPT; q++)
@@ -508,12 +510,10 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
     const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
     const unsigned jw = tile * kMTile + wave * 128;     // the wave's first alpha
-    // this lane's four columns, clamped into the row: columns past r only feed C columns the
-    // epilogue ignores
-    unsigned jc[4];
-#pragma unroll
-    for (int y = 0; y < 4; y++)
-        jc[y] = min(jw + y * 32 + lane % 32, r - 1);
+    // this lane's first column; GT rows are padded with zeros to whole tiles, and columns past r
+    // only feed C columns the epilogue ignores
+    const unsigned jl = jw + lane % 32;
+    const size_t rS = P.rS;
 
     for (unsigned o0 = 0; o0 < z; o0 += 32) {
         const int zc = min(32, (int)(z - o0));
@@ -541,10 +541,10 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int uu = u0 + 2 * q + (int)(lane / 32);
-                    const NOVA_GLOBAL float *row = GT + (size_t)spU[c0 + uu] * r;
+                    const NOVA_GLOBAL float *row = GT + (size_t)spU[c0 + uu] * rS + jl;
 #pragma unroll
                     for (int y = 0; y < 4; y++)
-                        dst[q][y] = row[jc[y]];
+                        dst[q][y] = row[y * 32];
                 }
             };
             auto mfmaTrip = [&](int u0, const float (&b)[4][4]) {
@@ -1174,6 +1174,9 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 static unsigned tile_width(unsigned jpt) { return jpt == 8 ? (unsigned)kMTile : kPB * jpt; }
 static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
+// Row stride of the transposed Gamma: r rounded up to the matrix-core tile.
+static unsigned gt_stride(unsigned r) { return (r + kMTile - 1) / kMTile * kMTile; }
+
 static bool pbvi_shape_ok(const nova::POMDP *p)
 {
     return p->rz <= (unsigned)kKMax && (size_t)p->rz * p->ns <= (size_t)kUMax;
@@ -1197,7 +1200,7 @@ static size_t inst_ws_bytes(const nova::POMDP *p)
            2 * align256(sizeof(float) * (size_t)p->m * p->n * p->ns) +
            align256(sizeof(float) * (size_t)p->m * p->n) +
            align256(sizeof(float) * (size_t)p->m * p->z * p->n) +
-           align256(sizeof(float) * (size_t)p->n * p->r) +
+           align256(sizeof(float) * (size_t)p->n * gt_stride(p->r)) +
            align256(sizeof(float) * tiles * rmz) + align256(sizeof(int) * tiles * rmz) +
            align256(sizeof(int) * rmz) + align256(sizeof(float) * (size_t)p->r * p->m) +
            align256(sizeof(int) * p->r);
@@ -1220,7 +1223,8 @@ static void inst_ws(const nova::POMDP *p, char *base, PbviInst &I)
     I.OTr = reinterpret_cast<float *>(c);
     c += align256(sizeof(float) * (size_t)p->m * p->z * p->n);
     I.GT = reinterpret_cast<float *>(c);
-    c += align256(sizeof(float) * (size_t)p->n * p->r);
+    I.rS = gt_stride(p->r);
+    c += align256(sizeof(float) * (size_t)p->n * I.rS);
     I.kmax = p->rz * p->ns;
     if (pbvi_shape_ok(p)) {
         const size_t lists = (size_t)p->r * p->m * p->rz * p->ns;
@@ -1330,7 +1334,7 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
     for (unsigned b = 0; b < c; b++) {
         PbviInst &I = pl.host[b];
         I.tiles = tiles_for(I.r, I.jpt);
-        tr[b + 1] = tr[b] + ((I.n + 63) / 64) * ((I.r + 63) / 64);
+        tr[b + 1] = tr[b] + ((I.n + 63) / 64) * (I.rS / 64);
         for (int v = 0; v < kVariants; v++)
             scs[v][b + 1] = scs[v][b] + ((jpt_slot(I.jpt) == v) ? I.rb * I.m * I.tiles : 0u);
         be[b + 1] = be[b] + I.rb;

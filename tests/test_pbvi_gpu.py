@@ -201,3 +201,34 @@ def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag
     # from FP64 (nova's own arithmetic drifts 0.58% at pbvi_cpu_n64 after 100 iterations): the
     # fast path must stay within 1e-4 or within twice nova's own drift
     assert err < max(1e-4, 2 * err_nova)
+
+
+def test_full_size_sample_vs_oracle(oracle):
+    """At the benchmark's size (n = 20,000, m = 16, z = 32, ns = 8, r = 4,096, rz = 64), one
+    fast iteration from a random Gamma, checked on sampled beliefs (first, middle, last)
+    against the float oracle: alpha choices except at near-ties, actions, and the new alphas
+    within 1e-5 relative."""
+    P = synth.workload("pbvi_n20000")
+    rng = np.random.default_rng(12)
+    Gt = (P["Gamma0"] + rng.uniform(-1, 1, P["Gamma0"].shape)).astype(np.float32)
+    set_mode(FAST)
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    s = POMDPPBVIGPU(p, GammaInitial=Gt)
+    s.update()
+    js, act = s.argmax_record()
+    G1 = s.get_policy().Gamma_array()
+    del s
+    r = Gt.shape[0]
+    for i in (0, r // 2, r - 1):
+        ref = oracle.pbvi_backup(P, Gt, i, i + 1)
+        jo, gap = ref["jstar"][0], ref["gap"][0]
+        tie = gap <= FAST_TIE_REL * np.maximum(1.0, np.abs(ref["aval"][0]))[:, None]
+        bad = js[i] != jo
+        assert not np.any(bad & ~tie), "belief %d: alpha choice differs outside near-ties" % i
+        if not bad.any():
+            assert act[i] == int(ref["pi"][0])
+            a, b = G1[i].astype(np.float64), ref["GammaPrime"][0].astype(np.float64)
+            err = np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b)))
+            print("pbvi_n20000 belief %d: choices match, alpha error %.3g" % (i, err), flush=True)
+            assert err <= 1e-5

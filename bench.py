@@ -170,9 +170,9 @@ def bench_pbvi(args):
     out = {"workload": "pbvi_n20000", "backups_per_s": r * iters / wall,
            "ms_per_iteration": ms / iters, "stage_ms": dict(zip(
                ["transpose", "score", "action", "materialize"], stages)),
-           "roofline": {"bound": "mfma_fp32" if variant == 8 else "valu_fp32",
-                        "kernel": "pbvi_score_mfma" if variant == 8 else
-                                  "pbvi_score<fast,%d>" % variant,
+           "roofline": {"bound": "mfma_fp32" if variant >= 8 else "valu_fp32",
+                        "kernel": ("pbvi_score_mfma<%d waves>" % (variant // 2)) if variant >= 8
+                                  else "pbvi_score<fast,%d>" % variant,
                         "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": achieved / 157.3, "algorithmic_flops_per_launch": flops}}
     del s

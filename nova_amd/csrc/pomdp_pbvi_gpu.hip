@@ -477,13 +477,16 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 constexpr int kMTile = 512;          // alphas per workgroup
 constexpr int kWChunk = 256;         // support pairs per staged W chunk
 
-__global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restrict__ insts,
+template <int NW, int NB>
+__global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
+    static_assert(NW * NB * 32 == kMTile, "a workgroup covers one alpha tile");
+    constexpr int NT = NW * 64;
     __shared__ int spU[kUMax];                // successor of each pair
     __shared__ float w[kWChunk * 32];         // [u][o] for the current chunk and 32 observations
-    __shared__ float redV[kPB / 64][32];
-    __shared__ int redJ[kPB / 64][32];
+    __shared__ float redV[NW][32];
+    __shared__ int redJ[NW][32];
 
     const unsigned b = find_inst(off, count, blockIdx.x);
     const PbviInst &P = insts[b];
@@ -503,13 +506,13 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
     const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
     const int uEnd16 = (uEnd + 15) & ~15;
     const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
-    for (int u = t; u < uEnd16; u += kPB)
+    for (int u = t; u < uEnd16; u += NT)
         spU[u] = (u < uEnd) ? pSp[u] : 0;
     __syncthreads();
     const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
     const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
-    const unsigned jw = tile * kMTile + wave * 128;     // the wave's first alpha
+    const unsigned jw = tile * kMTile + wave * NB * 32;  // the wave's first alpha
     // this lane's first column; GT rows are padded with zeros to whole tiles, and columns past r
     // only feed C columns the epilogue ignores
     const unsigned jl = jw + lane % 32;
@@ -517,16 +520,16 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
 
     for (unsigned o0 = 0; o0 < z; o0 += 32) {
         const int zc = min(32, (int)(z - o0));
-        floatx16_t acc[4];
+        floatx16_t acc[NB];
 #pragma unroll
-        for (int y = 0; y < 4; y++)
+        for (int y = 0; y < NB; y++)
 #pragma unroll
             for (int v = 0; v < 16; v++)
                 acc[y][v] = 0.0f;
         for (int c0 = 0; c0 < uEnd; c0 += kWChunk) {
             const int cPad = min(kWChunk, uEnd16 - c0);     // a multiple of 16
             __syncthreads();
-            for (int x = t; x < cPad * 32; x += kPB) {
+            for (int x = t; x < cPad * 32; x += NT) {
                 const int u = c0 + x / 32, oo = x % 32;
                 float v = 0.0f;
                 if (u < uEnd && oo < zc)
@@ -536,18 +539,18 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
             __syncthreads();
             // four pair-steps (8 pairs) per trip; the B operands of the next trip are loaded
             // before this trip's 16 MFMAs, so their latency hides behind the matrix work
-            float fb[4][4], nb[4][4];
-            auto loadB = [&](int u0, float (&dst)[4][4]) {
+            float fb[4][NB], nb[4][NB];
+            auto loadB = [&](int u0, float (&dst)[4][NB]) {
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int uu = u0 + 2 * q + (int)(lane / 32);
                     const NOVA_GLOBAL float *row = GT + (size_t)spU[c0 + uu] * rS + jl;
 #pragma unroll
-                    for (int y = 0; y < 4; y++)
+                    for (int y = 0; y < NB; y++)
                         dst[q][y] = row[y * 32];
                 }
             };
-            auto mfmaTrip = [&](int u0, const float (&b)[4][4]) {
+            auto mfmaTrip = [&](int u0, const float (&b)[4][NB]) {
                 float fa[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++)
@@ -555,7 +558,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
 #pragma unroll
                 for (int q = 0; q < 4; q++)
 #pragma unroll
-                    for (int y = 0; y < 4; y++)
+                    for (int y = 0; y < NB; y++)
                         acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[q], b[q][y], acc[y], 0, 0, 0);
             };
             // ping-pong between two register sets: trip t's operands were requested one trip
@@ -574,13 +577,13 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // per row (observation): first maximiser over the wave's 128 alphas, then the 4 waves
+        // per row (observation): first maximiser over the wave's alphas, then across the waves
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             float bv = kLow;
             int bj = INT_MAX;
 #pragma unroll
-            for (int y = 0; y < 4; y++) {
+            for (int y = 0; y < NB; y++) {
                 const unsigned j = jw + y * 32 + lane % 32;
                 if (j < r && acc[y][v] > bv) {
                     bv = acc[y][v];
@@ -605,7 +608,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score_mfma(const PbviInst *__restric
         if (t < (unsigned)zc) {
             float v = redV[0][t];
             int j = redJ[0][t];
-            for (int wv = 1; wv < kPB / 64; wv++)
+            for (int wv = 1; wv < NW; wv++)
                 if (better(redV[wv][t], redJ[wv][t], v, j)) {
                     v = redV[wv][t];
                     j = redJ[wv][t];
@@ -1170,8 +1173,8 @@ static bool dense_mode(int mode) { return mode == 2 || mode == 3; }
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Scoring variants: jpt = 1, 2, 4 alphas per lane on the vector ALUs (tiles of 256 * jpt alphas),
-// jpt = 8 the matrix-core kernel of fast mode (tiles of kMTile alphas).
-static unsigned tile_width(unsigned jpt) { return jpt == 8 ? (unsigned)kMTile : kPB * jpt; }
+// jpt = 8, 16 the matrix-core kernel of fast mode with 4 or 8 waves (tiles of kMTile alphas).
+static unsigned tile_width(unsigned jpt) { return jpt >= 8 ? (unsigned)kMTile : kPB * jpt; }
 static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
 // Row stride of the transposed Gamma: r rounded up to the matrix-core tile.
@@ -1294,8 +1297,11 @@ struct PbviPlan {
     std::vector<PbviInst> host;
 };
 
-static int jpt_slot(unsigned jpt) { return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : 3)); }
-constexpr int kVariants = 4;
+static int jpt_slot(unsigned jpt)
+{
+    return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : 4)));
+}
+constexpr int kVariants = 5;
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
 static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
@@ -1309,7 +1315,8 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
         if (jpt == 1) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
         if (jpt == 2) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
         if (jpt == 4) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
-        if (jpt == 8) hipLaunchKernelGGL(pbvi_score_mfma, g, dim3(kPB), 0, st, dInst, sc, c, iter);
+        if (jpt == 8) hipLaunchKernelGGL((pbvi_score_mfma<4, 4>), g, dim3(256), 0, st, dInst, sc, c, iter);
+        if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2>), g, dim3(512), 0, st, dInst, sc, c, iter);
     }
 }
 
@@ -1443,7 +1450,7 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     const unsigned c = pl.count;
     const unsigned *tr = pl.dOff, *be = pl.dOff + (kVariants + 1) * (c + 1);
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    static const unsigned kJpt[kVariants] = {1, 2, 4, 8};
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16};
     for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
@@ -1486,9 +1493,9 @@ static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mod
 // The scoring kernel's variants (1, 2 or 4 alphas per lane) trade occupancy against reuse of
 // the staged O*T factors. For each size class (r, belief-support pairs rz*ns and z, each
 // bucketed to a power of two) the first instance seen is timed with every variant on one
-// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4|8
-// forces a variant (8: the matrix-core kernel, fast mode only); NOVA_AMD_AUTOTUNE=0 disables
-// timing (default variant 2).
+// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4|8|16
+// forces a variant (8, 16: the matrix-core kernel with 4 or 8 waves, fast mode only);
+// NOVA_AMD_AUTOTUNE=0 disables timing (default variant 2).
 
 struct TuneKey {
     unsigned r, u, z;
@@ -1512,7 +1519,7 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
 {
     if (const char *f = std::getenv("NOVA_AMD_PBVI_JPT")) {
         const int v = std::atoi(f);
-        if (v == 1 || v == 2 || v == 4 || (v == 8 && mode != 1))
+        if (v == 1 || v == 2 || v == 4 || ((v == 8 || v == 16) && mode != 1))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
@@ -1527,8 +1534,8 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u, 8u}) {
-        if (jpt == 8 && mode == 1)
+    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u}) {
+        if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
         PbviPlan pl;
         pl.count = 1;

@@ -214,6 +214,8 @@ __global__ __launch_bounds__(kPB) void pbvi_transpose(const PbviInst *__restrict
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     __shared__ float tile[64][65];
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
     const PbviInst &P = insts[b];
     if (iter >= P.horizon)
@@ -311,6 +313,8 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
     __shared__ float redV[kPB / 64][kZC];
     __shared__ int redJ[kPB / 64][kZC];
 
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
     const PbviInst &P = insts[b];
     if (iter >= P.horizon)
@@ -488,6 +492,8 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     __shared__ float redV[NW][32];
     __shared__ int redJ[NW][32];
 
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
     const PbviInst &P = insts[b];
     if (iter >= P.horizon)
@@ -635,6 +641,8 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
     __shared__ float alphaK[kKMax];
     __shared__ float bS[kKMax];
     __shared__ int kEndS;
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
     const PbviInst &P = insts[b];
     if (iter >= P.horizon)
@@ -755,6 +763,8 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     extern __shared__ float row[];   // n floats when the instance stages rows in LDS
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
     const PbviInst &P = insts[b];
     if (iter >= P.horizon)
@@ -1286,22 +1296,28 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
 struct DenseWs;
 
+// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16); PbviPlan::totSc and the offset table
+// hold one entry per variant.
+constexpr int kVariants = 5;
+
 struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
     unsigned count = 0;
     PbviInst *dInst = nullptr;       // device copy of the descriptors
     unsigned *dOff = nullptr;        // [tr | sc(jpt=1) | sc(2) | sc(4) | be], count+1 each
-    unsigned totTr = 0, totSc[4] = {0, 0, 0, 0}, totBe = 0;
+    unsigned totTr = 0, totSc[kVariants] = {}, totBe = 0;
     size_t ldsBytes = 0;
     unsigned maxHorizon = 0;
     std::vector<PbviInst> host;
 };
 
+static_assert(sizeof(PbviPlan::totSc) == kVariants * sizeof(unsigned),
+              "one workgroup count per scoring variant");
+
 static int jpt_slot(unsigned jpt)
 {
     return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : 4)));
 }
-constexpr int kVariants = 5;
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
 static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,

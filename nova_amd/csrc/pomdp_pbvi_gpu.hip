@@ -938,30 +938,28 @@ __global__ __launch_bounds__(kPB) void dense_build_scales(unsigned n, unsigned k
     }
 }
 
-// 16 bytes of IN scaled elementwise by sc (4 or 8 floats)
-template <typename IN> __device__ __forceinline__ uint4 scale16(uint4 v, const float *sc);
-template <> __device__ __forceinline__ uint4 scale16<float>(uint4 v, const float *sc)
+// 16 bytes of IN scaled elementwise by s0 (and s1 for the upper four of eight bf16)
+template <typename IN> __device__ __forceinline__ uint4 scale16(uint4 v, float4 s0, float4 s1);
+template <> __device__ __forceinline__ uint4 scale16<float>(uint4 v, float4 s0, float4)
 {
     uint4 o;
-    o.x = __float_as_uint(__uint_as_float(v.x) * sc[0]);
-    o.y = __float_as_uint(__uint_as_float(v.y) * sc[1]);
-    o.z = __float_as_uint(__uint_as_float(v.z) * sc[2]);
-    o.w = __float_as_uint(__uint_as_float(v.w) * sc[3]);
+    o.x = __float_as_uint(__uint_as_float(v.x) * s0.x);
+    o.y = __float_as_uint(__uint_as_float(v.y) * s0.y);
+    o.z = __float_as_uint(__uint_as_float(v.z) * s0.z);
+    o.w = __float_as_uint(__uint_as_float(v.w) * s0.w);
     return o;
 }
-template <> __device__ __forceinline__ uint4 scale16<__bf16>(uint4 v, const float *sc)
+__device__ __forceinline__ unsigned scale_bf16x2(unsigned v, float lo, float hi)
 {
-    const unsigned in[4] = {v.x, v.y, v.z, v.w};
-    unsigned out[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const float lo = __uint_as_float(in[q] << 16) * sc[2 * q];
-        const float hi = __uint_as_float(in[q] & 0xffff0000u) * sc[2 * q + 1];
-        const __bf16 bl = (__bf16)lo, bh = (__bf16)hi;
-        out[q] = (unsigned)__builtin_bit_cast(unsigned short, bl) |
-                 ((unsigned)__builtin_bit_cast(unsigned short, bh) << 16);
-    }
-    return make_uint4(out[0], out[1], out[2], out[3]);
+    const __bf16 bl = (__bf16)(__uint_as_float(v << 16) * lo);
+    const __bf16 bh = (__bf16)(__uint_as_float(v & 0xffff0000u) * hi);
+    return (unsigned)__builtin_bit_cast(unsigned short, bl) |
+           ((unsigned)__builtin_bit_cast(unsigned short, bh) << 16);
+}
+template <> __device__ __forceinline__ uint4 scale16<__bf16>(uint4 v, float4 s0, float4 s1)
+{
+    return make_uint4(scale_bf16x2(v.x, s0.x, s0.y), scale_bf16x2(v.y, s0.z, s0.w),
+                      scale_bf16x2(v.z, s1.x, s1.y), scale_bf16x2(v.w, s1.z, s1.w));
 }
 
 template <typename IN>
@@ -987,30 +985,36 @@ __global__ __launch_bounds__(kGThreads) void dense_score(unsigned r, unsigned kp
 
     // staging: 16-byte chunk ch of rows row0 + 64 q (q < 4), for both operands
     const unsigned ch = t % 8, row0 = t / 8;
-    uint4 ra[4], rb[4];
+    // the next k-step's operands and gamma*O scales are all requested before the matrix work
+    // (named registers, no arrays: indexable arrays would live in scratch memory)
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+    float4 sc0, sc1;
+    const size_t rowOff = (size_t)row0 * kpad + ch * EPC, q64 = (size_t)64 * kpad;
     auto gload = [&](unsigned k0) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const size_t off = (size_t)(row0 + 64 * q) * kpad + k0 + ch * EPC;
-            ra[q] = *reinterpret_cast<const uint4 *>(Ua + off);
-            rb[q] = *reinterpret_cast<const uint4 *>(Gj + off);
-        }
+        const IN *ua = Ua + rowOff + k0, *gj = Gj + rowOff + k0;
+        ra0 = *reinterpret_cast<const uint4 *>(ua);
+        ra1 = *reinterpret_cast<const uint4 *>(ua + q64);
+        ra2 = *reinterpret_cast<const uint4 *>(ua + 2 * q64);
+        ra3 = *reinterpret_cast<const uint4 *>(ua + 3 * q64);
+        rb0 = *reinterpret_cast<const uint4 *>(gj);
+        rb1 = *reinterpret_cast<const uint4 *>(gj + q64);
+        rb2 = *reinterpret_cast<const uint4 *>(gj + 2 * q64);
+        rb3 = *reinterpret_cast<const uint4 *>(gj + 3 * q64);
+        sc0 = *reinterpret_cast<const float4 *>(sv + k0 + ch * EPC);
+        sc1 = (EPC == 8) ? *reinterpret_cast<const float4 *>(sv + k0 + ch * EPC + 4) : sc0;
     };
-    auto sstore = [&](unsigned k0, unsigned buf) {
-        float sc[EPC];
-#pragma unroll
-        for (int e = 0; e < EPC; e += 4) {
-            const float4 f = *reinterpret_cast<const float4 *>(sv + k0 + ch * EPC + e);
-            sc[e] = f.x; sc[e + 1] = f.y; sc[e + 2] = f.z; sc[e + 3] = f.w;
-        }
+    auto sstore = [&](unsigned buf) {
         unsigned char *A = glds + (size_t)buf * 2 * kGOperandBytes;
         unsigned char *Bm = A + kGOperandBytes;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const unsigned o = (row0 + 64 * q) * kGStride + ch * 16;
-            *reinterpret_cast<uint4 *>(A + o) = ra[q];
-            *reinterpret_cast<uint4 *>(Bm + o) = scale16<IN>(rb[q], sc);
-        }
+        const unsigned o = row0 * kGStride + ch * 16, q64s = 64 * kGStride;
+        *reinterpret_cast<uint4 *>(A + o) = ra0;
+        *reinterpret_cast<uint4 *>(A + o + q64s) = ra1;
+        *reinterpret_cast<uint4 *>(A + o + 2 * q64s) = ra2;
+        *reinterpret_cast<uint4 *>(A + o + 3 * q64s) = ra3;
+        *reinterpret_cast<uint4 *>(Bm + o) = scale16<IN>(rb0, sc0, sc1);
+        *reinterpret_cast<uint4 *>(Bm + o + q64s) = scale16<IN>(rb1, sc0, sc1);
+        *reinterpret_cast<uint4 *>(Bm + o + 2 * q64s) = scale16<IN>(rb2, sc0, sc1);
+        *reinterpret_cast<uint4 *>(Bm + o + 3 * q64s) = scale16<IN>(rb3, sc0, sc1);
     };
 
     floatx16_t acc[4][2];
@@ -1024,7 +1028,7 @@ __global__ __launch_bounds__(kGThreads) void dense_score(unsigned r, unsigned kp
 
     const unsigned nk = kpad / KS;
     gload(0);
-    sstore(0, 0);
+    sstore(0);
     __syncthreads();
     for (unsigned ks = 0; ks < nk; ks++) {
         const unsigned buf = ks % 2;
@@ -1072,7 +1076,7 @@ __global__ __launch_bounds__(kGThreads) void dense_score(unsigned r, unsigned kp
             }
         }
         if (ks + 1 < nk)
-            sstore((ks + 1) * KS, buf ^ 1);
+            sstore(buf ^ 1);
         __syncthreads();
     }
 

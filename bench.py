@@ -171,8 +171,9 @@ def bench_pbvi(args):
            "ms_per_iteration": ms / iters, "stage_ms": dict(zip(
                ["transpose", "score", "action", "materialize"], stages)),
            "roofline": {"bound": "mfma_fp32" if variant >= 8 else "valu_fp32",
-                        "kernel": ("pbvi_score_mfma<%d waves>" % (variant // 2)) if variant >= 8
-                                  else "pbvi_score<fast,%d>" % variant,
+                        "kernel": {8: "pbvi_score_mfma<4 waves>", 16: "pbvi_score_mfma<8 waves>",
+                                   32: "pbvi_score_mfma<8 waves, 8-step prefetch>"}.get(
+                                       variant, "pbvi_score<fast,%d>" % variant),
                         "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": achieved / 157.3, "algorithmic_flops_per_launch": flops}}
     del s
@@ -293,7 +294,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sweeps", type=int, default=10)
+    ap.add_argument("--cpu-sweeps", type=int, default=100)
     ap.add_argument("--sweeps", type=int, default=0,
                     help="sweeps per step (default: the workload's 200; fewer only for profiling)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,

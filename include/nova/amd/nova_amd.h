@@ -74,8 +74,8 @@ extern "C" int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps,
 extern "C" int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageMs);
 
 // The scoring kernel an initialized GPU PBVI solver uses: 1, 2, 4 = alphas per lane on the
-// vector ALUs, 8 and 16 = the fp32 matrix-core kernel with 4 or 8 waves per workgroup, 0 = the
-// dense-belief matrix-core GEMM.
+// vector ALUs, 8, 16, 32 = the fp32 matrix-core kernel with 4 waves, 8 waves, or 8 waves and a
+// deeper prefetch per workgroup, 0 = the dense-belief matrix-core GEMM.
 extern "C" int pomdp_pbvi_scoring_variant_gpu(const POMDPPBVIGPU *pbvi, unsigned int *variant);
 
 // ---- PBVI argmax replay (gradients of L = sum_b max_alpha alpha . b with respect to R)

@@ -481,12 +481,14 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 constexpr int kMTile = 512;          // alphas per workgroup
 constexpr int kWChunk = 256;         // support pairs per staged W chunk
 
-template <int NW, int NB>
+template <int NW, int NB, int PS>
 __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     static_assert(NW * NB * 32 == kMTile, "a workgroup covers one alpha tile");
+    static_assert(kWChunk % (4 * PS) == 0 && kUMax % (4 * PS) == 0, "whole loop steps");
     constexpr int NT = NW * 64;
+    constexpr int STEP = 4 * PS;              // pairs per loop step (two trips of PS pair-steps)
     __shared__ int spU[kUMax];                // successor of each pair
     __shared__ float w[kWChunk * 32];         // [u][o] for the current chunk and 32 observations
     __shared__ float redV[NW][32];
@@ -507,10 +509,10 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
-    // the precomputed pair list of (i, a): successors to LDS (padded with state 0 to a multiple
-    // of 16 pairs, whose factors are 0), factors read when staged
+    // the precomputed pair list of (i, a): successors to LDS (padded with state 0 to whole loop
+    // steps of pairs, whose factors are 0), factors read when staged
     const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
-    const int uEnd16 = (uEnd + 15) & ~15;
+    const int uEnd16 = (uEnd + STEP - 1) / STEP * STEP;
     const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
     for (int u = t; u < uEnd16; u += NT)
         spU[u] = (u < uEnd) ? pSp[u] : 0;
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
             for (int v = 0; v < 16; v++)
                 acc[y][v] = 0.0f;
         for (int c0 = 0; c0 < uEnd; c0 += kWChunk) {
-            const int cPad = min(kWChunk, uEnd16 - c0);     // a multiple of 16
+            const int cPad = min(kWChunk, uEnd16 - c0);     // a multiple of STEP
             __syncthreads();
             for (int x = t; x < cPad * 32; x += NT) {
                 const int u = c0 + x / 32, oo = x % 32;
@@ -543,12 +545,12 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
                 w[x] = v;
             }
             __syncthreads();
-            // four pair-steps (8 pairs) per trip; the B operands of the next trip are loaded
-            // before this trip's 16 MFMAs, so their latency hides behind the matrix work
-            float fb[4][NB], nb[4][NB];
-            auto loadB = [&](int u0, float (&dst)[4][NB]) {
+            // PS pair-steps (2 PS pairs) per trip; the B operands of the next trip are loaded
+            // before this trip's MFMAs, so their latency hides behind the matrix work
+            float fb[PS][NB], nb[PS][NB];
+            auto loadB = [&](int u0, float (&dst)[PS][NB]) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < PS; q++) {
                     const int uu = u0 + 2 * q + (int)(lane / 32);
                     const NOVA_GLOBAL float *row = GT + (size_t)spU[c0 + uu] * rS + jl;
 #pragma unroll
@@ -556,13 +558,13 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
                         dst[q][y] = row[y * 32];
                 }
             };
-            auto mfmaTrip = [&](int u0, const float (&b)[4][NB]) {
-                float fa[4];
+            auto mfmaTrip = [&](int u0, const float (&b)[PS][NB]) {
+                float fa[PS];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int q = 0; q < PS; q++)
                     fa[q] = w[(u0 + 2 * q + (int)(lane / 32)) * 32 + lane % 32];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int q = 0; q < PS; q++)
 #pragma unroll
                     for (int y = 0; y < NB; y++)
                         acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[q], b[q][y], acc[y], 0, 0, 0);
@@ -572,14 +574,14 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
             // staged), so the compiler can count the outstanding ones exactly.
             // sched_barrier keeps the scheduler from sinking the prefetch below the MFMAs.
             loadB(0, fb);
-            for (int u0 = 0; u0 < cPad; u0 += 16) {
-                loadB(u0 + 8, nb);
+            for (int u0 = 0; u0 < cPad; u0 += STEP) {
+                loadB(u0 + STEP / 2, nb);
                 __builtin_amdgcn_sched_barrier(0);
                 mfmaTrip(u0, fb);
                 __builtin_amdgcn_sched_barrier(0);
-                loadB(min(u0 + 16, cPad - 8), fb);
+                loadB(min(u0 + STEP, cPad - STEP / 2), fb);
                 __builtin_amdgcn_sched_barrier(0);
-                mfmaTrip(u0 + 8, nb);
+                mfmaTrip(u0 + STEP / 2, nb);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -1187,7 +1189,7 @@ static bool dense_mode(int mode) { return mode == 2 || mode == 3; }
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Scoring variants: jpt = 1, 2, 4 alphas per lane on the vector ALUs (tiles of 256 * jpt alphas),
-// jpt = 8, 16 the matrix-core kernel of fast mode with 4 or 8 waves (tiles of kMTile alphas).
+// jpt = 8, 16, 32 the matrix-core kernel of fast mode (tiles of kMTile alphas).
 static unsigned tile_width(unsigned jpt) { return jpt >= 8 ? (unsigned)kMTile : kPB * jpt; }
 static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
@@ -1300,9 +1302,9 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
 struct DenseWs;
 
-// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16); PbviPlan::totSc and the offset table
-// hold one entry per variant.
-constexpr int kVariants = 5;
+// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32); PbviPlan::totSc and the offset
+// table hold one entry per variant.
+constexpr int kVariants = 6;
 
 struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
@@ -1320,7 +1322,7 @@ static_assert(sizeof(PbviPlan::totSc) == kVariants * sizeof(unsigned),
 
 static int jpt_slot(unsigned jpt)
 {
-    return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : 4)));
+    return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : (jpt == 16 ? 4 : 5))));
 }
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
@@ -1335,8 +1337,9 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
         if (jpt == 1) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
         if (jpt == 2) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
         if (jpt == 4) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
-        if (jpt == 8) hipLaunchKernelGGL((pbvi_score_mfma<4, 4>), g, dim3(256), 0, st, dInst, sc, c, iter);
-        if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 8) hipLaunchKernelGGL((pbvi_score_mfma<4, 4, 4>), g, dim3(256), 0, st, dInst, sc, c, iter);
+        if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 4>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 32) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 8>), g, dim3(512), 0, st, dInst, sc, c, iter);
     }
 }
 
@@ -1470,7 +1473,7 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     const unsigned c = pl.count;
     const unsigned *tr = pl.dOff, *be = pl.dOff + (kVariants + 1) * (c + 1);
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16};
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32};
     for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
@@ -1513,8 +1516,9 @@ static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mod
 // The scoring kernel's variants (1, 2 or 4 alphas per lane) trade occupancy against reuse of
 // the staged O*T factors. For each size class (r, belief-support pairs rz*ns and z, each
 // bucketed to a power of two) the first instance seen is timed with every variant on one
-// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4|8|16
-// forces a variant (8, 16: the matrix-core kernel with 4 or 8 waves, fast mode only);
+// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4|8|16|32
+// forces a variant (8, 16, 32: the matrix-core kernel with 4 waves, 8 waves, or 8 waves and a
+// two-fold prefetch; fast mode only);
 // NOVA_AMD_AUTOTUNE=0 disables timing (default variant 2).
 
 struct TuneKey {
@@ -1539,7 +1543,7 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
 {
     if (const char *f = std::getenv("NOVA_AMD_PBVI_JPT")) {
         const int v = std::atoi(f);
-        if (v == 1 || v == 2 || v == 4 || ((v == 8 || v == 16) && mode != 1))
+        if (v == 1 || v == 2 || v == 4 || ((v == 8 || v == 16 || v == 32) && mode != 1))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
@@ -1554,7 +1558,7 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u}) {
+    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u}) {
         if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
         PbviPlan pl;

@@ -139,13 +139,13 @@ def sparse_values(P, G):
     return V
 
 
-@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8", "16"])
+@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8", "16", "32"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES, ids=[s[0] for s in SHAPES])
 def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, variant):
     """From the same Gamma (the oracle's after t iterations), one fast GPU iteration chooses
     the oracle's alphas except at near-ties; for the tuned scoring kernel and for each forced
-    one (1 and 4 alphas per lane on the vector ALUs, 8 and 16 = the matrix-core kernel with 4 or
-    8 waves)."""
+    one (1 and 4 alphas per lane on the vector ALUs; 8, 16, 32 = the matrix-core kernel with 4
+    waves, 8 waves, 8 waves and a deeper prefetch)."""
     if variant != "tuned":
         monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
     P = synth.pomdp_arrays(**_params(params))
@@ -181,7 +181,7 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
           % (name, variant, mismatch, total))
 
 
-@pytest.mark.parametrize("variant", ["tuned", "8", "16"])
+@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES[:2], ids=[s[0] for s in SHAPES[:2]])
 def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag, variant):
     if variant != "tuned":

@@ -486,10 +486,10 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     static_assert(NW * NB * 32 == kMTile, "a workgroup covers one alpha tile");
-    static_assert(kWChunk % (4 * PS) == 0 && kUMax % (4 * PS) == 0, "whole loop steps");
+    static_assert(kWChunk % (4 * PS) == 0, "whole loop steps per chunk");
     constexpr int NT = NW * 64;
     constexpr int STEP = 4 * PS;              // pairs per loop step (two trips of PS pair-steps)
-    __shared__ int spU[kUMax];                // successor of each pair
+    __shared__ int spU[kWChunk];              // successor of each pair of the current chunk
     __shared__ float w[kWChunk * 32];         // [u][o] for the current chunk and 32 observations
     __shared__ float redV[NW][32];
     __shared__ int redJ[NW][32];
@@ -509,14 +509,11 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
-    // the precomputed pair list of (i, a): successors to LDS (padded with state 0 to whole loop
-    // steps of pairs, whose factors are 0), factors read when staged
+    // the precomputed pair list of (i, a), any length, staged chunk by chunk (padded with state 0
+    // to whole loop steps of pairs, whose factors are 0)
     const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
     const int uEnd16 = (uEnd + STEP - 1) / STEP * STEP;
     const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
-    for (int u = t; u < uEnd16; u += NT)
-        spU[u] = (u < uEnd) ? pSp[u] : 0;
-    __syncthreads();
     const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
     const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
@@ -537,11 +534,14 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
         for (int c0 = 0; c0 < uEnd; c0 += kWChunk) {
             const int cPad = min(kWChunk, uEnd16 - c0);     // a multiple of STEP
             __syncthreads();
+            for (int u = t; u < cPad; u += NT)
+                spU[u] = (c0 + u < uEnd) ? pSp[c0 + u] : 0;
+            __syncthreads();
             for (int x = t; x < cPad * 32; x += NT) {
-                const int u = c0 + x / 32, oo = x % 32;
+                const int u = x / 32, oo = x % 32;
                 float v = 0.0f;
-                if (u < uEnd && oo < zc)
-                    v = pw[u] * Oa[(size_t)spU[u] * z + o0 + oo];
+                if (c0 + u < uEnd && oo < zc)
+                    v = pw[c0 + u] * Oa[(size_t)spU[u] * z + o0 + oo];
                 w[x] = v;
             }
             __syncthreads();
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
 #pragma unroll
                 for (int q = 0; q < PS; q++) {
                     const int uu = u0 + 2 * q + (int)(lane / 32);
-                    const NOVA_GLOBAL float *row = GT + (size_t)spU[c0 + uu] * rS + jl;
+                    const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
 #pragma unroll
                     for (int y = 0; y < NB; y++)
                         dst[q][y] = row[y * 32];
@@ -657,7 +657,8 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
         kEndS = (int)rz;
     __syncthreads();
     for (unsigned k = t; k < rz; k += kPB) {
-        bS[k] = P.B[(size_t)i * rz + k];
+        if (EXACT)
+            bS[k] = P.B[(size_t)i * rz + k];   // exact mode runs only with rz <= kKMax
         if (P.Z[(size_t)i * rz + k] < 0)
             atomicMin(&kEndS, (int)k);
     }
@@ -692,7 +693,7 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
                     float v = (o0 == 0) ? 0.0f : alphaK[0];
                     if (o0 == 0)
                         for (int k = 0; k < kEnd; k++)
-                            v += bS[k] * P.R[(size_t)P.Z[(size_t)i * rz + k] * m + a];
+                            v += P.B[(size_t)i * rz + k] * P.R[(size_t)P.Z[(size_t)i * rz + k] * m + a];
                     for (unsigned oo = 0; oo < oc; oo++)
                         v += bestO[oo];
                     alphaK[0] = v;                 // running action value
@@ -1196,15 +1197,31 @@ static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt
 // Row stride of the transposed Gamma: r rounded up to the matrix-core tile.
 static unsigned gt_stride(unsigned r) { return (r + kMTile - 1) / kMTile * kMTile; }
 
+// Supports the LDS-staged kernels take: exact mode and the vector-ALU scoring variants.
 static bool pbvi_shape_ok(const nova::POMDP *p)
 {
     return p->rz <= (unsigned)kKMax && (size_t)p->rz * p->ns <= (size_t)kUMax;
 }
 
-// Bytes of the fast-mode pair lists (none for shapes the sparse kernels do not take).
+// Whether the fast-mode pair lists (r*m*rz*ns entries of 8 bytes) are built: always, unless they
+// would pass 2^30 entries (beliefs with full support at large n belong to the dense modes).
+constexpr size_t kPairCap = size_t(1) << 30;
+static bool pairs_ok(const nova::POMDP *p)
+{
+    return (size_t)p->r * p->m * p->rz * p->ns <= kPairCap;
+}
+
+// The sparse backup can run this instance in this mode: exact mode needs the LDS-staged support,
+// fast mode the pair lists (the matrix-core scorer takes any support length).
+static bool sparse_shape_ok(const nova::POMDP *p, int mode)
+{
+    return mode == 1 ? pbvi_shape_ok(p) : pairs_ok(p);
+}
+
+// Bytes of the fast-mode pair lists.
 static size_t pair_bytes(const nova::POMDP *p)
 {
-    if (!pbvi_shape_ok(p))
+    if (!pairs_ok(p))
         return 0;
     const size_t lists = (size_t)p->r * p->m * p->rz * p->ns;
     return 2 * align256(sizeof(float) * lists) + align256(sizeof(unsigned) * (size_t)p->r * p->m);
@@ -1245,7 +1262,7 @@ static void inst_ws(const nova::POMDP *p, char *base, PbviInst &I)
     I.rS = gt_stride(p->r);
     c += align256(sizeof(float) * (size_t)p->n * I.rS);
     I.kmax = p->rz * p->ns;
-    if (pbvi_shape_ok(p)) {
+    if (pairs_ok(p)) {
         const size_t lists = (size_t)p->r * p->m * p->rz * p->ns;
         I.pairSp = reinterpret_cast<int *>(c);
         c += align256(sizeof(float) * lists);
@@ -1541,19 +1558,22 @@ static unsigned bucket(unsigned x)
 
 static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
 {
+    // the vector-ALU variants stage the support in LDS; longer supports take the matrix cores
+    const bool valuOk = I.rz <= (unsigned)kKMax && I.rz * I.ns <= (unsigned)kUMax;
     if (const char *f = std::getenv("NOVA_AMD_PBVI_JPT")) {
         const int v = std::atoi(f);
-        if (v == 1 || v == 2 || v == 4 || ((v == 8 || v == 16 || v == 32) && mode != 1))
+        if (((v == 1 || v == 2 || v == 4) && valuOk) ||
+                ((v == 8 || v == 16 || v == 32) && mode != 1))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
         if (std::atoi(f) == 0)
-            return 2;
+            return valuOk ? 2 : 16;
     const TuneKey key{bucket(I.r), bucket(I.rz * I.ns), bucket(I.z), mode == 1 ? 1 : 0};
     for (auto &kv : g_tuned)
         if (kv.first == key)
             return kv.second;
-    unsigned best = 2;
+    unsigned best = valuOk ? 2 : 16;
     float bestMs = 1e30f;
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
@@ -1561,6 +1581,8 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u}) {
         if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
+        if (jpt < 8 && !valuOk)
+            continue;
         PbviPlan pl;
         pl.count = 1;
         I.jpt = jpt;
@@ -1851,9 +1873,11 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
                      "GammaInitial set).");
         return NOVA_ERROR_INVALID_DATA;
     }
-    if (!dense_mode(g_pbvi_mode) && !pbvi_shape_ok(pomdp)) {
-        NOVA_ERR(fn, "Belief supports too long for the sparse backup (rz <= 512 and "
-                     "rz*ns <= 512 required); dense beliefs use modes 2/3.");
+    if (!dense_mode(g_pbvi_mode) && !sparse_shape_ok(pomdp, g_pbvi_mode)) {
+        NOVA_ERR(fn, g_pbvi_mode == 1
+                 ? "Belief supports too long for exact mode (rz <= 512 and rz*ns <= 512); "
+                   "fast mode takes any support length."
+                 : "r*m*rz*ns above 2^30: beliefs with full support use the dense modes 2/3.");
         return NOVA_ERROR_INVALID_DATA;
     }
     if (pbvi->d_alphaBA != nullptr)
@@ -2172,7 +2196,7 @@ int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps, POMDPPBVIGPU *const
     for (unsigned b = 0; b < count; b++) {
         const POMDP *p = pomdps[b];
         if (!pomdp_device_ready(p) || p->horizon < 1 || pbvis[b] == nullptr ||
-                pbvis[b]->GammaInitial == nullptr || !pbvi_shape_ok(p)) {
+                pbvis[b]->GammaInitial == nullptr || !sparse_shape_ok(p, g_pbvi_mode)) {
             NOVA_ERR(fn, "Invalid instance (device-resident POMDP, GammaInitial and supported "
                          "belief support lengths required).");
             return NOVA_ERROR_INVALID_DATA;
@@ -2319,7 +2343,8 @@ int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlp
     const char *fn = "pomdp_perseus_execute_gpu";
     if (!pomdp_device_ready(pomdp) || pomdp->gamma < 0.0f || pomdp->gamma > 1.0f ||
             pomdp->horizon < 1 || per == nullptr || per->GammaInitial == nullptr ||
-            policy == nullptr || dense_mode(g_pbvi_mode) || !pbvi_shape_ok(pomdp)) {
+            policy == nullptr || dense_mode(g_pbvi_mode) ||
+            !sparse_shape_ok(pomdp, g_pbvi_mode)) {
         NOVA_ERR(fn, "Invalid arguments (device-resident POMDP, GammaInitial, sparse mode).");
         return NOVA_ERROR_INVALID_DATA;
     }

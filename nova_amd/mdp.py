@@ -86,6 +86,29 @@ class MDP(_abi.NovaMDP):
         mdp.Rmax = float(R32.max())
         return mdp
 
+    def load(self, filename, filetype="cassandra", scalarize=lambda x: x[0]):
+        """Loads an MDP file (python/nova/mdp.py:157-200): 'cassandra' (.pomdp text without
+        observations) or 'raw' (nova's CSV layout with goals). Replaces this object's model."""
+        from .file_loader import FileLoader
+        if filetype not in ("cassandra", "raw"):
+            raise ValueError("Invalid file type '%s'." % filetype)
+        f = FileLoader()
+        if filetype == "cassandra":
+            f.load_cassandra(filename)
+        else:
+            f.load_raw_mdp(filename, scalarize)
+        goals = np.zeros(0, np.uint32) if f.goals is None else np.asarray(f.goals, np.uint32)
+        self.uninitialize_gpu()
+        self.uninitialize()
+        self.initialize(f.n, f.ns, f.m, f.gamma, f.horizon, f.epsilon, f.s0, goals.size)
+        fill(self.S, f.S, np.int32)
+        fill(self.T, f.T, np.float32)
+        fill(self.R, f.R, np.float32)
+        if goals.size:
+            fill(self.goals, goals, np.uint32)
+        self.Rmin, self.Rmax = f.Rmin, f.Rmax
+        return self
+
     def arrays(self):
         """(S, T, R) as numpy copies shaped [n,m,ns], [n,m,ns], [n,m]."""
         n, m, ns = self.n, self.m, self.ns

@@ -76,6 +76,43 @@ class POMDP(_abi.NovaPOMDP):
         p.Rmax = float(R32.max())
         return p
 
+    def load(self, filename, filetype="cassandra", scalarize=lambda x: x[0]):
+        """Loads a POMDP file (python/nova/pomdp.py:149-200): 'cassandra' (.pomdp text) or
+        'raw' (nova's CSV layout); scalarize picks one reward from a raw file's k rewards.
+        Replaces this object's model; the initial belief of the file is belief row 0."""
+        from .file_loader import FileLoader
+        if filetype not in ("cassandra", "raw"):
+            raise ValueError("Invalid file type '%s'." % filetype)
+        f = FileLoader()
+        if filetype == "cassandra":
+            f.load_cassandra(filename)
+        else:
+            f.load_raw_pomdp(filename, scalarize)
+        if f.Z is None:
+            raise ValueError("'%s' gives no initial belief" % filename)
+        self.uninitialize_gpu()
+        self.uninitialize()
+        self.initialize(f.n, f.ns, f.m, f.z, f.r, f.rz, f.gamma, f.horizon)
+        fill(self.S, f.S, np.int32)
+        fill(self.T, f.T, np.float32)
+        fill(self.O, f.O, np.float32)
+        fill(self.R, f.R, np.float32)
+        fill(self.Z, f.Z, np.int32)
+        fill(self.B, f.B, np.float32)
+        self.Rmin, self.Rmax = f.Rmin, f.Rmax
+        return self
+
+    def arrays(self):
+        """dict(S, T, O, R, Z, B, gamma, horizon) as numpy copies in nova's layouts."""
+        n, m, ns, z, r, rz = self.n, self.m, self.ns, self.z, self.r, self.rz
+        return dict(S=view(self.S, n * m * ns, np.int32).reshape(n, m, ns),
+                    T=view(self.T, n * m * ns, np.float32).reshape(n, m, ns),
+                    O=view(self.O, m * n * z, np.float32).reshape(m, n, z),
+                    R=view(self.R, n * m, np.float32).reshape(n, m),
+                    Z=view(self.Z, r * rz, np.int32).reshape(r, rz),
+                    B=view(self.B, r * rz, np.float32).reshape(r, rz),
+                    gamma=np.float32(self.gamma), horizon=int(self.horizon))
+
     @classmethod
     def from_dict(cls, P):
         return cls.from_arrays(P["S"], P["T"], P["O"], P["R"], P["Z"], P["B"], P["gamma"],

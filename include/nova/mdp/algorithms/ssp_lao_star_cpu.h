@@ -1,6 +1,6 @@
-// nova-mi355x: SSP LAO* (replaces include/nova/mdp/algorithms/ssp_lao_star_cpu.h:42-97).
-// Outside the Bellman-backup path this library accelerates: the symbols are exported so the
-// ctypes layer imports, and each returns NOVA_ERROR_INVALID_DATA with a message on stderr.
+// nova-mi355x: SSP LAO* (replaces include/nova/mdp/algorithms/ssp_lao_star_cpu.h:42-97), on the
+// host as in the reference. Same struct layout and entry points; R holds positive costs and
+// goals are absorbing.
 #ifndef NOVA_SSP_LAO_STAR_CPU_H
 #define NOVA_SSP_LAO_STAR_CPU_H
 

@@ -1,6 +1,6 @@
-// nova-mi355x: SSP RTDP (replaces include/nova/mdp/algorithms/ssp_rtdp_cpu.h:44-106).
-// Outside the Bellman-backup path: exported for ctypes import, each returns
-// NOVA_ERROR_INVALID_DATA with a message on stderr.
+// nova-mi355x: SSP RTDP (replaces include/nova/mdp/algorithms/ssp_rtdp_cpu.h:44-106), on the
+// host as in the reference. Same struct layout and entry points; R holds positive costs and
+// goals are absorbing.
 #ifndef NOVA_SSP_RTDP_CPU_H
 #define NOVA_SSP_RTDP_CPU_H
 

@@ -43,6 +43,8 @@ class Oracle:
             getattr(self.lib, "orc_sparse_vb_" + sfx).restype = R
         self.lib.orc_vi_solve_f64.restype = ct.c_uint
         self.lib.orc_adjoint_f64.restype = ct.c_uint
+        self.lib.orc_ssp_lao_star.restype = ct.c_uint
+        self.lib.orc_ssp_vi_f64.restype = ct.c_uint
 
     @staticmethod
     def _sfx(dtype):
@@ -201,6 +203,56 @@ class Oracle:
         out = 0 if horizon % 2 == 0 else 1
         return G[out][:counts[out]].copy(), pis[out][:counts[out]].copy()
 
+    # ---- stochastic shortest paths (src/mdp/algorithms/ssp_lao_star_cpu.cpp,
+    #      src/mdp/algorithms/ssp_rtdp_cpu.cpp)
+    @staticmethod
+    def _ssp_args(S, T, R, goals):
+        S, T, R = _c(S, np.int32), _c(T, np.float32), _c(R, np.float32)
+        goals = _c(np.asarray(goals), np.uint32)
+        return S, T, R, goals
+
+    def ssp_lao_star(self, S, T, R, s0, goals, horizon, epsilon, V0=None):
+        """LAO* to convergence; returns (V, pi, passes) over all n states: V >= 0 (positive
+        sign) on the states of the final solution graph, pi == m where never expanded."""
+        S, T, R, goals = self._ssp_args(S, T, R, goals)
+        n, m, ns = S.shape
+        V = np.empty(n, np.float32)
+        pi = np.empty(n, np.uint32)
+        stack = np.empty(n * ns + 1, np.uint32)
+        order = np.empty(n, np.uint32)
+        v0 = None if V0 is None else _c(V0, np.float32)
+        h = self.lib.orc_ssp_lao_star(n, ns, m, _p(S), _p(T), _p(R), s0, len(goals), _p(goals),
+                                      horizon, ct.c_float(epsilon), _p(v0), _p(V), _p(pi),
+                                      _p(stack), _p(order))
+        return V, pi, h
+
+    def ssp_rtdp(self, S, T, R, s0, goals, horizon, trials, rand, rand_max=2147483647,
+                 V0=None):
+        """`trials` RTDP trials drawing successors with rand() (e.g. libc's, seeded with
+        srand); returns (V, pi) over all n states, pi == m where never expanded."""
+        S, T, R, goals = self._ssp_args(S, T, R, goals)
+        n, m, ns = S.shape
+        V = np.empty(n, np.float32)
+        pi = np.empty(n, np.uint32)
+        cb = ct.CFUNCTYPE(ct.c_int)(lambda: int(rand()))
+        v0 = None if V0 is None else _c(V0, np.float32)
+        self.lib.orc_ssp_rtdp(n, ns, m, _p(S), _p(T), _p(R), s0, len(goals), _p(goals), horizon,
+                              trials, _p(v0), cb, rand_max, _p(V), _p(pi))
+        return V, pi
+
+    def ssp_vi_f64(self, S, T, R, goals, max_iter=1000000, tol=1e-12):
+        """Optimal SSP values in FP64 (goals absorbing at 0); returns (V, pi, sweeps, resid)."""
+        S, T, R, goals = self._ssp_args(S, T, R, goals)
+        n, m, ns = S.shape
+        V = np.empty(n)
+        work = np.empty(n)
+        pi = np.zeros(n, np.uint32)
+        res = ct.c_double()
+        it = self.lib.orc_ssp_vi_f64(n, ns, m, _p(S), _p(T), _p(R), len(goals), _p(goals),
+                                     _p(V), _p(work), _p(pi), max_iter, ct.c_double(tol),
+                                     ct.byref(res))
+        return V, pi, it, res.value
+
     # ---- FP64 references for the differentiable solve
     def vi_solve_f64(self, S, T, R, gamma, V0=None, max_iter=100000, tol=1e-12):
         n, m, ns = S.shape
@@ -282,6 +334,29 @@ class RefLib:
         pi = np.ctypeslib.as_array(pol.pi, (n,)).copy()
         self.lib.mdp_value_function_uninitialize(ct.byref(pol))
         return V, pi
+
+    def ssp_rtdp_execute(self, S, T, R, s0, goals, horizon, trials, V0=None):
+        """nova's ssp_rtdp_execute_cpu; returns (states, V, pi) of its policy. Draws with the
+        process's libc rand(): seed srand() first for a reproducible run."""
+        a = self._abi
+        mdp, keep = self._mdp(S, T, R, 1.0, horizon)
+        g = _c(np.asarray(goals), np.uint32)
+        keep.append(g)
+        mdp.s0, mdp.ng = int(s0), len(g)
+        mdp.goals = ct.cast(g.ctypes.data, a.c_uint_p)
+        rt = a.NovaSSPRTDPCPU()
+        rt.trials = int(trials)
+        if V0 is not None:
+            v0 = _c(V0, np.float32)
+            keep.append(v0)
+            rt.VInitial = ct.cast(v0.ctypes.data, a.c_float_p)
+        pol = a.NovaMDPValueFunction()
+        rc = self.lib.ssp_rtdp_execute_cpu(ct.byref(mdp), ct.byref(rt), ct.byref(pol))
+        assert rc == 0, rc
+        r = pol.r
+        out = tuple(np.ctypeslib.as_array(x, (r,)).copy() for x in (pol.S, pol.V, pol.pi))
+        self.lib.mdp_value_function_uninitialize(ct.byref(pol))
+        return out
 
     def vi_time_sweeps(self, S, T, R, gamma, sweeps, V0=None):
         """Seconds for `sweeps` calls of nova's mdp_vi_update_cpu (after its initialize)."""

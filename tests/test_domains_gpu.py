@@ -179,3 +179,42 @@ def test_fast_run_on_domain(oracle, name, k, horizon):
     print("%s: whole run of %d iterations, max_b |V(b) - V_fp64(b)| / ||V|| = %.3g (nova's "
           "float arithmetic: %.3g)" % (name, horizon, np.max(np.abs(v - v64)) / sc,
                                        np.max(np.abs(v32 - v64)) / sc), flush=True)
+
+
+RAW = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "raw")
+
+
+@pytest.mark.parametrize("name", ["grid_world_mdp.raw", "another_grid_world_mdp.raw"])
+def test_grid_world_vi_gpu(oracle, name):
+    """Value iteration on nova's raw grid worlds (tests/grid_world/grid_world.py:39-47 solves
+    them on the CPU and the GPU): bit-identical to the oracle over the file's 250 sweeps."""
+    from nova_amd.mdp import MDP
+    from nova_amd.mdp_vi import MDPValueIterationGPU, _default_v0
+    print("grid world %s: value iteration on the GPU" % name, flush=True)
+    mdp = MDP().load(os.path.join(RAW, name), filetype="raw")
+    S, T, R = mdp.arrays()
+    mdp.initialize_gpu()
+    pol = MDPValueIterationGPU(mdp).solve()
+    Vo, pio = oracle.vi_run(S, T, R, mdp.gamma, mdp.horizon, V0=_default_v0(mdp))
+    assert np.array_equal(bits(pol.V_array()), bits(Vo))
+    assert np.array_equal(pol.pi_array(), pio)
+
+
+@pytest.mark.parametrize("mode", [EXACT, FAST])
+def test_tiger_raw_pbvi_gpu(oracle, mode):
+    """PBVI on the raw tiger's 21 beliefs over its 250 iterations (tests/tiger/tiger.py:37-47
+    solves it on the GPU): exact mode bit-identical to the oracle, fast mode within 1e-5
+    relative of the FP64 oracle's values."""
+    print("tiger_pomdp.raw: PBVI mode %d" % mode, flush=True)
+    p = POMDP().load(os.path.join(RAW, "tiger_pomdp.raw"), filetype="raw")
+    P = p.arrays()
+    P["Gamma0"] = np.full((P["Z"].shape[0], 2),
+                          float(P["R"].min()) / (1 - float(P["gamma"])), np.float32)
+    G, pi = gpu_solve(P, mode)
+    if mode == EXACT:
+        Go, pio = oracle.pbvi_run(P)
+        assert np.array_equal(bits(G), bits(Go)) and np.array_equal(pi, pio)
+    else:
+        G64, _ = oracle.pbvi_run(P, dtype=np.float64)
+        v, v64 = values(P, G), values(P, G64)
+        assert np.max(np.abs(v - v64)) <= 1e-5 * max(1.0, np.max(np.abs(v64)))

@@ -193,3 +193,84 @@ def test_bad_file_is_reported(tmp_path):
                     "T: 0 : nowhere : 0 1.0\n")
     with pytest.raises(ValueError):
         FileLoader().load_cassandra(str(path))
+
+
+# ------------------------------------------------------------- nova's raw CSV test models
+
+RAW = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "raw")
+
+
+def test_raw_tiger_is_the_cassandra_tiger():
+    """tests/tiger/tiger_pomdp.raw and tiger_95.pomdp describe one model in two formats."""
+    raw = FileLoader()
+    raw.load_raw_pomdp(os.path.join(RAW, "tiger_pomdp.raw"))
+    cas = load("tiger_95.pomdp")
+    for key in ("S", "T", "O", "R"):
+        np.testing.assert_array_equal(getattr(raw, key), getattr(cas, key), err_msg=key)
+    assert (raw.n, raw.m, raw.z, raw.r, raw.rz, raw.horizon) == (2, 3, 2, 21, 2, 250)
+    assert raw.gamma == pytest.approx(0.95)
+    # 21 beliefs sweeping b(s1) from 0 to 1: the first and last as one-state supports
+    np.testing.assert_array_equal(raw.Z[0], [0, -1])
+    np.testing.assert_array_equal(raw.Z[20], [1, -1])
+    np.testing.assert_array_equal(raw.Z[1:20], np.tile([0, 1], (19, 1)))
+    i = np.arange(1, 20)
+    np.testing.assert_allclose(raw.B[1:20, 1], 0.05 * i, atol=1e-7)
+    np.testing.assert_allclose(raw.B.sum(axis=1), 1.0, atol=1e-6)
+
+
+def test_raw_grid_world_header():
+    f = FileLoader()
+    f.load_raw_mdp(os.path.join(RAW, "grid_world_mdp.raw"))
+    assert (f.n, f.ns, f.m, f.k, f.s0, f.ng, f.horizon) == (13, 3, 4, 1, 8, 1, 250)
+    assert f.gamma == pytest.approx(0.9)
+    np.testing.assert_array_equal(f.goals, [12])
+    valid = f.S >= 0
+    np.testing.assert_allclose(np.where(valid, f.T, 0).sum(axis=2), 1.0, atol=1e-6)
+    g = FileLoader()
+    g.load_raw_pomdp(os.path.join(RAW, "grid_world_pomdp.raw"))
+    assert (g.n, g.ns, g.m, g.z, g.r, g.rz, g.s0) == (13, 3, 4, 3, 21, 4, 8)
+    np.testing.assert_array_equal(g.S, f.S)
+    np.testing.assert_allclose(g.O.sum(axis=2), 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["grid_world_mdp.raw", "another_grid_world_mdp.raw"])
+def test_grid_world_vi_cpu(reflib, oracle, name):
+    """The library's CPU value iteration on nova's grid worlds: bit-identical to nova's own
+    CPU path, and after 250 sweeps at gamma = 0.9 (contraction 0.9^250 ~ 4e-12) at the FP64
+    Bellman fixed point within float rounding."""
+    from conftest import bits
+    from nova_amd.mdp import MDP
+    from nova_amd.mdp_vi import MDPValueIterationCPU, _default_v0
+    mdp = MDP().load(os.path.join(RAW, name), filetype="raw")
+    S, T, R = mdp.arrays()
+    pol = MDPValueIterationCPU(mdp).solve()
+    V, pi = pol.V_array(), pol.pi_array()
+    Vr, pir = reflib.vi_execute(S, T, R, mdp.gamma, mdp.horizon, V0=_default_v0(mdp))
+    assert np.array_equal(bits(V), bits(Vr)) and np.array_equal(pi, pir)
+    Vs, _, _, res = oracle.vi_solve_f64(S, T, R, mdp.gamma)
+    assert res < 1e-10
+    assert np.max(np.abs(V - Vs)) <= 1e-5 * max(1.0, np.max(np.abs(Vs)))
+
+
+def test_tiger_raw_pbvi_cpu(reflib):
+    """PBVI on the raw tiger's 21 beliefs over its 250 iterations: bit-identical to nova's CPU
+    path, and the values mirror under swapping the two tiger positions (b_i and b_{20-i})."""
+    from conftest import bits
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import POMDPPBVICPU
+    p = POMDP().load(os.path.join(RAW, "tiger_pomdp.raw"), filetype="raw")
+    P = p.arrays()
+    g0 = np.full((P["Z"].shape[0], 2), float(P["R"].min()) / (1 - float(P["gamma"])),
+                 np.float32)
+    P["Gamma0"] = g0
+    pol = POMDPPBVICPU(p, GammaInitial=g0).solve()
+    G, pi = pol.Gamma_array(), pol.pi_array()
+    Gr, pir = reflib.pbvi_execute(P)
+    assert np.array_equal(bits(G), bits(Gr)) and np.array_equal(pi, pir)
+    b = np.zeros((21, 2))
+    for i in range(21):
+        for k in range(2):
+            if P["Z"][i, k] >= 0:
+                b[i, P["Z"][i, k]] = P["B"][i, k]
+    V = (b @ G.T.astype(np.float64)).max(axis=1)
+    np.testing.assert_allclose(V, V[::-1], rtol=1e-4)

@@ -1,0 +1,257 @@
+"""Stochastic shortest paths: LAO* and RTDP (src/mdp/algorithms/ssp_lao_star_cpu.cpp,
+src/mdp/algorithms/ssp_rtdp_cpu.cpp), host solvers of the drop-in ABI. CPU only.
+
+The oracle's restatements are pinned first:
+* RTDP bit for bit against nova's own build (oracle/_ref) on the same libc rand() stream;
+* LAO* (nova's build of it cannot run: its expand step returns no value) against the optimal
+  SSP values, from FP64 value iteration that is itself pinned by a closed form and by the
+  linear system of its greedy policy;
+then the library's solvers bit for bit against the oracle, on nova's grid worlds
+(tests/grid_world/domains/*_ssp.raw there) and on seeded random SSPs.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+import pytest
+
+from nova_amd.mdp import MDP
+from nova_amd.ssp import SSPLAOStarCPU, SSPRTDPCPU
+
+RAW = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "raw")
+libc = ct.CDLL("libc.so.6")
+
+
+def grid(name):
+    mdp = MDP().load(os.path.join(RAW, name + ".raw"), filetype="raw")
+    S, T, R = mdp.arrays()
+    goals = np.ctypeslib.as_array(mdp.goals, (mdp.ng,)).copy()
+    return mdp, S, T, R, goals
+
+
+def random_ssp(n, m, ns, seed, n_goals=3, n_dead=0):
+    """Successors uniform without replacement, T Dirichlet(1), costs uniform [1, 2]; goals
+    absorbing at cost 0; dead ends certain self-loops of cost 5; s0 an ordinary state."""
+    rng = np.random.default_rng(seed)
+    S = np.stack([rng.choice(n, ns, replace=False) for _ in range(n * m)]).reshape(n, m, ns)
+    T = rng.exponential(size=(n, m, ns))
+    T = (T / T.sum(axis=2, keepdims=True)).astype(np.float32)
+    R = rng.uniform(1, 2, size=(n, m)).astype(np.float32)
+    special = rng.choice(n, n_goals + n_dead + 1, replace=False)
+    goals, dead, s0 = special[:n_goals], special[n_goals:n_goals + n_dead], int(special[-1])
+    for s, c in [(g, 0.0) for g in goals] + [(d, 5.0) for d in dead]:
+        S[s] = -1
+        S[s, :, 0] = s
+        T[s] = 0
+        T[s, :, 0] = 1
+        R[s] = c
+    return S.astype(np.int32), T, R, s0, goals.astype(np.uint32)
+
+
+def to_mdp(S, T, R, s0, goals, horizon, epsilon):
+    n, m, ns = S.shape
+    mdp = MDP.from_arrays(S, T, R, gamma=1.0, horizon=horizon, epsilon=epsilon, s0=s0,
+                          goals=goals)
+    return mdp
+
+
+# ------------------------------------------------------------------------ oracle pins
+
+def test_ssp_vi_f64_closed_form(oracle):
+    # state 0: action 0 reaches the goal (state 1) w.p. 1/2 else stays, cost 1 -> V = 2;
+    # action 1 reaches it surely at cost 3. Goal: absorbing, cost 0.
+    S = np.array([[[1, 0], [1, -1]], [[1, -1], [1, -1]]], np.int32)
+    T = np.array([[[0.5, 0.5], [1, 0]], [[1, 0], [1, 0]]], np.float32)
+    R = np.array([[1, 3], [0, 0]], np.float32)
+    V, pi, it, res = oracle.ssp_vi_f64(S, T, R, [1])
+    assert res < 1e-12 and abs(V[0] - 2.0) < 1e-10 and V[1] == 0.0 and pi[0] == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_ssp_vi_f64_solves_its_policy_system(oracle, seed):
+    S, T, R, s0, goals = random_ssp(120, 3, 4, seed)
+    V, pi, it, res = oracle.ssp_vi_f64(S, T, R, goals)
+    assert res < 1e-11
+    n = S.shape[0]
+    # V = R_pi + P_pi V on the ordinary states, V = 0 at goals: a dense solve
+    A = np.eye(n)
+    b = np.zeros(n)
+    for s in range(n):
+        if s in goals:
+            continue
+        a = pi[s]
+        b[s] = R[s, a]
+        for i in range(S.shape[2]):
+            if S[s, a, i] < 0:
+                break
+            A[s, S[s, a, i]] -= T[s, a, i]
+    Vd = np.linalg.solve(A, b)
+    assert np.max(np.abs(V - Vd)) < 1e-8 * max(1.0, np.max(np.abs(Vd)))
+    # and no action improves on it (Bellman optimality, 1e-9 slack)
+    for s in range(n):
+        if s in goals:
+            continue
+        for a in range(S.shape[1]):
+            q = R[s, a] + sum(T[s, a, i] * V[S[s, a, i]] for i in range(S.shape[2])
+                              if S[s, a, i] >= 0)
+            assert q >= V[s] - 1e-9
+
+
+def test_oracle_lao_star_closed_form(oracle):
+    S = np.array([[[1, 0], [1, -1]], [[1, -1], [1, -1]]], np.int32)
+    T = np.array([[[0.5, 0.5], [1, 0]], [[1, 0], [1, 0]]], np.float32)
+    R = np.array([[1, 3], [0, 0]], np.float32)
+    V, pi, h = oracle.ssp_lao_star(S, T, R, 0, [1], 1000, 1e-6)
+    assert abs(V[0] - 2.0) < 1e-5 and pi[0] == 0 and V[1] == 0.0
+
+
+def test_oracle_lao_star_maze_is_exact(oracle):
+    """another_grid_world_ssp is deterministic (ns = 1): with passes enough, LAO* ends with
+    V(s0) equal to the shortest path cost and its solution graph is exactly that path."""
+    mdp, S, T, R, goals = grid("another_grid_world_ssp")
+    Vs, pis, _, _ = oracle.ssp_vi_f64(S, T, R, goals, max_iter=5000)
+    V, pi, h = oracle.ssp_lao_star(S, T, R, mdp.s0, goals, 5000, mdp.epsilon)
+    assert h < 5000
+    assert V[mdp.s0] == Vs[mdp.s0] == 25.0
+    path, s = [mdp.s0], mdp.s0
+    while s not in goals:
+        s = int(S[s, pis[s], 0])
+        path.append(s)
+    assert sorted(path) == sorted(np.nonzero(~np.signbit(V))[0].tolist())
+
+
+@pytest.mark.parametrize("case", ["grid_world_ssp", "random"])
+def test_oracle_lao_star_reaches_optimal_values(oracle, case):
+    """With a tight epsilon LAO* stops at the optimal values on its solution graph, whose
+    actions are optimal (checked against FP64 value iteration)."""
+    if case == "random":
+        S, T, R, s0, goals = random_ssp(400, 4, 4, 11)
+    else:
+        mdp, S, T, R, goals = grid(case)
+        s0 = mdp.s0
+    Vs, pis, _, res = oracle.ssp_vi_f64(S, T, R, goals)
+    assert res < 1e-10
+    V, pi, h = oracle.ssp_lao_star(S, T, R, s0, goals, 100000, 1e-7)
+    assert h < 100000
+    keep = ~np.signbit(V)
+    assert keep[s0]
+    err = np.max(np.abs(V[keep] - Vs[keep])) / max(1.0, np.max(np.abs(Vs[keep])))
+    assert err < 1e-5, err
+    ordinary = keep & ~np.isin(np.arange(len(V)), goals)
+    assert np.array_equal(pi[ordinary], pis[ordinary])
+
+
+@pytest.mark.parametrize("seed", [3, 17])
+@pytest.mark.parametrize("case", ["grid_world_ssp", "another_grid_world_ssp", "random",
+                                  "random_dead_ends"])
+def test_oracle_rtdp_bitwise_vs_reference(oracle, reflib, case, seed):
+    if case.startswith("random"):
+        S, T, R, s0, goals = random_ssp(300, 4, 5, seed, n_dead=3 if "dead" in case else 0)
+        horizon = 200
+    else:
+        mdp, S, T, R, goals = grid(case)
+        s0, horizon = mdp.s0, mdp.horizon
+    libc.srand(seed)
+    V, pi = oracle.ssp_rtdp(S, T, R, s0, goals, horizon, 60, libc.rand)
+    libc.srand(seed)
+    Sr, Vr, pir = reflib.ssp_rtdp_execute(S, T, R, s0, goals, horizon, 60)
+    kept = np.nonzero(pi < S.shape[1])[0]
+    assert np.array_equal(kept, Sr)
+    assert np.array_equal(V[kept].view(np.uint32), Vr.view(np.uint32))
+    assert np.array_equal(pi[kept], pir)
+
+
+def test_oracle_rtdp_converges_to_optimal(oracle):
+    """Trials from an admissible (zero) heuristic drive V(s0) up to V*(s0) on grid_world_ssp."""
+    mdp, S, T, R, goals = grid("grid_world_ssp")
+    Vs, _, _, _ = oracle.ssp_vi_f64(S, T, R, goals)
+    libc.srand(1)
+    V, pi = oracle.ssp_rtdp(S, T, R, mdp.s0, goals, mdp.horizon, 3000, libc.rand)
+    assert abs(V[mdp.s0] - Vs[mdp.s0]) < 1e-3 * Vs[mdp.s0]
+
+
+# ------------------------------------------------------------- the library's solvers
+
+def _policy(pol):
+    r = pol.r
+    return (np.ctypeslib.as_array(pol.S, (r,)).copy(), pol.V_array().copy(),
+            pol.pi_array().copy())
+
+
+CASES = ["grid_world_ssp", "another_grid_world_ssp", "random", "random_dead_ends"]
+
+
+def _case(case, seed=5):
+    if case.startswith("random"):
+        S, T, R, s0, goals = random_ssp(300, 4, 5, seed, n_dead=3 if "dead" in case else 0)
+        return to_mdp(S, T, R, s0, goals, 200, 1e-3), S, T, R, s0, goals
+    mdp, S, T, R, goals = grid(case)
+    return mdp, S, T, R, mdp.s0, goals
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_lao_star_bitwise_vs_oracle(oracle, case):
+    mdp, S, T, R, s0, goals = _case(case)
+    Sp, Vp, pip = _policy(SSPLAOStarCPU(mdp).solve())
+    V, pi, _ = oracle.ssp_lao_star(S, T, R, s0, goals, mdp.horizon, mdp.epsilon)
+    kept = np.nonzero(~np.signbit(V))[0]
+    assert np.array_equal(Sp, kept)
+    assert np.array_equal(Vp.view(np.uint32), V[kept].view(np.uint32))
+    assert np.array_equal(pip, pi[kept])
+
+
+def test_lao_star_heuristic(oracle):
+    """A heuristic VInitial is used as given: the optimal values themselves (in float) make
+    the first passes converge at once."""
+    mdp, S, T, R, goals = grid("grid_world_ssp")
+    Vs, _, _, _ = oracle.ssp_vi_f64(S, T, R, goals)
+    h = Vs.astype(np.float32)
+    Sp, Vp, pip = _policy(SSPLAOStarCPU(mdp, VInitial=h).solve())
+    V, pi, passes = oracle.ssp_lao_star(S, T, R, mdp.s0, goals, mdp.horizon, mdp.epsilon, V0=h)
+    kept = np.nonzero(~np.signbit(V))[0]
+    assert np.array_equal(Sp, kept) and np.array_equal(Vp.view(np.uint32),
+                                                       V[kept].view(np.uint32))
+    assert passes < 10
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_rtdp_bitwise_vs_oracle_and_reference(oracle, reflib, case):
+    mdp, S, T, R, s0, goals = _case(case)
+    libc.srand(9)
+    Sp, Vp, pip = _policy(SSPRTDPCPU(mdp, trials=80).solve())
+    libc.srand(9)
+    V, pi = oracle.ssp_rtdp(S, T, R, s0, goals, mdp.horizon, 80, libc.rand)
+    kept = np.nonzero(pi < S.shape[1])[0]
+    assert np.array_equal(Sp, kept)
+    assert np.array_equal(Vp.view(np.uint32), V[kept].view(np.uint32))
+    assert np.array_equal(pip, pi[kept])
+    libc.srand(9)
+    Sr, Vr, pir = reflib.ssp_rtdp_execute(S, T, R, s0, goals, mdp.horizon, 80)
+    assert np.array_equal(Sp, Sr) and np.array_equal(Vp.view(np.uint32), Vr.view(np.uint32))
+
+
+def test_update_steps_match_execute(oracle):
+    """Driving LAO* through initialize / update / get_policy gives execute's result once the
+    states start unexpanded (execute sets pi = m after its initialize, as nova's does)."""
+    mdp, S, T, R, goals = grid("grid_world_ssp")
+    ref = _policy(SSPLAOStarCPU(mdp).solve())
+    lao = SSPLAOStarCPU(mdp)
+    for s in range(mdp.n):
+        lao.pi[s] = mdp.m
+    steps = 0
+    while lao.update() != 9:
+        steps += 1
+        assert steps < 1000
+    got = _policy(lao.get_policy())
+    assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+
+
+def test_invalid_arguments_are_errors():
+    mdp, S, T, R, goals = grid("grid_world_ssp")
+    r = SSPRTDPCPU(mdp, trials=1)
+    r.trials = 0
+    with pytest.raises(Exception):
+        r.solve()
+    no_goals = MDP.from_arrays(S, T, R, gamma=1.0, horizon=10)
+    with pytest.raises(Exception):
+        SSPLAOStarCPU(no_goals).solve()

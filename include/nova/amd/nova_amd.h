@@ -126,6 +126,34 @@ extern "C" int pomdp_expand_random_seeded_gpu(const POMDP *pomdp, unsigned int s
 // (src/pomdp/utilities/pomdp_expand_cpu.cpp:166-266). Needs rz*ns <= 1024.
 extern "C" int pomdp_expand_distinct_beliefs_gpu(POMDP *pomdp);
 
+// ---- executing alpha-vector policies over many beliefs (no counterpart in the reference,
+//      which evaluates one belief per call)
+
+// pomdp_alpha_vectors_value_and_action (src/pomdp/policies/pomdp_alpha_vectors.cpp:52-79) for
+// k dense beliefs at once. Device arrays: d_Gamma (r x n alphas, row-major), d_pi (r actions),
+// d_B (k x n beliefs, row-major); outputs d_V (k values), d_action (k actions) and, when not
+// null, d_alpha (k indices of the chosen alpha, -1 when no alpha beats -1e10, in which case the
+// value is -1e10 and the action 0, as the reference leaves them). precision: 0 = exact (each
+// dot product summed over the states in order with separately rounded products and sums:
+// equal to the reference bit for bit), 1 = fp32 on the matrix cores, 2 = bf16 operands with
+// fp32 accumulation on the matrix cores (the -1e10 sentinel itself rounds to -9.9992e9 there).
+// r <= 4,194,240. Synchronous; calls are serialised.
+extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsigned int r,
+        const float *d_Gamma, const unsigned int *d_pi, unsigned int k, const float *d_B,
+        int precision, float *d_V, unsigned int *d_action, int *d_alpha);
+
+// pomdp_belief_update_cpu (src/pomdp/utilities/pomdp_model_cpu.cpp:89-137) for k (belief,
+// action, observation) triples at once, bit for bit: d_B (k x n beliefs), d_action and
+// d_observation (k each) on the device; d_Bp (k x n) receives the updated beliefs and
+// d_status (k ints) NOVA_SUCCESS, NOVA_WARNING_INVALID_BELIEF (the observation is impossible:
+// the row keeps the unnormalised values, as the reference returns them) or
+// NOVA_ERROR_INVALID_DATA (action or observation out of range). The POMDP must be resident
+// on the device and keep its host S and T, from which predecessor lists are built on first
+// use and kept until another model is passed. Synchronous; calls are serialised.
+extern "C" int nova_amd_belief_update_batch_gpu(const POMDP *pomdp, unsigned int k,
+        const float *d_B, const unsigned int *d_action, const unsigned int *d_observation,
+        float *d_Bp, int *d_status);
+
 // delete[] of a float array the library allocated for the caller (Bnew of the expansions,
 // bp of pomdp_belief_update_cpu), for callers that cannot call delete[] themselves.
 extern "C" int nova_amd_free_floats(float *p);

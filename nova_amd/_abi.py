@@ -203,6 +203,11 @@ EXTRA_SIGNATURES = {
     "pomdp_expand_distinct_beliefs_gpu": [_POMDP],
     "pomdp_pbvi_scoring_variant_gpu": [_PBG, _P(ct.c_uint)],
     "pomdp_perseus_execute_gpu": [_POMDP, _PER, _PAV],
+    "nova_amd_alpha_vectors_value_and_action_gpu": [U, U, ct.c_void_p, ct.c_void_p, U,
+                                                    ct.c_void_p, ct.c_int, ct.c_void_p,
+                                                    ct.c_void_p, ct.c_void_p],
+    "nova_amd_belief_update_batch_gpu": [_POMDP, U, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p, ct.c_void_p],
     "mdp_vi_solve_gpu": [_MDP, _VIG, ct.c_void_p, ct.c_double, U, U, _P(ct.c_uint),
                          _P(ct.c_double)],
     "mdp_vi_adjoint_gpu": [_MDP, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,

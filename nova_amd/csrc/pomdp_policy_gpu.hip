@@ -1,0 +1,473 @@
+// nova-mi355x: executing alpha-vector policies over many beliefs at once on one MI355X.
+//
+// 1. Value and action of k dense beliefs (src/pomdp/policies/pomdp_alpha_vectors.cpp:52-79 per
+//    belief: V(b) = max_j Gamma_j . b over the r alphas, the action pi[j*] of the first
+//    maximiser, V = -1e10 and action 0 when no alpha beats -1e10). Three arithmetic modes:
+//      exact - every dot product summed over s = 0..n-1 with each product and sum rounded
+//              separately, as the reference: V and the action equal nova's bit for bit. A
+//              vector-ALU tile kernel: 64 beliefs x 64 alphas per workgroup, each lane a 4 x 4
+//              block, both operands staged state-major in LDS 32 states at a time.
+//      fp32  - the dense GEMM of dense_gemm.h on the matrix cores (v_mfma_f32_32x32x2_f32):
+//              fp32 products, fp32 sums in the matrix cores' order.
+//      bf16  - the same GEMM on bf16 operands with fp32 accumulation
+//              (v_mfma_f32_32x32x16_bf16): about 3 significant digits per operand.
+//    All modes write per-(belief, alpha tile) maxima that one merge kernel reduces in tile
+//    order (the earlier tile wins ties, i.e. the first maximiser overall).
+// 2. Belief updates of k (belief, action, observation) triples
+//    (src/pomdp/utilities/pomdp_model_cpu.cpp:89-137 per triple): b'(s') = O[a,s',o] *
+//    sum_s T[s,a,s'] b(s), normalised by its sum over s'. The sum over predecessors runs over a
+//    list ordered by (s, successor slot) -- the order in which the reference's scatter adds to
+//    b'(s') -- and the normaliser is summed serially over s' ascending, so b' equals nova's
+//    bit for bit. A triple whose normaliser is below 1e-9 in magnitude keeps the unnormalised
+//    b' and reports NOVA_WARNING_INVALID_BELIEF, as the reference returns it.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include <nova/amd/nova_amd.h>
+#include <nova/pomdp/pomdp.h>
+
+#include "dense_gemm.h"
+#include "nova_hip.h"
+
+namespace nova_amd {
+
+namespace {
+
+constexpr int kVT = 64;          // beliefs and alphas per exact tile
+constexpr int kVS = 32;          // states per LDS step of the exact tile
+constexpr int kVThreads = 256;   // 16 x 16 lanes, each a 4 x 4 block of (belief, alpha)
+
+// Exact tile: partial first maxima of b_i . Gamma_j over the tile's alphas, j ascending, with
+// strict '>' from -1e10 (alpha_vectors.cpp:64-76).
+__global__ __launch_bounds__(kVThreads) void alpha_value_exact(unsigned k, unsigned n,
+        unsigned r, const float *__restrict__ B, const float *__restrict__ G,
+        float *__restrict__ partV, int *__restrict__ partJ)
+{
+    __shared__ __attribute__((aligned(16))) float bs[kVS][kVT + 4];
+    __shared__ __attribute__((aligned(16))) float gs[kVS][kVT + 4];
+    const unsigned i0 = blockIdx.x * kVT, j0 = blockIdx.y * kVT, t = threadIdx.x;
+    const unsigned bi = t / 16, aj = t % 16;
+    float acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+            acc[x][y] = 0.0f;
+    for (unsigned s0 = 0; s0 < n; s0 += kVS) {
+        // stage 64 rows x 32 states of each operand, transposed to state-major; entries past n
+        // (and rows past k or r) are zeros, whose products add +0 and leave every sum as is
+        for (unsigned x = t; x < kVT * kVS; x += kVThreads) {
+            const unsigned row = x / kVS, s = x % kVS;
+            const bool inS = s0 + s < n;
+            bs[s][row] = (inS && i0 + row < k) ? B[(size_t)(i0 + row) * n + s0 + s] : 0.0f;
+            gs[s][row] = (inS && j0 + row < r) ? G[(size_t)(j0 + row) * n + s0 + s] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int s = 0; s < kVS; s++) {
+            const float4 bv = *reinterpret_cast<const float4 *>(&bs[s][bi * 4]);
+            const float4 gv = *reinterpret_cast<const float4 *>(&gs[s][aj * 4]);
+            const float b4[4] = {bv.x, bv.y, bv.z, bv.w};
+            const float g4[4] = {gv.x, gv.y, gv.z, gv.w};
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    acc[x][y] = __fadd_rn(acc[x][y], __fmul_rn(g4[y], b4[x]));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        float bv = kLow;
+        int bj = INT_MAX;
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const unsigned j = j0 + aj * 4 + y;
+            if (j < r && acc[x][y] > bv) {
+                bv = acc[x][y];
+                bj = (int)j;
+            }
+        }
+        // the 16 lanes of this belief row are 16 consecutive lanes of one wave
+        for (int off = 8; off >= 1; off >>= 1) {
+            const float ov = __shfl_xor(bv, off, 16);
+            const int oj = __shfl_xor(bj, off, 16);
+            if (better(ov, oj, bv, bj)) {
+                bv = ov;
+                bj = oj;
+            }
+        }
+        const unsigned i = i0 + bi * 4 + x;
+        if (aj == 0 && i < k) {
+            partV[(size_t)blockIdx.y * k + i] = bv;
+            partJ[(size_t)blockIdx.y * k + i] = bj;
+        }
+    }
+}
+
+// Merge of the alpha tiles per belief, tile 0 first: V, the winning alpha (or -1) and its
+// action (0 when none beat -1e10, as the reference leaves a).
+__global__ __launch_bounds__(256) void alpha_value_merge(unsigned k, unsigned tiles,
+        const float *__restrict__ partV, const int *__restrict__ partJ,
+        const unsigned *__restrict__ pi, float *__restrict__ V, unsigned *__restrict__ act,
+        int *__restrict__ jOut)
+{
+    const unsigned i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= k)
+        return;
+    float bv = kLow;
+    int bj = INT_MAX;
+    for (unsigned tl = 0; tl < tiles; tl++) {
+        const float v = partV[(size_t)tl * k + i];
+        const int j = partJ[(size_t)tl * k + i];
+        if (better(v, j, bv, bj)) {
+            bv = v;
+            bj = j;
+        }
+    }
+    const bool none = (bj == INT_MAX) || !(bv > kLow);
+    V[i] = none ? kLow : bv;
+    act[i] = none ? 0u : pi[bj];
+    if (jOut)
+        jOut[i] = none ? -1 : bj;
+}
+
+__global__ __launch_bounds__(256) void fill_ones(unsigned count, float *__restrict__ x)
+{
+    const unsigned i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count)
+        x[i] = 1.0f;
+}
+
+// One workgroup per triple q: u(s') over the ordered predecessor lists of action a, then
+// b'(s') = u(s') * O[a,s',o] staged in LDS (or written out when n is too large for LDS), the
+// normaliser summed serially by lane 0, and b' divided by it.
+constexpr int kUB = 256;
+constexpr int kUpdLdsFloats = 32 * 1024;
+
+__global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned m, unsigned z,
+        unsigned k, const unsigned *__restrict__ pstart, const int *__restrict__ pS,
+        const float *__restrict__ pT, const float *__restrict__ O, const float *__restrict__ B,
+        const unsigned *__restrict__ act, const unsigned *__restrict__ obs,
+        float *__restrict__ Bp, int *__restrict__ status, int useLds)
+{
+    extern __shared__ float row[];
+    __shared__ float norm;
+    const unsigned q = blockIdx.x, t = threadIdx.x;
+    const unsigned a = act[q], o = obs[q];
+    const float *b = B + (size_t)q * n;
+    float *out = Bp + (size_t)q * n;
+    if (a >= m || o >= z) {
+        if (t == 0)
+            status[q] = NOVA_ERROR_INVALID_DATA;
+        return;
+    }
+    const unsigned *ps = pstart + (size_t)a * (n + 1);
+    const float *Oa = O + (size_t)a * n * z;
+    for (unsigned sp = t; sp < n; sp += kUB) {
+        float u = 0.0f;
+        for (unsigned x = ps[sp]; x < ps[sp + 1]; x++)
+            u = __fadd_rn(u, __fmul_rn(pT[x], b[pS[x]]));
+        const float v = __fmul_rn(u, Oa[(size_t)sp * z + o]);
+        if (useLds)
+            row[sp] = v;
+        out[sp] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        float c = 0.0f;
+        if (useLds)
+            for (unsigned sp = 0; sp < n; sp++)
+                c = __fadd_rn(c, row[sp]);
+        else
+            for (unsigned sp = 0; sp < n; sp++)
+                c = __fadd_rn(c, out[sp]);
+        norm = c;
+    }
+    __syncthreads();
+    const float c = norm;
+    if ((double)fabsf(c) < NOVA_FLT_ERR_TOL) {
+        if (t == 0)
+            status[q] = NOVA_WARNING_INVALID_BELIEF;
+        return;
+    }
+    for (unsigned sp = t; sp < n; sp += kUB)
+        out[sp] = __fdiv_rn(useLds ? row[sp] : out[sp], c);
+    if (t == 0)
+        status[q] = NOVA_SUCCESS;
+}
+
+// A device buffer kept between calls and grown on demand (one per use, all devices share
+// the process's current device; calls are serialised by the mutex of their use).
+struct Arena {
+    void *mem = nullptr;
+    size_t bytes = 0;
+    int device = -1;
+};
+
+int arena_get(Arena &A, size_t bytes, const char *fn, void *&out)
+{
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (A.mem != nullptr && (A.bytes < bytes || A.device != dev)) {
+        hipDeviceSynchronize();
+        device_free(A.mem, fn, "policy workspace");
+        A.mem = nullptr;
+        A.bytes = 0;
+    }
+    if (A.mem == nullptr) {
+        const int rc = device_alloc(&A.mem, bytes, fn, "policy workspace");
+        if (rc != NOVA_SUCCESS)
+            return rc;
+        A.bytes = bytes;
+        A.device = dev;
+    }
+    out = A.mem;
+    return NOVA_SUCCESS;
+}
+
+std::mutex g_valueMutex;
+Arena g_valueArena;
+struct DenseShape {
+    unsigned k = 0, n = 0, r = 0;
+    int precision = -1;
+    void *mem = nullptr;
+} g_valueShape;
+
+std::mutex g_updateMutex;
+// predecessor lists of the last model seen, keyed by its device arrays
+struct PredLists {
+    const void *dS = nullptr, *dT = nullptr;
+    unsigned n = 0, m = 0, ns = 0;
+    void *mem = nullptr;
+    size_t bytes = 0;
+    unsigned *pstart = nullptr;
+    int *pS = nullptr;
+    float *pT = nullptr;
+} g_pred;
+
+inline size_t up256(size_t x) { return (x + 255) / 256 * 256; }
+
+} // namespace
+
+// Predecessor lists per action of p's model, (s, T[s,a,l]) for every valid successor entry,
+// grouped by successor s' and ordered by (s, l) within each group; built on the host from p's
+// host arrays and kept on the device until another model is passed.
+static int pred_lists(const nova::POMDP *p, const char *fn)
+{
+    if (g_pred.mem != nullptr && g_pred.dS == p->d_S && g_pred.dT == p->d_T && g_pred.n == p->n &&
+            g_pred.m == p->m && g_pred.ns == p->ns)
+        return NOVA_SUCCESS;
+    const unsigned n = p->n, m = p->m, ns = p->ns;
+    std::vector<unsigned> pstart((size_t)m * (n + 1), 0);
+    for (unsigned s = 0; s < n; s++)
+        for (unsigned a = 0; a < m; a++)
+            for (unsigned l = 0; l < ns; l++) {
+                const int sp = p->S[idx3(s, a, l, m, ns)];
+                if (sp < 0)
+                    break;
+                pstart[(size_t)a * (n + 1) + sp + 1]++;
+            }
+    size_t total = 0;
+    for (unsigned a = 0; a < m; a++) {
+        unsigned *rowp = &pstart[(size_t)a * (n + 1)];
+        rowp[0] = (unsigned)total;
+        for (unsigned sp = 0; sp < n; sp++)
+            rowp[sp + 1] += rowp[sp];
+        total = rowp[n];
+    }
+    std::vector<int> pS(std::max<size_t>(total, 1));
+    std::vector<float> pT(std::max<size_t>(total, 1));
+    std::vector<unsigned> fill(pstart);
+    for (unsigned s = 0; s < n; s++)
+        for (unsigned a = 0; a < m; a++)
+            for (unsigned l = 0; l < ns; l++) {
+                const size_t x = idx3(s, a, l, m, ns);
+                const int sp = p->S[x];
+                if (sp < 0)
+                    break;
+                const unsigned at = fill[(size_t)a * (n + 1) + sp]++;
+                pS[at] = (int)s;
+                pT[at] = p->T[x];
+            }
+    if (g_pred.mem != nullptr) {
+        hipDeviceSynchronize();
+        device_free(g_pred.mem, fn, "predecessor lists");
+        g_pred = PredLists();
+    }
+    const size_t b0 = up256(sizeof(unsigned) * pstart.size()), b1 = up256(sizeof(int) * pS.size());
+    const size_t bytes = b0 + b1 + up256(sizeof(float) * pT.size());
+    void *mem = nullptr;
+    int rc = device_alloc(&mem, bytes, fn, "predecessor lists");
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    char *c = static_cast<char *>(mem);
+    hipError_t e = hipMemcpy(c, pstart.data(), sizeof(unsigned) * pstart.size(),
+                             hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c + b0, pS.data(), sizeof(int) * pS.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c + b0 + b1, pT.data(), sizeof(float) * pT.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        device_free(mem, fn, "predecessor lists");
+        return hip_fail(e, fn, "Failed to copy the predecessor lists.",
+                        NOVA_ERROR_MEMCPY_TO_DEVICE);
+    }
+    g_pred.dS = p->d_S;
+    g_pred.dT = p->d_T;
+    g_pred.n = n;
+    g_pred.m = m;
+    g_pred.ns = ns;
+    g_pred.mem = mem;
+    g_pred.bytes = bytes;
+    g_pred.pstart = reinterpret_cast<unsigned *>(c);
+    g_pred.pS = reinterpret_cast<int *>(c + b0);
+    g_pred.pT = reinterpret_cast<float *>(c + b0 + b1);
+    return NOVA_SUCCESS;
+}
+
+} // namespace nova_amd
+
+using namespace nova_amd;
+
+extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsigned int r,
+        const float *d_Gamma, const unsigned int *d_pi, unsigned int k, const float *d_B,
+        int precision, float *d_V, unsigned int *d_action, int *d_alpha)
+{
+    const char *fn = "nova_amd_alpha_vectors_value_and_action_gpu";
+    if (n == 0 || r == 0 || k == 0 || d_Gamma == nullptr || d_pi == nullptr ||
+            d_B == nullptr || d_V == nullptr || d_action == nullptr || precision < 0 ||
+            precision > 2 || (r + kVT - 1) / kVT > 65535u) {
+        NOVA_ERR(fn, "Invalid arguments (n, r, k > 0, r <= 4,194,240; device arrays; "
+                     "precision 0, 1 or 2).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    std::lock_guard<std::mutex> lock(g_valueMutex);
+    hipStream_t st = stream();
+    const bool exact = precision == 0;
+    const unsigned tileW = exact ? (unsigned)kVT : (unsigned)kGT;
+    const unsigned colTiles = (r + tileW - 1) / tileW;
+    const unsigned rowTiles = (k + kGT - 1) / kGT;
+    const size_t inSize = precision == 2 ? sizeof(__bf16) : sizeof(float);
+    const unsigned kpad = (n + kGKPad - 1) / kGKPad * kGKPad;
+    const size_t bPart = up256((size_t)colTiles * k * sizeof(float));
+    size_t bytes = 2 * bPart;
+    size_t bA = 0, bG = 0, bS = 0;
+    if (!exact) {
+        bA = up256((size_t)rowTiles * kGT * kpad * inSize);
+        bG = up256((size_t)colTiles * kGT * kpad * inSize);
+        bS = up256(sizeof(float) * kpad);
+        bytes += bA + bG + bS;
+    }
+    void *mem = nullptr;
+    int rc = arena_get(g_valueArena, bytes, fn, mem);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    char *c = static_cast<char *>(mem);
+    float *partV = reinterpret_cast<float *>(c);
+    int *partJ = reinterpret_cast<int *>(c + bPart);
+    hipError_t e = hipSuccess;
+    if (exact) {
+        g_valueShape = DenseShape();   // the partials may overwrite the dense operands' padding
+        hipLaunchKernelGGL(alpha_value_exact, dim3((k + kVT - 1) / kVT, colTiles),
+                           dim3(kVThreads), 0, st, k, n, r, d_B, d_Gamma, partV, partJ);
+    } else {
+        void *A = c + 2 * bPart, *Gi = c + 2 * bPart + bA;
+        float *ones = reinterpret_cast<float *>(c + 2 * bPart + bA + bG);
+        // the zero padding survives between calls of one shape; another shape clears it
+        const bool same = g_valueShape.mem == mem && g_valueShape.k == k &&
+                          g_valueShape.n == n && g_valueShape.r == r &&
+                          g_valueShape.precision == precision;
+        if (!same) {
+            e = hipMemsetAsync(A, 0, bA + bG, st);
+            hipLaunchKernelGGL(fill_ones, dim3((kpad + 255) / 256), dim3(256), 0, st, kpad, ones);
+            g_valueShape.mem = mem;
+            g_valueShape.k = k;
+            g_valueShape.n = n;
+            g_valueShape.r = r;
+            g_valueShape.precision = precision;
+        }
+        const size_t kn = (size_t)k * n, rn = (size_t)r * n;
+        const unsigned ga = (unsigned)std::min<size_t>((kn + kGConvThreads - 1) / kGConvThreads,
+                                                       65536);
+        const unsigned gg = (unsigned)std::min<size_t>((rn + kGConvThreads - 1) / kGConvThreads,
+                                                       65536);
+        const dim3 sg(rowTiles * colTiles, 1);
+        if (precision == 2) {
+            hipFuncSetAttribute((const void *)dense_score<__bf16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
+            hipLaunchKernelGGL(dense_convert<__bf16>, dim3(ga), dim3(kGConvThreads), 0, st, n,
+                               kpad, kn, d_B, static_cast<__bf16 *>(A));
+            hipLaunchKernelGGL(dense_convert<__bf16>, dim3(gg), dim3(kGConvThreads), 0, st, n,
+                               kpad, rn, d_Gamma, static_cast<__bf16 *>(Gi));
+            hipLaunchKernelGGL(dense_score<__bf16>, sg, dim3(kGThreads), kGLdsBytes, st, k, r,
+                               kpad, 1u, 1u, rowTiles, colTiles, static_cast<const __bf16 *>(A),
+                               static_cast<const __bf16 *>(Gi), ones, partV, partJ);
+        } else {
+            hipFuncSetAttribute((const void *)dense_score<float>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
+            hipLaunchKernelGGL(dense_convert<float>, dim3(ga), dim3(kGConvThreads), 0, st, n,
+                               kpad, kn, d_B, static_cast<float *>(A));
+            hipLaunchKernelGGL(dense_convert<float>, dim3(gg), dim3(kGConvThreads), 0, st, n,
+                               kpad, rn, d_Gamma, static_cast<float *>(Gi));
+            hipLaunchKernelGGL(dense_score<float>, sg, dim3(kGThreads), kGLdsBytes, st, k, r,
+                               kpad, 1u, 1u, rowTiles, colTiles, static_cast<const float *>(A),
+                               static_cast<const float *>(Gi), ones, partV, partJ);
+        }
+    }
+    hipLaunchKernelGGL(alpha_value_merge, dim3((k + 255) / 256), dim3(256), 0, st, k, colTiles,
+                       partV, partJ, d_pi, d_V, d_action, d_alpha);
+    if (e == hipSuccess)
+        e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Kernel execution failed.", NOVA_ERROR_KERNEL_EXECUTION);
+    return NOVA_SUCCESS;
+}
+
+extern "C" int nova_amd_belief_update_batch_gpu(const nova::POMDP *pomdp, unsigned int k,
+        const float *d_B, const unsigned int *d_action, const unsigned int *d_observation,
+        float *d_Bp, int *d_status)
+{
+    const char *fn = "nova_amd_belief_update_batch_gpu";
+    if (pomdp == nullptr || pomdp->n == 0 || pomdp->ns == 0 || pomdp->m == 0 ||
+            pomdp->z == 0 || pomdp->S == nullptr || pomdp->T == nullptr ||
+            pomdp->d_S == nullptr || pomdp->d_O == nullptr || k == 0 || d_B == nullptr ||
+            d_action == nullptr || d_observation == nullptr || d_Bp == nullptr ||
+            d_status == nullptr) {
+        NOVA_ERR(fn, "Invalid arguments (a device-resident POMDP with its host arrays, "
+                     "k > 0, device arrays).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const long long bad = first_bad_successor(pomdp->S, (size_t)pomdp->n * pomdp->m, pomdp->ns,
+                                              pomdp->n);
+    if (bad >= 0) {
+        NOVA_ERR(fn, "A successor index lies outside [0, n).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    std::lock_guard<std::mutex> lock(g_updateMutex);
+    int rc = pred_lists(pomdp, fn);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    hipStream_t st = stream();
+    const int useLds = pomdp->n <= (unsigned)kUpdLdsFloats ? 1 : 0;
+    const size_t lds = useLds ? sizeof(float) * pomdp->n : 0;
+    if (useLds)
+        hipFuncSetAttribute((const void *)belief_update_batch,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kUpdLdsFloats * (int)sizeof(float));
+    hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, pomdp->n, pomdp->m,
+                       pomdp->z, k, g_pred.pstart, g_pred.pS, g_pred.pT, pomdp->d_O, d_B,
+                       d_action, d_observation, d_Bp, d_status, useLds);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Kernel execution failed.", NOVA_ERROR_KERNEL_EXECUTION);
+    return NOVA_SUCCESS;
+}

@@ -160,6 +160,31 @@ class POMDP(_abi.NovaPOMDP):
         if rc != 0:
             raise Exception("Belief expansion '%s' failed with code %d." % (method, rc))
 
+    def belief_update_batch(self, B, actions, observations):
+        """tau(b, a, o) for every row of B (k x n) with its action and observation, on the GPU
+        through nova_amd_belief_update_batch_gpu (bit for bit pomdp_belief_update_cpu). The
+        POMDP must be resident (initialize_gpu). Returns torch tensors on the GPU: (B', status)
+        with status 0, 10 (impossible observation: the row is left unnormalised) or 1."""
+        import torch
+        dev = torch.device("cuda")
+        Bt = torch.as_tensor(B, dtype=torch.float32).to(dev).contiguous()
+        k = Bt.shape[0]
+        if Bt.dim() != 2 or Bt.shape[1] != self.n:
+            raise ValueError("B must be k x n")
+        a = torch.as_tensor(np.asarray(actions, np.int32)).to(dev).contiguous()
+        o = torch.as_tensor(np.asarray(observations, np.int32)).to(dev).contiguous()
+        if a.numel() != k or o.numel() != k:
+            raise ValueError("one action and one observation per belief")
+        Bp = torch.empty_like(Bt)
+        status = torch.empty(k, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        rc = self._lib.nova_amd_belief_update_batch_gpu(self, k, Bt.data_ptr(), a.data_ptr(),
+                                                       o.data_ptr(), Bp.data_ptr(),
+                                                       status.data_ptr())
+        if rc != 0:
+            raise Exception("nova_amd_belief_update_batch_gpu failed with code %d" % rc)
+        return Bp, status
+
     def belief_update(self, b, a, o):
         """tau(b, a, o) through pomdp_belief_update_cpu; returns (code, b')."""
         b = np.ascontiguousarray(b, np.float32)

@@ -288,6 +288,81 @@ def bench_batch(args):
     return out
 
 
+def bench_policy(args):
+    """Executing a policy over many beliefs: k = 16,384 dense beliefs against r = 4,096 alphas
+    at n = 20,000 (value and action of every belief: a k x r x n GEMM with a fused row argmax),
+    in each precision; and k = 16,384 belief updates on the n = 20,000, m = 16, z = 32, ns = 8
+    model. Each timed call is the synchronous entry point from device-resident inputs."""
+    import torch
+    from nova_amd import _abi, synth
+    from nova_amd.pomdp import POMDP
+    lib = _abi.load()
+    k, n, r = args.policy_beliefs, 20000, 4096
+    rng = np.random.default_rng(21)
+    dev = torch.device("cuda")
+    G = torch.from_numpy(rng.uniform(-3, 3, (r, n)).astype(np.float32)).to(dev)
+    pi = torch.from_numpy(rng.integers(0, 16, r).astype(np.int32)).to(dev)
+    B = torch.from_numpy(rng.exponential(size=(k, n)).astype(np.float32)).to(dev)
+    B /= B.sum(dim=1, keepdim=True)
+    V = torch.empty(k, dtype=torch.float32, device=dev)
+    act = torch.empty(k, dtype=torch.int32, device=dev)
+    J = {}
+    flops = 2.0 * k * r * n
+    out = {"workload": "policy_k%d_r%d_n%d" % (k, r, n), "algorithmic_flops_per_call": flops}
+    peaks = {"fp32": 157.3, "bf16": 2500.0}
+    for prec, code in (("exact", 0), ("fp32", 1), ("bf16", 2)):
+        kk = k if code else args.policy_exact_beliefs        # exact: a leading sample of rows
+        j = torch.empty(kk, dtype=torch.int32, device=dev)
+        call = lambda: lib.nova_amd_alpha_vectors_value_and_action_gpu(
+            n, r, G.data_ptr(), pi.data_ptr(), kk, B.data_ptr(), code, V.data_ptr(),
+            act.data_ptr(), j.data_ptr())
+        assert call() == 0
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert call() == 0
+        dt = (time.perf_counter() - t0) / reps
+        J[prec] = j.cpu().numpy()
+        tf = 2.0 * kk * r * n / dt / 1e12
+        leg = {"beliefs": kk, "ms_per_call": dt * 1e3, "beliefs_per_s": kk / dt,
+               "tflops": tf}
+        if prec in peaks:
+            leg["roofline"] = {"bound": "mfma", "kernel": "dense_score<%s>" % prec,
+                               "achieved": tf, "peak": peaks[prec], "unit": "TFLOP/s",
+                               "frac": tf / peaks[prec],
+                               "note": "whole call: operand conversion, GEMM and merge"}
+        out[prec] = leg
+        print("policy %s leg done" % prec, file=sys.stderr, flush=True)
+    ke = args.policy_exact_beliefs
+    out["alpha_disagreement_fp32_vs_exact"] = float(np.mean(J["fp32"][:ke] != J["exact"]))
+    out["alpha_disagreement_bf16_vs_exact"] = float(np.mean(J["bf16"][:ke] != J["exact"]))
+    del G, B
+    # batched belief updates on the PBVI model
+    P = synth.workload("pbvi_n20000")
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    kb = args.policy_beliefs
+    Bb = torch.from_numpy(rng.exponential(size=(kb, P["S"].shape[0])).astype(np.float32)).to(dev)
+    Bb /= Bb.sum(dim=1, keepdim=True)
+    a = torch.from_numpy(rng.integers(0, P["S"].shape[1], kb).astype(np.int32)).to(dev)
+    o = torch.from_numpy(rng.integers(0, P["O"].shape[2], kb).astype(np.int32)).to(dev)
+    Bp = torch.empty_like(Bb)
+    st = torch.empty(kb, dtype=torch.int32, device=dev)
+    call = lambda: lib.nova_amd_belief_update_batch_gpu(p, kb, Bb.data_ptr(), a.data_ptr(),
+                                                        o.data_ptr(), Bp.data_ptr(), st.data_ptr())
+    assert call() == 0
+    t0 = time.perf_counter()
+    for _ in range(3):
+        assert call() == 0
+    dt = (time.perf_counter() - t0) / 3
+    out["belief_update"] = {"updates": kb, "ms_per_call": dt * 1e3, "updates_per_s": kb / dt,
+                            "impossible_observations": int((st != 0).sum().item())}
+    del p
+    print("policy belief-update leg done", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +381,9 @@ def main():
     ap.add_argument("--no-dense", action="store_true", help="skip the dense-belief MFMA leg")
     ap.add_argument("--cpu-beliefs", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--no-policy", action="store_true", help="skip the policy-execution leg")
+    ap.add_argument("--policy-beliefs", type=int, default=16384)
+    ap.add_argument("--policy-exact-beliefs", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path is single-GPU; see the "
@@ -330,6 +408,8 @@ def main():
     if not args.no_batch:
         extras["batch"] = bench_batch(args)
         print("batch leg done", file=sys.stderr, flush=True)
+    if not args.no_policy:
+        extras["policy"] = bench_policy(args)
     if extras:
         out["extras"] = extras
     print(json.dumps(out))

@@ -18,7 +18,8 @@
 //    sum_s T[s,a,s'] b(s), normalised by its sum over s'. The sum over predecessors runs over a
 //    list ordered by (s, successor slot) -- the order in which the reference's scatter adds to
 //    b'(s') -- and the normaliser is summed serially over s' ascending, so b' equals nova's
-//    bit for bit. A triple whose normaliser is below 1e-9 in magnitude keeps the unnormalised
+//    bit for bit. The belief is gathered from LDS and O is read from an observation-major copy
+//    built once per model. A triple whose normaliser is below 1e-9 in magnitude keeps the unnormalised
 //    b' and reports NOVA_WARNING_INVALID_BELIEF, as the reference returns it.
 #include <algorithm>
 #include <climits>
@@ -143,15 +144,17 @@ __global__ __launch_bounds__(256) void fill_ones(unsigned count, float *__restri
         x[i] = 1.0f;
 }
 
-// One workgroup per triple q: u(s') over the ordered predecessor lists of action a, then
-// b'(s') = u(s') * O[a,s',o] staged in LDS (or written out when n is too large for LDS), the
-// normaliser summed serially by lane 0, and b' divided by it.
-constexpr int kUB = 256;
+// One workgroup per triple q. b is staged in LDS; u(s') is summed over the ordered
+// predecessor lists of action a from there, b'(s') = u(s') * O[a,s',o] is read from the
+// observation-major copy Ot[a][o][s'] (coalesced) and written out; b' is then copied back into
+// the LDS buffer, lane 0 sums it serially for the normaliser, and b' is divided by it. When n
+// exceeds the LDS buffer the same steps read b and b' from global memory.
+constexpr int kUB = 1024;
 constexpr int kUpdLdsFloats = 32 * 1024;
 
 __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned m, unsigned z,
         unsigned k, const unsigned *__restrict__ pstart, const int *__restrict__ pS,
-        const float *__restrict__ pT, const float *__restrict__ O, const float *__restrict__ B,
+        const float *__restrict__ pT, const float *__restrict__ Ot, const float *__restrict__ B,
         const unsigned *__restrict__ act, const unsigned *__restrict__ obs,
         float *__restrict__ Bp, int *__restrict__ status, int useLds)
 {
@@ -166,26 +169,33 @@ __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned 
             status[q] = NOVA_ERROR_INVALID_DATA;
         return;
     }
+    if (useLds) {
+        for (unsigned s = t; s < n; s += kUB)
+            row[s] = b[s];
+        __syncthreads();
+    }
+    const float *src = useLds ? row : b;
     const unsigned *ps = pstart + (size_t)a * (n + 1);
-    const float *Oa = O + (size_t)a * n * z;
+    const float *Oao = Ot + ((size_t)a * z + o) * n;
     for (unsigned sp = t; sp < n; sp += kUB) {
         float u = 0.0f;
         for (unsigned x = ps[sp]; x < ps[sp + 1]; x++)
-            u = __fadd_rn(u, __fmul_rn(pT[x], b[pS[x]]));
-        const float v = __fmul_rn(u, Oa[(size_t)sp * z + o]);
-        if (useLds)
-            row[sp] = v;
-        out[sp] = v;
+            u = __fadd_rn(u, __fmul_rn(pT[x], src[pS[x]]));
+        out[sp] = __fmul_rn(u, Oao[sp]);
     }
     __syncthreads();
+    if (useLds) {
+        // each lane re-reads the entries it wrote itself
+        for (unsigned sp = t; sp < n; sp += kUB)
+            row[sp] = out[sp];
+        __syncthreads();
+    }
     if (t == 0) {
+        const float *v = useLds ? row : out;
         float c = 0.0f;
-        if (useLds)
-            for (unsigned sp = 0; sp < n; sp++)
-                c = __fadd_rn(c, row[sp]);
-        else
-            for (unsigned sp = 0; sp < n; sp++)
-                c = __fadd_rn(c, out[sp]);
+#pragma unroll 8
+        for (unsigned sp = 0; sp < n; sp++)
+            c = __fadd_rn(c, v[sp]);
         norm = c;
     }
     __syncthreads();
@@ -199,6 +209,18 @@ __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned 
         out[sp] = __fdiv_rn(useLds ? row[sp] : out[sp], c);
     if (t == 0)
         status[q] = NOVA_SUCCESS;
+}
+
+// Ot[a][o][s'] = O[a][s'][o]
+__global__ __launch_bounds__(256) void observation_major(unsigned n, unsigned z, size_t count,
+        const float *__restrict__ O, float *__restrict__ Ot)
+{
+    for (size_t x = blockIdx.x * (size_t)256 + threadIdx.x; x < count;
+            x += (size_t)gridDim.x * 256) {
+        const size_t sp = x % n, ao = x / n;
+        const size_t a = ao / z, o = ao % z;
+        Ot[x] = O[(a * n + sp) * z + o];
+    }
 }
 
 // A device buffer kept between calls and grown on demand (one per use, all devices share
@@ -241,13 +263,14 @@ struct DenseShape {
 std::mutex g_updateMutex;
 // predecessor lists of the last model seen, keyed by its device arrays
 struct PredLists {
-    const void *dS = nullptr, *dT = nullptr;
-    unsigned n = 0, m = 0, ns = 0;
+    const void *dS = nullptr, *dT = nullptr, *dO = nullptr;
+    unsigned n = 0, m = 0, ns = 0, z = 0;
     void *mem = nullptr;
     size_t bytes = 0;
     unsigned *pstart = nullptr;
     int *pS = nullptr;
     float *pT = nullptr;
+    float *Ot = nullptr;      // O observation-major: Ot[a][o][s'] = O[a][s'][o]
 } g_pred;
 
 inline size_t up256(size_t x) { return (x + 255) / 256 * 256; }
@@ -255,12 +278,14 @@ inline size_t up256(size_t x) { return (x + 255) / 256 * 256; }
 } // namespace
 
 // Predecessor lists per action of p's model, (s, T[s,a,l]) for every valid successor entry,
-// grouped by successor s' and ordered by (s, l) within each group; built on the host from p's
-// host arrays and kept on the device until another model is passed.
+// grouped by successor s' and ordered by (s, l) within each group, built on the host from p's
+// host arrays; and the observation-major copy of O, built on the device. Kept until another
+// model (other device arrays or sizes) is passed.
 static int pred_lists(const nova::POMDP *p, const char *fn)
 {
-    if (g_pred.mem != nullptr && g_pred.dS == p->d_S && g_pred.dT == p->d_T && g_pred.n == p->n &&
-            g_pred.m == p->m && g_pred.ns == p->ns)
+    if (g_pred.mem != nullptr && g_pred.dS == p->d_S && g_pred.dT == p->d_T &&
+            g_pred.dO == p->d_O && g_pred.n == p->n && g_pred.m == p->m && g_pred.ns == p->ns &&
+            g_pred.z == p->z)
         return NOVA_SUCCESS;
     const unsigned n = p->n, m = p->m, ns = p->ns;
     std::vector<unsigned> pstart((size_t)m * (n + 1), 0);
@@ -300,7 +325,9 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
         g_pred = PredLists();
     }
     const size_t b0 = up256(sizeof(unsigned) * pstart.size()), b1 = up256(sizeof(int) * pS.size());
-    const size_t bytes = b0 + b1 + up256(sizeof(float) * pT.size());
+    const size_t b2 = up256(sizeof(float) * pT.size());
+    const size_t otCount = (size_t)m * p->z * n;
+    const size_t bytes = b0 + b1 + b2 + up256(sizeof(float) * otCount);
     void *mem = nullptr;
     int rc = device_alloc(&mem, bytes, fn, "predecessor lists");
     if (rc != NOVA_SUCCESS)
@@ -317,16 +344,32 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
         return hip_fail(e, fn, "Failed to copy the predecessor lists.",
                         NOVA_ERROR_MEMCPY_TO_DEVICE);
     }
+    float *Ot = reinterpret_cast<float *>(c + b0 + b1 + b2);
+    hipStream_t st = stream();
+    hipLaunchKernelGGL(observation_major,
+                       dim3((unsigned)std::min<size_t>((otCount + 255) / 256, 65536)), dim3(256),
+                       0, st, n, p->z, otCount, p->d_O, Ot);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        device_free(mem, fn, "predecessor lists");
+        return hip_fail(e, fn, "Failed to build the observation-major copy of O.",
+                        NOVA_ERROR_KERNEL_EXECUTION);
+    }
     g_pred.dS = p->d_S;
     g_pred.dT = p->d_T;
+    g_pred.dO = p->d_O;
     g_pred.n = n;
     g_pred.m = m;
     g_pred.ns = ns;
+    g_pred.z = p->z;
     g_pred.mem = mem;
     g_pred.bytes = bytes;
     g_pred.pstart = reinterpret_cast<unsigned *>(c);
     g_pred.pS = reinterpret_cast<int *>(c + b0);
     g_pred.pT = reinterpret_cast<float *>(c + b0 + b1);
+    g_pred.Ot = Ot;
     return NOVA_SUCCESS;
 }
 
@@ -462,7 +505,7 @@ extern "C" int nova_amd_belief_update_batch_gpu(const nova::POMDP *pomdp, unsign
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             kUpdLdsFloats * (int)sizeof(float));
     hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, pomdp->n, pomdp->m,
-                       pomdp->z, k, g_pred.pstart, g_pred.pS, g_pred.pT, pomdp->d_O, d_B,
+                       pomdp->z, k, g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B,
                        d_action, d_observation, d_Bp, d_status, useLds);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)

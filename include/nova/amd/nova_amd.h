@@ -154,6 +154,12 @@ extern "C" int nova_amd_belief_update_batch_gpu(const POMDP *pomdp, unsigned int
         const float *d_B, const unsigned int *d_action, const unsigned int *d_observation,
         float *d_Bp, int *d_status);
 
+// Alpha choice inside pomdp_expand_pema_cpu (no counterpart in the reference): 0 (default)
+// reproduces nova's loop (pomdp_expand_cpu.cpp:391-402), which never accumulates the dot
+// product and so keeps alpha 0 unless a single product lies below -1e10; 1 takes the first
+// maximiser of alpha . b over the policy's alphas. Applies to later calls in the process.
+extern "C" int nova_amd_pema_mode(int mode);
+
 // delete[] of a float array the library allocated for the caller (Bnew of the expansions,
 // bp of pomdp_belief_update_cpu), for callers that cannot call delete[] themselves.
 extern "C" int nova_amd_free_floats(float *p);

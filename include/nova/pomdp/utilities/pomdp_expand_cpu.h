@@ -13,8 +13,9 @@ namespace nova {
 extern "C" int pomdp_expand_random_cpu(POMDP *pomdp, unsigned int numBeliefsToAdd);
 // For each belief, the successor tau(b,a,o) farthest (L1) from the set (pomdp_expand_cpu.cpp:166-266).
 extern "C" int pomdp_expand_distinct_beliefs_cpu(POMDP *pomdp);
-// PEMA: not provided (its alpha argmax is defective in the reference,
-// pomdp_expand_cpu.cpp:391-402); returns NOVA_ERROR_INVALID_DATA.
+// PEMA: appends the successor with the largest expected approximation error
+// (pomdp_expand_cpu.cpp:269-466). By default it reproduces nova's alpha choice, whose argmax is
+// defective (:391-402); nova_amd_pema_mode(1) (nova/amd/nova_amd.h) corrects it.
 extern "C" int pomdp_expand_pema_cpu(POMDP *pomdp, const POMDPAlphaVectors *policy);
 
 } // namespace nova

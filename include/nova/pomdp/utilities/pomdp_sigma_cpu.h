@@ -1,6 +1,6 @@
-// nova-mi355x: sigma-approximation (replaces include/nova/pomdp/utilities/pomdp_sigma_cpu.h:43).
-// Preprocessing of B, outside the backup path: exported for ctypes import and returns
-// NOVA_ERROR_INVALID_DATA with a message on stderr.
+// nova-mi355x: sigma-approximation (replaces include/nova/pomdp/utilities/pomdp_sigma_cpu.h:43):
+// keeps each belief's numDesiredNonZeroValues largest entries, renormalised, and reports the
+// smallest kept mass as sigma. Host arrays only; re-run pomdp_initialize_gpu afterwards.
 #ifndef NOVA_POMDP_SIGMA_CPU_H
 #define NOVA_POMDP_SIGMA_CPU_H
 

@@ -1,4 +1,5 @@
-// nova-mi355x: belief-set expansion on the host (random trajectories, distinct beliefs).
+// nova-mi355x: belief-set expansion on the host (random trajectories, distinct beliefs, PEMA).
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -7,6 +8,7 @@
 
 #include <nova/pomdp/utilities/pomdp_expand_cpu.h>
 #include <nova/pomdp/utilities/pomdp_model_cpu.h>
+#include <nova/amd/nova_amd.h>
 
 #include "nova_common.h"
 
@@ -149,14 +151,157 @@ int pomdp_expand_distinct_beliefs_cpu(POMDP *pomdp)
     return pomdp_add_new_raw_beliefs_cpu(pomdp, r, Bnew.data());
 }
 
-// PEMA is outside this library's scope: the reference's alpha argmax inside it is defective
-// (pomdp_expand_cpu.cpp:391-402 assigns instead of accumulating and compares the wrong way).
+// PEMA: adds the one successor belief whose expected approximation error is largest
+// (src/pomdp/utilities/pomdp_expand_cpu.cpp:269-466). For every belief b of the set, action a
+// and observation o with a valid successor b' = tau(b, a, o): Pr(o | b, a), the set belief c
+// closest to b' in L1 (first minimum), an alpha chosen for c, and
+//   eps(b') = sum_s ((b'(s) >= b(s) ? Rmax : Rmin) / (1 - gamma) - alpha(s)) * (b'(s) - c(s));
+// per action the observation with the largest Pr * eps and the sum of Pr * eps over o; per
+// belief the action with the largest sum; over the set the first belief with the largest
+// value, whose successor under its action and observation is appended (b itself when that
+// update is invalid). Every sum runs in the reference's order with float rounding.
+//
+// The alpha choice: nova's loop (:391-402) assigns instead of accumulating and compares the
+// wrong way, so it picks alpha 0 unless some single product alpha(s) * c(s) lies below -1e10
+// (then the last such alpha). That is the default here, for results equal to nova's;
+// nova_amd_pema_mode(1) selects the first maximiser of alpha . c instead.
+static std::atomic<int> g_pemaMode{0};
+
+int nova_amd_pema_mode(int mode)
+{
+    if (mode != 0 && mode != 1) {
+        NOVA_ERR("nova_amd_pema_mode", "Mode must be 0 (nova's alpha choice) or 1 (argmax).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    g_pemaMode.store(mode);
+    return NOVA_SUCCESS;
+}
+
+// The alpha used for the closest belief c (see above).
+static unsigned pema_alpha(const POMDPAlphaVectors *policy, const float *c, int mode)
+{
+    const unsigned n = policy->n;
+    unsigned best = 0;
+    if (mode == 0) {
+        for (unsigned j = 0; j < policy->r; j++)
+            for (unsigned s = 0; s < n; s++)
+                if (policy->Gamma[(size_t)j * n + s] * c[s] < nova_amd::kLow)
+                    best = j;
+        return best;
+    }
+    float bestV = nova_amd::kLow;
+    for (unsigned j = 0; j < policy->r; j++) {
+        float v = 0.0f;
+        for (unsigned s = 0; s < n; s++)
+            v += policy->Gamma[(size_t)j * n + s] * c[s];
+        if (v > bestV) {
+            bestV = v;
+            best = j;
+        }
+    }
+    return best;
+}
+
 int pomdp_expand_pema_cpu(POMDP *pomdp, const POMDPAlphaVectors *policy)
 {
-    (void)pomdp;
-    (void)policy;
-    NOVA_ERR("pomdp_expand_pema_cpu", "PEMA expansion is not provided by this library.");
-    return NOVA_ERROR_INVALID_DATA;
+    const char *fn = "pomdp_expand_pema_cpu";
+    if (!nova_amd::expand_args_ok(pomdp) || policy == nullptr || policy->r == 0 ||
+            policy->n != pomdp->n || policy->Gamma == nullptr) {
+        NOVA_ERR(fn, "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const unsigned n = pomdp->n, m = pomdp->m, z = pomdp->z, r = pomdp->r, rz = pomdp->rz;
+    const unsigned ns = pomdp->ns;
+    const int mode = g_pemaMode.load();
+    float Rmax = nova_amd::kLow, Rmin = nova_amd::kHigh;
+    for (size_t x = 0; x < (size_t)n * m; x++) {
+        if (Rmax < pomdp->R[x])
+            Rmax = pomdp->R[x];
+        if (Rmin > pomdp->R[x])
+            Rmin = pomdp->R[x];
+    }
+    // dense copies of the whole set, built once (the reference rebuilds them per comparison)
+    std::vector<float> dense((size_t)r * n);
+    for (unsigned j = 0; j < r; j++)
+        nova_amd::dense_belief(pomdp, j, dense.data() + (size_t)j * n);
+    std::vector<float> Bnew(n, 0.0f);
+    std::vector<unsigned> oStar(m);
+    std::vector<float> oStarValue(m);
+    float bound = nova_amd::kLow;
+    for (unsigned i = 0; i < r; i++) {
+        const float *b = dense.data() + (size_t)i * n;
+        unsigned aStar = 0;
+        float aStarValue = nova_amd::kLow;
+        for (unsigned a = 0; a < m; a++) {
+            oStar[a] = 0;
+            oStarValue[a] = nova_amd::kLow;
+        }
+        for (unsigned a = 0; a < m; a++) {
+            float aValue = 0.0f;
+            for (unsigned o = 0; o < z; o++) {
+                float pr = 0.0f;
+                for (unsigned k = 0; k < rz; k++) {
+                    const int s = pomdp->Z[(size_t)i * rz + k];
+                    if (s < 0)
+                        break;
+                    const size_t row = nova_amd::idx3((unsigned)s, a, 0, m, ns);
+                    for (unsigned l = 0; l < ns; l++) {
+                        const int sp = pomdp->S[row + l];
+                        if (sp < 0)
+                            break;
+                        pr += pomdp->T[row + l] * pomdp->O[((size_t)a * n + sp) * z + o] *
+                              pomdp->B[(size_t)i * rz + k];
+                    }
+                }
+                float *bp = nullptr;
+                if (pomdp_belief_update_cpu(pomdp, b, a, o, bp) != NOVA_SUCCESS) {
+                    delete[] bp;
+                    continue;
+                }
+                // with no set belief nearer than 1e10 the reference's copy stays
+                // uninitialised; index 0 stands in for it
+                unsigned closest = 0;
+                float closestDist = nova_amd::kHigh;
+                for (unsigned j = 0; j < r; j++) {
+                    const float *c = dense.data() + (size_t)j * n;
+                    float d = 0.0f;
+                    for (unsigned s = 0; s < n; s++)
+                        d += std::fabs(bp[s] - c[s]);
+                    if (d < closestDist) {
+                        closest = j;
+                        closestDist = d;
+                    }
+                }
+                const float *c = dense.data() + (size_t)closest * n;
+                const float *alpha = policy->Gamma + (size_t)pema_alpha(policy, c, mode) * n;
+                float eps = 0.0f;
+                for (unsigned s = 0; s < n; s++) {
+                    const float lim = (bp[s] >= b[s] ? Rmax : Rmin) / (1.0f - pomdp->gamma);
+                    eps += (lim - alpha[s]) * (bp[s] - c[s]);
+                }
+                if (pr * eps > oStarValue[a]) {
+                    oStar[a] = o;
+                    oStarValue[a] = pr * eps;
+                }
+                aValue += pr * eps;
+                delete[] bp;
+            }
+            if (aValue > aStarValue) {
+                aStar = a;
+                aStarValue = aValue;
+            }
+        }
+        if (aStarValue > bound) {
+            float *bp = nullptr;
+            if (pomdp_belief_update_cpu(pomdp, b, aStar, oStar[aStar], bp) == NOVA_SUCCESS)
+                std::memcpy(Bnew.data(), bp, sizeof(float) * n);
+            else
+                std::memcpy(Bnew.data(), b, sizeof(float) * n);
+            delete[] bp;
+            bound = aStarValue;
+        }
+    }
+    return pomdp_add_new_raw_beliefs_cpu(pomdp, 1, Bnew.data());
 }
 
 } // namespace nova

@@ -124,7 +124,7 @@ class POMDP(_abi.NovaPOMDP):
                 view(self.B, self.r * self.rz, np.float32).reshape(self.r, self.rz))
 
     def expand(self, method="random", numBeliefsToAdd=1000, pemaPolicy=None, gpu=False,
-               seed=None):
+               seed=None, pemaAlgorithm=None):
         """Expands the belief set (python/nova/pomdp.py:202-229). With gpu=True the random
         trajectories (pomdp_expand_random_gpu, or its seeded variant when seed is given) or the
         distinct-belief scoring run on the device; the POMDP must then be resident. Call
@@ -154,11 +154,23 @@ class POMDP(_abi.NovaPOMDP):
         elif method == "distinct_beliefs":
             rc = self._lib.pomdp_expand_distinct_beliefs_cpu(self)
         elif method == "pema":
+            if pemaPolicy is None:
+                pemaPolicy = pemaAlgorithm.solve()      # python/nova/pomdp.py:226-229
             rc = self._lib.pomdp_expand_pema_cpu(self, ct.byref(pemaPolicy))
         else:
             raise Exception("Failed to expand. Method '%s' is not defined." % method)
         if rc != 0:
             raise Exception("Belief expansion '%s' failed with code %d." % (method, rc))
+
+    def sigma_approximate(self, numDesiredNonZeroValues):
+        """Keeps each belief's numDesiredNonZeroValues largest entries, renormalised
+        (pomdp_sigma_cpu); returns sigma, the smallest kept mass. Host arrays only: call
+        initialize_gpu() again before GPU work."""
+        sig = ct.c_float()
+        rc = self._lib.pomdp_sigma_cpu(self, int(numDesiredNonZeroValues), ct.byref(sig))
+        if rc != 0:
+            raise Exception("pomdp_sigma_cpu failed with code %d" % rc)
+        return sig.value
 
     def belief_update_batch(self, B, actions, observations):
         """tau(b, a, o) for every row of B (k x n) with its action and observation, on the GPU

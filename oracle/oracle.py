@@ -424,6 +424,59 @@ class RefLib:
         out = np.ctypeslib.as_array(bp, (p.n,)).copy() if bp else None
         return rc, out
 
+    def _owned_pomdp(self, P):
+        """P copied into arrays nova's library allocated itself (its expansions free and
+        replace Z and B)."""
+        a = self._abi
+        p = a.NovaPOMDP()
+        n, m, ns = P["S"].shape
+        z = P["O"].shape[2]
+        r, rz = P["Z"].shape
+        rc = self.lib.pomdp_initialize_cpu(ct.byref(p), n, ns, m, z, r, rz,
+                                           ct.c_float(P["gamma"]), max(1, int(P["horizon"])))
+        assert rc == 0
+        for name, arr, dt in (("S", P["S"], np.int32), ("T", P["T"], np.float32),
+                              ("O", P["O"], np.float32), ("R", P["R"], np.float32),
+                              ("Z", P["Z"], np.int32), ("B", P["B"], np.float32)):
+            src = _c(arr, dt)
+            ct.memmove(getattr(p, name), src.ctypes.data, src.nbytes)
+        return p
+
+    def _beliefs_and_free(self, p):
+        r2, rz2 = p.r, p.rz
+        Z2 = np.ctypeslib.as_array(p.Z, (r2 * rz2,)).reshape(r2, rz2).copy()
+        B2 = np.ctypeslib.as_array(p.B, (r2 * rz2,)).reshape(r2, rz2).copy()
+        self.lib.pomdp_uninitialize_cpu(ct.byref(p))
+        return Z2, B2
+
+    def expand_pema(self, P, Gamma, pi, steps=1):
+        """`steps` calls of nova's pomdp_expand_pema_cpu with the alpha set (Gamma, pi); returns
+        the final (Z, B)."""
+        a = self._abi
+        p = self._owned_pomdp(P)
+        G = _c(Gamma, np.float32)
+        pis = _c(pi, np.uint32)
+        pol = a.NovaPOMDPAlphaVectors()
+        pol.n, pol.m, pol.r = G.shape[1], P["S"].shape[1], G.shape[0]
+        pol.Gamma = ct.cast(G.ctypes.data, a.c_float_p)
+        pol.pi = ct.cast(pis.ctypes.data, a.c_uint_p)
+        for _ in range(steps):
+            rc = self.lib.pomdp_expand_pema_cpu(ct.byref(p), ct.byref(pol))
+            assert rc == 0, rc
+        return self._beliefs_and_free(p)
+
+    def sigma(self, P, k):
+        """nova's pomdp_sigma_cpu(P, k); returns (Z, B, sigma). nova's initialisation loop writes
+        k*n entries into arrays of r*k: callers keep r >= n (and k <= rz, which it also reads
+        past)."""
+        p = self._owned_pomdp(P)
+        assert P["Z"].shape[0] >= P["S"].shape[0] and k <= P["Z"].shape[1]
+        sig = ct.c_float()
+        rc = self.lib.pomdp_sigma_cpu(ct.byref(p), k, ct.byref(sig))
+        assert rc == 0, rc
+        Z2, B2 = self._beliefs_and_free(p)
+        return Z2, B2, sig.value
+
     def expand_distinct(self, P):
         """Runs nova's pomdp_expand_distinct_beliefs_cpu on a library-owned copy of P; returns
         the new (Z, B) arrays."""

@@ -22,6 +22,12 @@ extern "C" int mdp_vi_run_gpu(const MDP *mdp, MDPVIGPU *vi, const float *d_V0,
 
 // ---- differentiable value iteration
 
+// Which sweep kernel a solver runs: 0 = one thread per state, 1 = the wave-tiled sweep,
+// 2 = the successor-sorted tiles that mdp_vi_initialize_gpu lays out for models of at least
+// 2^24 (state, action, successor) slots with 8 <= ns <= 255 (NOVA_AMD_VI_KERNEL=tiled|sorted
+// overrides the size rule). All three give the same V and pi bit for bit.
+extern "C" int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *kernel);
+
 // Solves V* = max_a (R + gamma T.V*) on the device to an FP64 residual max|V' - V| <= tol.
 // Starts from vi->d_V (float; vi initialized or its d_* fields set by the caller) with float
 // sweeps while their residual still halves every 8 sweeps, then continues in FP64 until the

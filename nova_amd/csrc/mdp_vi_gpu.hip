@@ -16,8 +16,11 @@
 //     reference's tie rule, and V'/pi are written coalesced.
 // A plain one-thread-per-state kernel covers shapes the tiled kernel does not (ns > 256,
 // m > 2048, misaligned arrays).
+#include <cstdlib>
 #include <cstring>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include <nova/amd/nova_amd.h>
 #include <nova/mdp/algorithms/mdp_vi_gpu.h>
@@ -198,6 +201,146 @@ template <int E> TiledFn tiled_for(unsigned G)
     }
 }
 
+// ---- successor-sorted tiles (large models)
+//
+// The sweep above is bound by L2 requests: each of its n*m*ns gathers V[S[s,a,i]] is one
+// request, since lanes of a wave hit unrelated lines. A one-time layout removes a good part
+// of them: the entries of each tile of kSortedEntries consecutive (s, a, i) slots are stored
+// sorted by successor, with their T and their slot in the tile. A workgroup streams its tile
+// in that order -- neighbouring gathers now share cache lines within a wave instruction and
+// across consecutive ones -- and scatters each product (gamma*T)*V into an LDS array at its
+// slot; every row is then summed from LDS in its own order, so the sums and V' are the same
+// bit for bit as above. Entries past a row's first negative successor are left out.
+constexpr unsigned kSortedEntries = 16384;   // slots per tile: two workgroups per CU fit LDS
+constexpr int kSortedThreads = 1024;
+constexpr int kSortedBatch = 8;             // entries per lane per round of gathers
+
+__global__ __launch_bounds__(256) void vi_row_counts(size_t rows, unsigned ns,
+        const int *__restrict__ S, unsigned char *__restrict__ cnt)
+{
+    const size_t row = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (row >= rows)
+        return;
+    unsigned c = 0;
+    while (c < ns && S[row * ns + c] >= 0)
+        c++;
+    cnt[row] = (unsigned char)(c > 255 ? 255 : c);   // ns <= 255 for this layout
+}
+
+// key = successor (clamped below n) for a valid slot, n past a row's end or past the model;
+// value = the slot within its tile, numbered entry-major (i * rowsPerTile + local row) so that
+// the row sums read their products from LDS with consecutive lanes on consecutive words
+__global__ __launch_bounds__(256) void vi_sort_keys(size_t total, unsigned n, unsigned m,
+        unsigned ns, unsigned statesPerTile, unsigned tileEntries, const int *__restrict__ S,
+        const unsigned char *__restrict__ cnt, int *__restrict__ keys,
+        unsigned *__restrict__ vals)
+{
+    const size_t x = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (x >= total)
+        return;
+    const size_t tile = x / tileEntries;
+    const unsigned e = (unsigned)(x % tileEntries);
+    const unsigned rowLen = m * ns;
+    const size_t s = tile * statesPerTile + e / rowLen;
+    const unsigned rem = e % rowLen, i = rem % ns;
+    int key = (int)n;
+    if (s < n && (e / rowLen) < statesPerTile && i < cnt[s * m + rem / ns])
+        key = min(S[s * rowLen + rem], (int)n - 1);
+    keys[x] = key;
+    vals[x] = i * (statesPerTile * m) + (e / rowLen) * m + rem / ns;
+}
+
+// the permuted T and 16-bit slots, and the valid count of every tile
+__global__ __launch_bounds__(256) void vi_sort_finish(size_t total, unsigned n, unsigned m,
+        unsigned ns, unsigned statesPerTile, unsigned tileEntries, const float *__restrict__ T,
+        const int *__restrict__ sortedS, const unsigned *__restrict__ vals,
+        float *__restrict__ sortedT, unsigned short *__restrict__ slot,
+        unsigned *__restrict__ tileValid)
+{
+    const size_t x = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (x >= total)
+        return;
+    const size_t tile = x / tileEntries;
+    const unsigned e = vals[x];                       // i * rowsPerTile + local row
+    const unsigned rowsPerTile = statesPerTile * m;
+    const unsigned i = e / rowsPerTile, row = e % rowsPerTile;
+    const size_t s = tile * statesPerTile + row / m;
+    const bool valid = sortedS[x] < (int)n;
+    sortedT[x] = valid ? T[(s * m + row % m) * ns + i] : 0.0f;
+    slot[x] = (unsigned short)e;
+    // the last valid entry of a tile records the count (keys n sort after every valid one)
+    const bool lastValid = valid && ((x + 1) % tileEntries == 0 || sortedS[x + 1] >= (int)n);
+    if (lastValid)
+        tileValid[tile] = (unsigned)(x % tileEntries) + 1;
+}
+
+__global__ __launch_bounds__(kSortedThreads) void vi_sweep_sorted(unsigned n, unsigned m,
+        unsigned ns, unsigned statesPerTile, unsigned tileEntries, float gamma,
+        const int *__restrict__ sortedS, const float *__restrict__ sortedT,
+        const unsigned short *__restrict__ slot, const unsigned *__restrict__ tileValid,
+        const unsigned char *__restrict__ cnt, const float *__restrict__ R,
+        const float *__restrict__ V, float *__restrict__ Vp, unsigned *__restrict__ pi,
+        int writePi)
+{
+    extern __shared__ float lds[];
+    float *P = lds;                       // tileEntries products, by slot (entry-major)
+    float *Q = lds + tileEntries;         // statesPerTile * m row sums
+    const unsigned tile = blockIdx.x, t = threadIdx.x;
+    const size_t base = (size_t)tile * tileEntries;
+    const unsigned valid = tileValid[tile];
+    // kSortedBatch entries per lane in flight: all stream loads, then all gathers, then the
+    // LDS stores (a tile takes two or three such rounds)
+    for (unsigned x0 = t; x0 < valid; x0 += kSortedBatch * kSortedThreads) {
+        int sp[kSortedBatch];
+        float tt[kSortedBatch];
+        unsigned short sl[kSortedBatch];
+#pragma unroll
+        for (int u = 0; u < kSortedBatch; u++) {
+            const unsigned x = x0 + u * kSortedThreads;
+            const bool in = x < valid;
+            sp[u] = in ? __builtin_nontemporal_load(sortedS + base + x) : 0;
+            tt[u] = in ? __builtin_nontemporal_load(sortedT + base + x) : 0.0f;
+            sl[u] = in ? slot[base + x] : 0;
+        }
+        float v[kSortedBatch];
+#pragma unroll
+        for (int u = 0; u < kSortedBatch; u++)
+            v[u] = V[sp[u]];
+#pragma unroll
+        for (int u = 0; u < kSortedBatch; u++)
+            if (x0 + u * kSortedThreads < valid)
+                P[sl[u]] = __fmul_rn(__fmul_rn(gamma, tt[u]), v[u]);
+    }
+    __syncthreads();
+    const unsigned s0 = tile * statesPerTile;
+    const unsigned states = min(statesPerTile, n - s0);
+    const unsigned rows = states * m;
+    for (unsigned row = t; row < rows; row += kSortedThreads) {
+        const size_t g = (size_t)s0 * m + row;
+        float q = R[g];
+        const unsigned c = cnt[g];
+        const unsigned rowsPerTile = statesPerTile * m;
+        for (unsigned i = 0; i < c; i++)
+            q = __fadd_rn(q, P[i * rowsPerTile + row]);
+        Q[row] = q;
+    }
+    __syncthreads();
+    for (unsigned x = t; x < states; x += kSortedThreads) {
+        const float *qs = Q + (size_t)x * m;
+        float best = qs[0];
+        unsigned bestA = 0;
+        for (unsigned a = 1; a < m; a++) {
+            if (qs[a] > best) {
+                best = qs[a];
+                bestA = a;
+            }
+        }
+        Vp[s0 + x] = best;
+        if (writePi)
+            pi[s0 + x] = bestA;
+    }
+}
+
 unsigned pow2_at_least(unsigned x)
 {
     unsigned p = 1;
@@ -208,16 +351,168 @@ unsigned pow2_at_least(unsigned x)
 
 } // namespace
 
+// The successor-sorted layout of one solver's model (built by mdp_vi_initialize_gpu for large
+// models, freed with the solver; keyed by the solver's V buffer and the model's arrays).
+struct ViSorted {
+    const float *owner = nullptr;
+    const int *dS = nullptr;
+    const float *dT = nullptr;
+    unsigned n = 0, m = 0, ns = 0, statesPerTile = 0, tileEntries = 0, tiles = 0;
+    void *mem = nullptr;
+    int *sS = nullptr;
+    float *sT = nullptr;
+    unsigned short *slot = nullptr;
+    unsigned char *cnt = nullptr;
+    unsigned *tileValid = nullptr;
+    size_t ldsBytes = 0;
+};
+static std::vector<ViSorted> g_sorted;
+
+static const ViSorted *sorted_for(const nova::MDP *mdp, const nova::MDPVIGPU *vi)
+{
+    for (const ViSorted &L : g_sorted)
+        if (L.owner == vi->d_V && L.dS == mdp->d_S && L.dT == mdp->d_T && L.n == mdp->n &&
+                L.m == mdp->m && L.ns == mdp->ns)
+            return &L;
+    return nullptr;
+}
+
+static void sorted_drop(const float *owner, const char *fn)
+{
+    for (size_t x = 0; x < g_sorted.size();) {
+        if (g_sorted[x].owner == owner) {
+            hipDeviceSynchronize();
+            device_free(g_sorted[x].mem, fn, "the sorted sweep layout");
+            g_sorted.erase(g_sorted.begin() + (long)x);
+        } else {
+            x++;
+        }
+    }
+}
+
+// Shapes the sorted layout takes: 8 <= ns <= 255 (16-bit slots, byte row counts, LDS for the
+// tile's products and row sums), m*ns <= kSortedEntries. NOVA_AMD_VI_KERNEL=tiled never builds
+// it, =sorted builds it for every such shape; by default models of at least 2^24 (s, a, i)
+// slots get it, where the sweep is bound by gather requests.
+static bool sorted_wanted(const nova::MDP *mdp)
+{
+    const size_t slots = (size_t)mdp->n * mdp->m * mdp->ns;
+    const bool shapeOk = mdp->ns >= 8 && mdp->ns <= 255 &&
+                         (size_t)mdp->m * mdp->ns <= kSortedEntries &&
+                         slots + kSortedEntries < ((size_t)1 << 31);
+    if (!shapeOk)
+        return false;
+    if (const char *f = std::getenv("NOVA_AMD_VI_KERNEL")) {
+        if (std::strcmp(f, "tiled") == 0)
+            return false;
+        if (std::strcmp(f, "sorted") == 0)
+            return true;
+    }
+    return slots >= ((size_t)1 << 24);
+}
+
+static size_t up256(size_t x) { return (x + 255) / 256 * 256; }
+
+static int sorted_build(const nova::MDP *mdp, const float *owner, hipStream_t st, const char *fn)
+{
+    const unsigned n = mdp->n, m = mdp->m, ns = mdp->ns;
+    ViSorted L;
+    L.owner = owner;
+    L.dS = mdp->d_S;
+    L.dT = mdp->d_T;
+    L.n = n;
+    L.m = m;
+    L.ns = ns;
+    L.statesPerTile = kSortedEntries / (m * ns);
+    L.tileEntries = L.statesPerTile * m * ns;
+    L.tiles = (n + L.statesPerTile - 1) / L.statesPerTile;
+    L.ldsBytes = sizeof(float) * ((size_t)L.tileEntries + (size_t)L.statesPerTile * m);
+    const size_t total = (size_t)L.tiles * L.tileEntries, rows = (size_t)n * m;
+    const size_t bS = up256(4 * total), bT = up256(4 * total), bSl = up256(2 * total);
+    const size_t bC = up256(rows), bV = up256(4 * (size_t)L.tiles);
+    int rc = device_alloc(&L.mem, bS + bT + bSl + bC + bV, fn, "the sorted sweep layout");
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    char *c = static_cast<char *>(L.mem);
+    L.sS = reinterpret_cast<int *>(c);
+    L.sT = reinterpret_cast<float *>(c + bS);
+    L.slot = reinterpret_cast<unsigned short *>(c + bS + bT);
+    L.cnt = reinterpret_cast<unsigned char *>(c + bS + bT + bSl);
+    L.tileValid = reinterpret_cast<unsigned *>(c + bS + bT + bSl + bC);
+
+    // temporaries: keys, slot values in and out, segment offsets, sort storage
+    void *tmp = nullptr, *cub = nullptr;
+    const size_t bK = up256(4 * total), bO = up256(4 * ((size_t)L.tiles + 1));
+    rc = device_alloc(&tmp, bK + 2 * bK + bO, fn, "sort buffers");
+    if (rc != NOVA_SUCCESS) {
+        device_free(L.mem, fn, "the sorted sweep layout");
+        return rc;
+    }
+    char *tc = static_cast<char *>(tmp);
+    int *keys = reinterpret_cast<int *>(tc);
+    unsigned *valsIn = reinterpret_cast<unsigned *>(tc + bK);
+    unsigned *valsOut = reinterpret_cast<unsigned *>(tc + 2 * bK);
+    int *off = reinterpret_cast<int *>(tc + 3 * bK);
+    std::vector<int> hoff((size_t)L.tiles + 1);
+    for (size_t x = 0; x <= L.tiles; x++)
+        hoff[x] = (int)(x * L.tileEntries);
+    hipError_t e = hipMemcpy(off, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice);
+    const unsigned gR = (unsigned)((rows + 255) / 256), gE = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(vi_row_counts, dim3(gR), dim3(256), 0, st, rows, ns, mdp->d_S, L.cnt);
+    hipLaunchKernelGGL(vi_sort_keys, dim3(gE), dim3(256), 0, st, total, n, m, ns,
+                       L.statesPerTile, L.tileEntries, mdp->d_S, L.cnt, keys, valsIn);
+    int endBit = 1;
+    while (endBit < 31 && ((size_t)1 << endBit) <= n)
+        endBit++;
+    size_t cubBytes = 0;
+    if (e == hipSuccess)
+        e = hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, cubBytes, keys, L.sS, valsIn,
+                valsOut, (int)total, (int)L.tiles, off, off + 1, 0, endBit, st);
+    if (e == hipSuccess)
+        rc = device_alloc(&cub, std::max<size_t>(cubBytes, 256), fn, "sort storage");
+    if (e == hipSuccess && rc == NOVA_SUCCESS) {
+        e = hipcub::DeviceSegmentedRadixSort::SortPairs(cub, cubBytes, keys, L.sS, valsIn,
+                valsOut, (int)total, (int)L.tiles, off, off + 1, 0, endBit, st);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(L.tileValid, 0, 4 * (size_t)L.tiles, st);
+        hipLaunchKernelGGL(vi_sort_finish, dim3(gE), dim3(256), 0, st, total, n, m, ns,
+                           L.statesPerTile, L.tileEntries, mdp->d_T, L.sS, valsOut, L.sT, L.slot,
+                           L.tileValid);
+        if (e == hipSuccess)
+            e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+    }
+    if (cub)
+        device_free(cub, fn, "sort storage");
+    device_free(tmp, fn, "sort buffers");
+    if (e == hipSuccess && rc == NOVA_SUCCESS)
+        e = hipFuncSetAttribute((const void *)vi_sweep_sorted,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.ldsBytes);
+    if (e != hipSuccess || rc != NOVA_SUCCESS) {
+        device_free(L.mem, fn, "the sorted sweep layout");
+        return rc != NOVA_SUCCESS ? rc : hip_fail(e, fn, "Failed to build the sorted layout.",
+                                                  NOVA_ERROR_KERNEL_EXECUTION);
+    }
+    g_sorted.push_back(L);
+    return NOVA_SUCCESS;
+}
+
 // A sweep launch plan for one MDP shape.
 struct ViPlan {
     TiledFn tiled = nullptr;
+    const ViSorted *sorted = nullptr;
     unsigned statesPerTile = 0;
     unsigned grid = 0;
 };
 
-static ViPlan vi_plan(const nova::MDP *mdp)
+static ViPlan vi_plan(const nova::MDP *mdp, const nova::MDPVIGPU *vi)
 {
     ViPlan p;
+    if ((p.sorted = sorted_for(mdp, vi)) != nullptr) {
+        p.grid = p.sorted->tiles;
+        return p;
+    }
     const unsigned n = mdp->n, m = mdp->m, ns = mdp->ns;
     const bool aligned = ((reinterpret_cast<uintptr_t>(mdp->d_S) |
                            reinterpret_cast<uintptr_t>(mdp->d_T)) & 15u) == 0;
@@ -242,6 +537,14 @@ static ViPlan vi_plan(const nova::MDP *mdp)
 static void vi_launch(const ViPlan &p, const nova::MDP *mdp, const float *V, float *Vp,
                       unsigned *pi, bool writePi, hipStream_t st)
 {
+    if (p.sorted) {
+        const ViSorted &L = *p.sorted;
+        hipLaunchKernelGGL(vi_sweep_sorted, dim3(p.grid), dim3(kSortedThreads), L.ldsBytes, st,
+                           mdp->n, mdp->m, mdp->ns, L.statesPerTile, L.tileEntries, mdp->gamma,
+                           L.sS, L.sT, L.slot, L.tileValid, L.cnt, mdp->d_R, V, Vp, pi,
+                           (int)writePi);
+        return;
+    }
     if (p.tiled)
         hipLaunchKernelGGL(p.tiled, dim3(p.grid), dim3(kBlock), 0, st, mdp->n, mdp->m, mdp->ns,
                            p.statesPerTile, mdp->gamma, mdp->d_S, mdp->d_T, mdp->d_R, V, Vp, pi,
@@ -265,10 +568,11 @@ static ViGraph g_graph;
 static int vi_run_sweeps(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned sweeps,
                          hipStream_t st, hipEvent_t t0, hipEvent_t t1, const char *fn)
 {
-    const ViPlan plan = vi_plan(mdp);
+    const ViPlan plan = vi_plan(mdp, vi);
     const unsigned parity = vi->currentHorizon % 2;
     const void *key[8] = {mdp->d_S, mdp->d_T, mdp->d_R, vi->d_V, vi->d_VPrime, vi->d_pi,
-                          (const void *)plan.tiled, nullptr};
+                          (const void *)plan.tiled,
+                          plan.sorted ? (const void *)plan.sorted->mem : nullptr};
     bool hit = g_graph.exec != nullptr && g_graph.sweeps == sweeps && g_graph.parity == parity &&
                g_graph.n == mdp->n && g_graph.m == mdp->m && g_graph.ns == mdp->ns &&
                g_graph.gamma == mdp->gamma && std::memcmp(g_graph.key, key, sizeof(key)) == 0;
@@ -342,6 +646,7 @@ int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
         NOVA_ERR(fn, "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
     }
+    sorted_drop(vi->d_V, fn);
     device_free_if_owned(vi->d_V, fn, "the value function");
     device_free_if_owned(vi->d_VPrime, fn, "the value function (prime)");
     device_free_if_owned(vi->d_pi, fn, "the policy (pi)");
@@ -370,8 +675,11 @@ int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
         e = hipMemcpy(vi->d_VPrime, V0, n * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemset(vi->d_pi, 0, n * sizeof(unsigned));
-    return hip_fail(e, fn, "Failed to copy memory from host to device for V/pi.",
-                    NOVA_ERROR_MEMCPY_TO_DEVICE);
+    rc = hip_fail(e, fn, "Failed to copy memory from host to device for V/pi.",
+                  NOVA_ERROR_MEMCPY_TO_DEVICE);
+    if (rc == NOVA_SUCCESS && mdp->d_S != nullptr && mdp->d_T != nullptr && sorted_wanted(mdp))
+        rc = sorted_build(mdp, vi->d_V, stream(), fn);
+    return rc;
 }
 
 int mdp_vi_uninitialize_gpu(const MDP *mdp, MDPVIGPU *vi)
@@ -383,6 +691,7 @@ int mdp_vi_uninitialize_gpu(const MDP *mdp, MDPVIGPU *vi)
     }
     vi->currentHorizon = 0;
     int rc = NOVA_SUCCESS;
+    sorted_drop(vi->d_V, fn);
     if (device_free_if_owned(vi->d_V, fn, "the value function") != NOVA_SUCCESS)
         rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(vi->d_VPrime, fn, "the value function (prime)") != NOVA_SUCCESS)
@@ -399,7 +708,7 @@ int mdp_vi_update_gpu(const MDP *mdp, MDPVIGPU *vi)
 {
     const char *fn = "mdp_vi_update_gpu";
     hipStream_t st = stream();
-    const ViPlan plan = vi_plan(mdp);
+    const ViPlan plan = vi_plan(mdp, vi);
     const bool even = (vi->currentHorizon % 2) == 0;
     vi_launch(plan, mdp, even ? vi->d_V : vi->d_VPrime, even ? vi->d_VPrime : vi->d_V, vi->d_pi,
               true, st);
@@ -459,6 +768,17 @@ int mdp_vi_execute_gpu(const MDP *mdp, MDPVIGPU *vi, MDPValueFunction *policy)
         rc = mdp_vi_get_policy_gpu(mdp, vi, policy);
     const int rc2 = mdp_vi_uninitialize_gpu(mdp, vi);
     return (rc != NOVA_SUCCESS) ? rc : rc2;
+}
+
+int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *kernel)
+{
+    if (mdp == nullptr || vi == nullptr || kernel == nullptr) {
+        NOVA_ERR("mdp_vi_sweep_kernel_gpu", "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const ViPlan p = vi_plan(mdp, vi);
+    *kernel = p.sorted ? 2u : (p.tiled ? 1u : 0u);
+    return NOVA_SUCCESS;
 }
 
 int mdp_vi_run_gpu(const MDP *mdp, MDPVIGPU *vi, const float *d_V0, unsigned int sweeps,

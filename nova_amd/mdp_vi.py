@@ -94,6 +94,13 @@ class MDPValueIterationGPU(_abi.NovaMDPVIGPU):
             raise Exception("mdp_vi_run_gpu failed with code %d" % rc)
         return ms.value
 
+    def sweep_kernel(self):
+        """0: one thread per state, 1: wave-tiled sweep, 2: successor-sorted tiles."""
+        k = ct.c_uint(0)
+        if self._lib.mdp_vi_sweep_kernel_gpu(self.mdpPtr, self, ct.byref(k)) != 0:
+            raise Exception("mdp_vi_sweep_kernel_gpu failed")
+        return k.value
+
     def device_values(self):
         """(V, pi) read back from the device without building a policy."""
         pol = self.get_policy()

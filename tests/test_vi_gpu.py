@@ -146,3 +146,55 @@ def test_oracle_sample_at_full_size(oracle):
     idx = np.sort(rng.choice(mdp.n, 20000, replace=False))
     Vo, pio = oracle.vi_sweep(w["S"][idx], w["T"][idx], w["R"][idx], w["gamma"], v0)
     assert np.array_equal(bits(V[idx]), bits(Vo)) and np.array_equal(pi[idx], pio)
+
+
+@pytest.mark.parametrize("n,m,ns,horizon,seed", [
+    (1000, 8, 16, 40, 31),      # 256 states per tile, four tiles
+    (777, 3, 12, 25, 32),       # a partial last tile
+    (2000, 1, 64, 10, 33),      # m = 1, wide rows
+    (300, 16, 200, 6, 34),      # m * ns = 3,200: ten states per tile
+])
+def test_sorted_tiles_bitwise_vs_oracle(oracle, monkeypatch, n, m, ns, horizon, seed):
+    """The successor-sorted layout, forced at small sizes: same V and pi bit for bit, with
+    ragged successor lists (entries after a -1 must stay out of the layout)."""
+    monkeypatch.setenv("NOVA_AMD_VI_KERNEL", "sorted")
+    w = synth.mdp_arrays(n=n, m=m, ns=ns, seed=seed, gamma=0.95)
+    S = w["S"].copy()
+    rng = np.random.default_rng(seed)
+    cut = rng.random((n, m)) < 0.3
+    pos = rng.integers(0, ns, size=(n, m))
+    for s, a in zip(*np.nonzero(cut)):
+        S[s, a, pos[s, a]] = -1
+    S[0, 0, :] = -1                                  # an empty row
+    v0 = rng.uniform(-5, 5, n).astype(np.float32)
+    mdp = MDP.from_arrays(S, w["T"], w["R"], gamma=w["gamma"], horizon=horizon)
+    mdp.initialize_gpu()
+    vi = MDPValueIterationGPU(mdp, numThreads=512, Vinitial=v0)
+    assert vi.sweep_kernel() == 2
+    pol = vi.solve()
+    Vo, pio = oracle.vi_run(S, w["T"], w["R"], w["gamma"], horizon, V0=v0)
+    assert np.array_equal(bits(pol.V_array()), bits(Vo)) and np.array_equal(pol.pi_array(), pio)
+
+
+def test_full_size_sorted_equals_tiled(monkeypatch):
+    """At n = 1e6, m = 8, ns = 16 the default is the sorted layout; four sweeps of it equal
+    four sweeps of the wave-tiled kernel bit for bit."""
+    w = synth.workload("vi_n1e6")
+    mdp = MDP.from_arrays(w["S"], w["T"], w["R"], gamma=w["gamma"], horizon=1)
+    mdp.initialize_gpu()
+    rng = np.random.default_rng(5)
+    v0 = rng.uniform(-50, 0, mdp.n).astype(np.float32)
+    out = {}
+    for kernel in ("default", "tiled"):
+        if kernel == "tiled":
+            monkeypatch.setenv("NOVA_AMD_VI_KERNEL", "tiled")
+        vi = MDPValueIterationGPU(mdp, Vinitial=v0)
+        out[kernel] = vi.sweep_kernel()
+        for _ in range(4):
+            vi.update()
+        V, pi = vi.device_values()
+        out[kernel + "_V"], out[kernel + "_pi"] = V, pi
+        del vi
+    assert out["default"] == 2 and out["tiled"] == 1
+    assert np.array_equal(bits(out["default_V"]), bits(out["tiled_V"]))
+    assert np.array_equal(out["default_pi"], out["tiled_pi"])

@@ -176,7 +176,8 @@ def bench_pbvi(args):
     out = {"workload": "pbvi_n20000", "backups_per_s": r * iters / wall,
            "ms_per_iteration": ms / iters, "stage_ms": dict(zip(
                ["transpose", "score", "action", "materialize"], stages)),
-           "roofline": {"bound": "mfma_fp32" if variant >= 8 else "valu_fp32",
+           "roofline": {"bound": "mfma" if variant >= 8 else "valu",
+                        "precision": "fp32",
                         "kernel": {8: "pbvi_score_mfma<4 waves>", 16: "pbvi_score_mfma<8 waves>",
                                    32: "pbvi_score_mfma<8 waves, 8-step prefetch>"}.get(
                                        variant, "pbvi_score<fast,%d>" % variant),

@@ -15,7 +15,8 @@
 // the matrix cores work on the other; each wave accumulates a 128 x 64 sub-tile as 4 x 2 blocks
 // of 32 x 32 (v_mfma_f32_32x32x16_bf16 on bf16 operands, v_mfma_f32_32x32x2_f32 on fp32
 // operands, fp32 accumulation in both). Workgroups are numbered so that each XCD takes a
-// contiguous run of tiles and shares their A and B panels in its L2.
+// contiguous run of tiles, ordered in groups of four tile rows, and shares their A and B
+// panels in its L2.
 #ifndef NOVA_AMD_DENSE_GEMM_H
 #define NOVA_AMD_DENSE_GEMM_H
 
@@ -43,6 +44,7 @@ constexpr int kGOperandBytes = kGT * kGStride;  // one operand slice
 constexpr size_t kGLdsBytes = 4 * (size_t)kGOperandBytes;   // 2 buffers x 2 operands = 147,456
 constexpr unsigned kGKPad = 64;                 // column padding of the dense operands
 constexpr int kGConvThreads = 256;              // lanes of the conversion kernel
+constexpr unsigned kGGroup = 4;                 // tile rows per group of the tile order
 
 template <typename IN> struct DenseIn;
 template <> struct DenseIn<__bf16> {
@@ -100,8 +102,13 @@ __global__ __launch_bounds__(kGThreads) void dense_score(unsigned rows, unsigned
     // XCD-aware order: workgroups are dealt to the 8 XCDs in turn, so XCD x gets a contiguous
     // run of tiles (whole tile rows), sharing the U_a and Gamma panels in its L2
     const unsigned T = rowTiles * colTiles;
-    const unsigned tileIdx = (T % 8 == 0) ? (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8 : blockIdx.x;
-    const unsigned itile = tileIdx / colTiles, jtile = tileIdx % colTiles;
+    const unsigned q = (T % 8 == 0) ? (blockIdx.x % 8) * (T / 8) + blockIdx.x / 8 : blockIdx.x;
+    // tiles in groups of kGGroup tile rows, column-major inside a group: the 32 workgroups an
+    // XCD runs at once then cover about 4 x 8 tiles and share 12 operand panels in its L2,
+    // where a row-major run of 2 x 16 tiles would share 18
+    const unsigned groupSize = kGGroup * colTiles, firstRow = (q / groupSize) * kGGroup;
+    const unsigned rowsHere = min(kGGroup, rowTiles - firstRow), inGroup = q % groupSize;
+    const unsigned itile = firstRow + inGroup % rowsHere, jtile = inGroup / rowsHere;
     const unsigned i0 = itile * kGT, j0 = jtile * kGT;
     const unsigned t = threadIdx.x, l = t % 64, w = t / 64, wr = w / 4, wc = w % 4;
     const size_t rpad = (size_t)rowTiles * kGT;

@@ -370,6 +370,74 @@ def bench_policy(args):
     return out
 
 
+# nova's benchmark domains (tests/benchmarks/parallel/parallel.py:43-61): file, expansion steps
+DOMAINS = [("tiger_95", 4), ("4x3_95", 5), ("shuttle_95", 4), ("paint_95", 4),
+           ("aloha_10", 6), ("tiger_grid", 6), ("drive_san_francisco", 8), ("hallway2", 7),
+           ("tag", 8)]
+
+
+def bench_domains(args):
+    """PBVI on nova's shipped domains as its parallel benchmark runs them
+    (tests/benchmarks/parallel/parallel.py:64-127): the file's model, 2^k - 1 beliefs added by
+    random expansion (here the seeded GPU expansion), 50 iterations from Gamma0 = Rmin/(1-gamma).
+    GPU: the fast-mode solve (wall clock, after one warm-up solve). CPU: the oracle port's
+    backup of a leading sample of beliefs against the whole initial alpha set on one pinned
+    core (per-backup work does not depend on the belief), as backups/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, set_mode
+    o = Oracle()
+    set_mode(FAST)
+    horizon = 50
+    out = {"workload": "nova's benchmark domains, PBVI, 50 iterations", "domains": {}}
+    ddir = os.path.join(ROOT, "tests", "golden", "domains")
+    for name, k in DOMAINS:
+        p = POMDP().load(os.path.join(ddir, name + ".pomdp"))
+        p.initialize_gpu()
+        p.expand("random", 2 ** k - 1, gpu=True, seed=1)
+        P = p.arrays()
+        del p
+        P["horizon"] = horizon
+        r, n = P["Z"].shape[0], P["S"].shape[0]
+        P["Gamma0"] = np.full((r, n), float(P["R"].min()) / (1.0 - float(P["gamma"])),
+                              np.float32)
+        q = POMDP.from_dict(P)
+        q.initialize_gpu()
+        times = []
+        for _ in range(2):
+            s = POMDPPBVIGPU(q, GammaInitial=P["Gamma0"])
+            variant = s.scoring_variant()           # before solve, which releases the solver
+            t0 = time.perf_counter()
+            s.solve()
+            times.append(time.perf_counter() - t0)
+            del s
+        del q
+        gpu_bps = r * horizon / times[-1]
+        entry = {"n": n, "m": P["S"].shape[1], "z": P["O"].shape[2], "ns": P["S"].shape[2],
+                 "r": r, "rz": P["Z"].shape[1], "gpu_solve_ms": times[-1] * 1e3,
+                 "gpu_backups_per_s": gpu_bps, "scoring_variant": variant}
+        if not args.no_cpu:
+            old = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {min(old)})
+            try:
+                t0 = time.perf_counter()
+                o.pbvi_backup(P, P["Gamma0"], 0, 1, diag=False)
+                t1 = time.perf_counter() - t0
+                kc = int(min(r, max(1, 0.3 / max(t1, 1e-6))))
+                t0 = time.perf_counter()
+                o.pbvi_backup(P, P["Gamma0"], 0, kc, diag=False)
+                tc = time.perf_counter() - t0
+            finally:
+                os.sched_setaffinity(0, old)
+            entry["cpu_baseline"] = {"value": kc / tc, "unit": "backups/s", "cores": 1,
+                                     "kind": "port", "sample": "%d of %d beliefs" % (kc, r)}
+            entry["gpu_over_cpu"] = gpu_bps / (kc / tc)
+        out["domains"][name] = entry
+        print("domain %s done" % name, file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,6 +457,7 @@ def main():
     ap.add_argument("--cpu-beliefs", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-execution leg")
+    ap.add_argument("--no-domains", action="store_true", help="skip nova's benchmark domains")
     ap.add_argument("--policy-beliefs", type=int, default=16384)
     ap.add_argument("--policy-exact-beliefs", type=int, default=1024)
     args = ap.parse_args()
@@ -417,6 +486,8 @@ def main():
         print("batch leg done", file=sys.stderr, flush=True)
     if not args.no_policy:
         extras["policy"] = bench_policy(args)
+    if not args.no_domains:
+        extras["domains"] = bench_domains(args)
     if extras:
         out["extras"] = extras
     print(json.dumps(out))

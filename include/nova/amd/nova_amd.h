@@ -166,6 +166,12 @@ extern "C" int nova_amd_belief_update_batch_gpu(const POMDP *pomdp, unsigned int
 // maximiser of alpha . b over the policy's alphas. Applies to later calls in the process.
 extern "C" int nova_amd_pema_mode(int mode);
 
+// pomdp_expand_pema_cpu (pomdp_expand_cpu.cpp:269-466) with the work of every (belief,
+// action, observation) on the GPU: the same appended belief bit for bit, in either alpha
+// choice. The POMDP must be resident on the device and keep its host arrays; its host Z and B
+// grow by one belief (call pomdp_initialize_gpu again to refresh the device copy).
+extern "C" int pomdp_expand_pema_gpu(POMDP *pomdp, const POMDPAlphaVectors *policy);
+
 // delete[] of a float array the library allocated for the caller (Bnew of the expansions,
 // bp of pomdp_belief_update_cpu), for callers that cannot call delete[] themselves.
 extern "C" int nova_amd_free_floats(float *p);

@@ -209,6 +209,7 @@ EXTRA_SIGNATURES = {
     "nova_amd_belief_update_batch_gpu": [_POMDP, U, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                          ct.c_void_p, ct.c_void_p],
     "nova_amd_pema_mode": [ct.c_int],
+    "pomdp_expand_pema_gpu": [_POMDP, _PAV],
     "mdp_vi_sweep_kernel_gpu": [_MDP, _VIG, _P(ct.c_uint)],
     "mdp_vi_solve_gpu": [_MDP, _VIG, ct.c_void_p, ct.c_double, U, U, _P(ct.c_uint),
                          _P(ct.c_double)],

@@ -177,6 +177,15 @@ int nova_amd_pema_mode(int mode)
     return NOVA_SUCCESS;
 }
 
+} // namespace nova
+
+namespace nova_amd {
+// The current alpha choice of PEMA, for the device variant.
+int pema_mode_current() { return nova::g_pemaMode.load(); }
+} // namespace nova_amd
+
+namespace nova {
+
 // The alpha used for the closest belief c (see above).
 static unsigned pema_alpha(const POMDPAlphaVectors *policy, const float *c, int mode)
 {

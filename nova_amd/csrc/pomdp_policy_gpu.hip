@@ -155,14 +155,15 @@ constexpr int kUpdLdsFloats = 32 * 1024;
 __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned m, unsigned z,
         unsigned k, const unsigned *__restrict__ pstart, const int *__restrict__ pS,
         const float *__restrict__ pT, const float *__restrict__ Ot, const float *__restrict__ B,
-        const unsigned *__restrict__ act, const unsigned *__restrict__ obs,
-        float *__restrict__ Bp, int *__restrict__ status, int useLds)
+        const unsigned *__restrict__ rowOf, const unsigned *__restrict__ act,
+        const unsigned *__restrict__ obs, float *__restrict__ Bp, int *__restrict__ status,
+        int useLds)
 {
     extern __shared__ float row[];
     __shared__ float norm;
     const unsigned q = blockIdx.x, t = threadIdx.x;
     const unsigned a = act[q], o = obs[q];
-    const float *b = B + (size_t)q * n;
+    const float *b = B + (size_t)(rowOf ? rowOf[q] : q) * n;   // rowOf: belief row of triple q
     float *out = Bp + (size_t)q * n;
     if (a >= m || o >= z) {
         if (t == 0)
@@ -375,6 +376,45 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
 
 } // namespace nova_amd
 
+namespace nova_amd {
+
+// tau(b, a, o) for k triples whose beliefs are rows rowOf[q] of B (rowOf null: row q), bit for
+// bit nova's pomdp_belief_update_cpu; shared by the batched entry point and the GPU PEMA.
+int belief_update_rows(const nova::POMDP *pomdp, unsigned k, const float *d_B,
+                       const unsigned *d_rowOf, const unsigned *d_action,
+                       const unsigned *d_observation, float *d_Bp, int *d_status,
+                       const char *fn)
+{
+    const long long bad = first_bad_successor(pomdp->S, (size_t)pomdp->n * pomdp->m, pomdp->ns,
+                                              pomdp->n);
+    if (bad >= 0) {
+        NOVA_ERR(fn, "A successor index lies outside [0, n).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    std::lock_guard<std::mutex> lock(g_updateMutex);
+    int rc = pred_lists(pomdp, fn);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    hipStream_t st = stream();
+    const int useLds = pomdp->n <= (unsigned)kUpdLdsFloats ? 1 : 0;
+    const size_t lds = useLds ? sizeof(float) * pomdp->n : 0;
+    if (useLds)
+        hipFuncSetAttribute((const void *)belief_update_batch,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kUpdLdsFloats * (int)sizeof(float));
+    hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, pomdp->n, pomdp->m,
+                       pomdp->z, k, g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B,
+                       d_rowOf, d_action, d_observation, d_Bp, d_status, useLds);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Kernel execution failed.", NOVA_ERROR_KERNEL_EXECUTION);
+    return NOVA_SUCCESS;
+}
+
+} // namespace nova_amd
+
 using namespace nova_amd;
 
 extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsigned int r,
@@ -487,30 +527,6 @@ extern "C" int nova_amd_belief_update_batch_gpu(const nova::POMDP *pomdp, unsign
                      "k > 0, device arrays).");
         return NOVA_ERROR_INVALID_DATA;
     }
-    const long long bad = first_bad_successor(pomdp->S, (size_t)pomdp->n * pomdp->m, pomdp->ns,
-                                              pomdp->n);
-    if (bad >= 0) {
-        NOVA_ERR(fn, "A successor index lies outside [0, n).");
-        return NOVA_ERROR_INVALID_DATA;
-    }
-    std::lock_guard<std::mutex> lock(g_updateMutex);
-    int rc = pred_lists(pomdp, fn);
-    if (rc != NOVA_SUCCESS)
-        return rc;
-    hipStream_t st = stream();
-    const int useLds = pomdp->n <= (unsigned)kUpdLdsFloats ? 1 : 0;
-    const size_t lds = useLds ? sizeof(float) * pomdp->n : 0;
-    if (useLds)
-        hipFuncSetAttribute((const void *)belief_update_batch,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            kUpdLdsFloats * (int)sizeof(float));
-    hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, pomdp->n, pomdp->m,
-                       pomdp->z, k, g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B,
-                       d_action, d_observation, d_Bp, d_status, useLds);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(st);
-    if (e != hipSuccess)
-        return hip_fail(e, fn, "Kernel execution failed.", NOVA_ERROR_KERNEL_EXECUTION);
-    return NOVA_SUCCESS;
+    return belief_update_rows(pomdp, k, d_B, nullptr, d_action, d_observation, d_Bp, d_status,
+                              fn);
 }

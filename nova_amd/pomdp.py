@@ -48,6 +48,15 @@ class POMDP(_abi.NovaPOMDP):
             raise Exception("Failed to initialize the GPU variables for the POMDP.")
         self.gpuIsInitialized = True
 
+    def _refresh_beliefs_gpu(self):
+        """Re-uploads Z and B after the host belief set changed (r or rz may have grown), so
+        that device kernels never read the old arrays with the new sizes."""
+        if not self.gpuIsInitialized or self.r == 0:
+            return
+        if (self._lib.pomdp_initialize_nonzero_beliefs_gpu(self) != 0 or
+                self._lib.pomdp_initialize_belief_points_gpu(self) != 0):
+            raise Exception("Failed to refresh the device copy of the belief set.")
+
     def uninitialize_gpu(self):
         if not self.gpuIsInitialized:
             return
@@ -127,8 +136,8 @@ class POMDP(_abi.NovaPOMDP):
                seed=None, pemaAlgorithm=None):
         """Expands the belief set (python/nova/pomdp.py:202-229). With gpu=True the random
         trajectories (pomdp_expand_random_gpu, or its seeded variant when seed is given) or the
-        distinct-belief scoring run on the device; the POMDP must then be resident. Call
-        initialize_gpu() again afterwards to refresh the device copy of Z/B."""
+        distinct-belief scoring run on the device; the POMDP must then be resident. When the
+        POMDP is resident, the device copy of Z/B is refreshed after the host set grows."""
         if gpu and method == "random":
             nnz = ct.c_uint(0)
             Bnew = _abi.c_float_p()
@@ -144,6 +153,10 @@ class POMDP(_abi.NovaPOMDP):
                 self._lib.nova_amd_free_floats(Bnew)
         elif gpu and method == "distinct_beliefs":
             rc = self._lib.pomdp_expand_distinct_beliefs_gpu(self)
+        elif gpu and method == "pema":
+            if pemaPolicy is None:
+                pemaPolicy = pemaAlgorithm.solve()
+            rc = self._lib.pomdp_expand_pema_gpu(self, ct.byref(pemaPolicy))
         elif gpu:
             raise Exception("Expansion '%s' has no GPU variant." % method)
         elif seed is not None:
@@ -161,15 +174,17 @@ class POMDP(_abi.NovaPOMDP):
             raise Exception("Failed to expand. Method '%s' is not defined." % method)
         if rc != 0:
             raise Exception("Belief expansion '%s' failed with code %d." % (method, rc))
+        self._refresh_beliefs_gpu()
 
     def sigma_approximate(self, numDesiredNonZeroValues):
         """Keeps each belief's numDesiredNonZeroValues largest entries, renormalised
-        (pomdp_sigma_cpu); returns sigma, the smallest kept mass. Host arrays only: call
-        initialize_gpu() again before GPU work."""
+        (pomdp_sigma_cpu); returns sigma, the smallest kept mass. A resident POMDP's device
+        copy of Z/B is refreshed."""
         sig = ct.c_float()
         rc = self._lib.pomdp_sigma_cpu(self, int(numDesiredNonZeroValues), ct.byref(sig))
         if rc != 0:
             raise Exception("pomdp_sigma_cpu failed with code %d" % rc)
+        self._refresh_beliefs_gpu()
         return sig.value
 
     def belief_update_batch(self, B, actions, observations):

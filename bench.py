@@ -435,7 +435,48 @@ def bench_domains(args):
             entry["gpu_over_cpu"] = gpu_bps / (kc / tc)
         out["domains"][name] = entry
         print("domain %s done" % name, file=sys.stderr, flush=True)
+        if name in PEMA_DOMAINS:
+            out["pema"] = out.get("pema", {})
+            out["pema"][name] = bench_pema_domain(P, horizon)
     return out
+
+
+PEMA_DOMAINS = ("tiger_grid", "hallway2")
+
+
+def bench_pema_domain(P, horizon):
+    """One PEMA call (pomdp_expand_pema_cpu, nova's alpha choice) on the expanded set against
+    the GPU PBVI policy of that set: the host entry point on one core vs the GPU one; the
+    appended beliefs must be equal bit for bit."""
+    import ctypes as ct
+    from nova_amd import _abi
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import POMDPPBVIGPU
+    q = POMDP.from_dict(P)
+    q.initialize_gpu()
+    pol = POMDPPBVIGPU(q, GammaInitial=P["Gamma0"]).solve()
+    res = {}
+    for gpu in (False, True):
+        x = POMDP.from_dict(P)
+        if gpu:
+            x.initialize_gpu()
+        old = os.sched_getaffinity(0)
+        if not gpu:
+            os.sched_setaffinity(0, {min(old)})
+        try:
+            t0 = time.perf_counter()
+            x.expand("pema", pemaPolicy=pol, gpu=gpu)
+            res[gpu] = (time.perf_counter() - t0, x.beliefs())
+        finally:
+            os.sched_setaffinity(0, old)
+        del x
+    (tc, (Zc, Bc)), (tg, (Zg, Bg)) = res[False], res[True]
+    same = bool(np.array_equal(Zc, Zg) and np.array_equal(Bc.view(np.uint32), Bg.view(np.uint32)))
+    r, n = P["Z"].shape[0], P["S"].shape[0]
+    m, z = P["S"].shape[1], P["O"].shape[2]
+    return {"r": r, "alphas": int(pol.r), "closest_search_ops": r * m * z * r * n,
+            "gpu_ms": tg * 1e3, "cpu_ms": tc * 1e3, "gpu_over_cpu": tc / tg,
+            "cpu_cores": 1, "equal_bitwise": same}
 
 
 def main():

@@ -252,12 +252,16 @@ def bench_dense(args):
            "action_disagreement_bf16_vs_fp32": float(np.mean(b["act"] != f["act"])),
            "value_rel_error_bf16_vs_fp32": float(np.max(np.abs(vb - vf)) /
                                                  max(1.0, np.max(np.abs(vf))))}
+    kernels = {"bf16": ("dense_score_bf16", "score stage: per chunk of (a, o) pairs the scaled "
+                        "Gamma (dense_scale_pairs) and the GEMM"),
+               "fp32": ("dense_score<fp32>", "score stage: the GEMM")}
     for label in ("bf16", "fp32"):
         x = res[label]
         out[label] = {"stage_ms": x["stage_ms"],
-                      "roofline": {"bound": "mfma", "kernel": "dense_score<%s>" % label,
+                      "roofline": {"bound": "mfma", "kernel": kernels[label][0],
                                    "achieved": x["achieved"], "peak": x["peak"],
-                                   "unit": "TFLOP/s", "frac": x["achieved"] / x["peak"]}}
+                                   "unit": "TFLOP/s", "frac": x["achieved"] / x["peak"],
+                                   "note": kernels[label][1]}}
     return out
 
 
@@ -335,7 +339,8 @@ def bench_policy(args):
         leg = {"beliefs": kk, "ms_per_call": dt * 1e3, "beliefs_per_s": kk / dt,
                "tflops": tf}
         if prec in peaks:
-            leg["roofline"] = {"bound": "mfma", "kernel": "dense_score<%s>" % prec,
+            leg["roofline"] = {"bound": "mfma",
+                               "kernel": "dense_score_bf16" if prec == "bf16" else "dense_score<fp32>",
                                "achieved": tf, "peak": peaks[prec], "unit": "TFLOP/s",
                                "frac": tf / peaks[prec],
                                "note": "whole call: operand conversion, GEMM and merge"}

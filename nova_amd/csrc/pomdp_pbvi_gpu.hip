@@ -839,23 +839,25 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
 //   C[i][j] = sum_s' U_a[i][s'] * (gamma O[a,s',o]) * Gamma[j][s'],   U_a = B T_a (r x n),
 // a GEMM per (a, o) of r x n by n x r. U_a does not depend on Gamma and is built once per
 // initialize. Operands live in HBM zero-padded to rpad = 256-multiple rows and kpad =
-// 64-multiple columns, so the GEMM loop needs no bounds checks. A workgroup of 8 waves owns a
-// 256 x 256 tile of C: each k-step stages 256 x 128-byte slices of U_a and of Gamma (the latter
-// scaled by gamma O[a,.,o] in registers on the way) into one of two LDS buffers while the matrix
-// cores work on the other; each wave accumulates a 128 x 64 sub-tile as 4 x 2 blocks of 32 x 32
-// (v_mfma_f32_32x32x16_bf16 on bf16 operands, or v_mfma_f32_32x32x2_f32 on fp32 operands, fp32
-// accumulation in both). The epilogue reduces each row to its maximum over the tile's 256
-// alphas (first maximiser), the same partials the sparse kernel writes. The GEMM kernel
-// (dense_score) and the operand conversion live in dense_gemm.h.
+// 64-multiple columns, so the GEMM loop needs no bounds checks.
+//   bf16 (mode 2): Gamma is converted to bf16 once per iteration; then, per chunk of (a, o)
+//     pairs, dense_scale_pairs forms bf16(Gamma * gamma O[a,.,o]) and dense_score_bf16 scores
+//     the chunk against U_a on the matrix cores (k-blocked operands copied straight into LDS).
+//   fp32 (mode 3): dense_score<float> scales Gamma by gamma O[a,.,o] in registers on the way
+//     into LDS.
+// The epilogue reduces each row to its maximum over the tile's 256 alphas (first maximiser),
+// the same partials the sparse kernel writes. The GEMM kernels and the operand conversions
+// live in dense_gemm.h.
 
 // U[a][i][s'] = sum over predecessors (s, l) of s' under a of T(s,a,l) b_i(s), with the dense
 // belief row b_i staged in LDS from its sparse (Z, B) representation. Rows of kpad columns,
-// rpad rows per action; the padding keeps its zeros.
+// rpad rows per action; the padding keeps its zeros. blocked: the k-blocked layout of
+// dense_score_bf16, [kpad / 32][m * rpad][32], instead of [m][rpad][kpad].
 template <typename IN>
-__global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, unsigned rpad,
-        unsigned kpad, const int *__restrict__ Z, const float *__restrict__ B,
-        const unsigned *__restrict__ pstart, const int *__restrict__ pS,
-        const float *__restrict__ pT, IN *__restrict__ U)
+__global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned m, unsigned rz,
+        unsigned rpad, unsigned kpad, int blocked, const int *__restrict__ Z,
+        const float *__restrict__ B, const unsigned *__restrict__ pstart,
+        const int *__restrict__ pS, const float *__restrict__ pT, IN *__restrict__ U)
 {
     extern __shared__ float brow[];
     const unsigned a = blockIdx.y, i = blockIdx.x, t = threadIdx.x;
@@ -875,12 +877,14 @@ __global__ __launch_bounds__(kPB) void dense_build_u(unsigned n, unsigned rz, un
         brow[Z[(size_t)i * rz + k]] = B[(size_t)i * rz + k];
     __syncthreads();
     const unsigned *ps = pstart + (size_t)a * (n + 1);
-    IN *out = U + ((size_t)a * rpad + i) * kpad;
+    const size_t row = (size_t)a * rpad + i, allRows = (size_t)m * rpad;
     for (unsigned sp = t; sp < n; sp += kPB) {
         float acc = 0.0f;
         for (unsigned q = ps[sp]; q < ps[sp + 1]; q++)
             acc += pT[q] * brow[pS[q]];
-        out[sp] = DenseIn<IN>::cvt(acc);
+        const size_t at = blocked ? ((size_t)(sp / kGBlk) * allRows + row) * kGBlk + sp % kGBlk
+                                  : row * kpad + sp;
+        U[at] = DenseIn<IN>::cvt(acc);
     }
 }
 
@@ -1185,9 +1189,14 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
 struct DenseWs {
     const void *key = nullptr;       // POMDPPBVIGPU::d_alphaBA it belongs to
     int mode = 0;
+    bool blocked = false;            // bf16 through dense_score_bf16 (k-blocked operands)
     void *mem = nullptr;
-    void *U = nullptr;               // [m][rpad][kpad] in the scoring precision
-    void *GI = nullptr;              // [rpad][kpad] Gamma in the scoring precision
+    void *U = nullptr;               // [m][rpad][kpad] in the scoring precision (blocked:
+                                     // [kpad / 32][m * rpad][32])
+    void *GI = nullptr;              // [rpad][kpad] Gamma in the scoring precision (blocked:
+                                     // [kpad / 32][rpad][32])
+    void *BP = nullptr;              // blocked: Gamma scaled by gamma*O for a chunk of pairs
+    unsigned chunk = 0;              // blocked: pairs per chunk
     float *Os = nullptr;             // [m][z][kpad] gamma*O
     float *Rb = nullptr;             // [r][m] b . R[:,a]
     float *partVal = nullptr;        // [tiles][r][m][z]
@@ -1221,7 +1230,30 @@ static void dense_launch_score(const DenseWs &D, const float *G, hipStream_t st,
     const size_t rn = (size_t)D.r * D.n;
     const unsigned cg = (unsigned)std::min<size_t>((rn + kGConvThreads - 1) / kGConvThreads, 65536);
     const dim3 sg(D.tiles * D.tiles, D.m * D.z);
-    if (D.mode == 2) {
+    if (D.blocked) {
+        // Gamma once, then per chunk of (a, o) pairs B_p = bf16(Gamma * gamma O[a,.,o]) and
+        // the GEMM of the chunk's pairs against U_a
+        const size_t rowsPad = D.rpad, per = rowsPad * D.kpad;
+        const unsigned cb = (unsigned)std::min<size_t>((per + kGConvThreads - 1) / kGConvThreads,
+                                                       65536);
+        hipLaunchKernelGGL(dense_convert_blk, dim3(cb), dim3(kGConvThreads), 0, st, D.n, D.kpad,
+                           rn, rowsPad, G, static_cast<__bf16 *>(D.GI));
+        if (ev)
+            hipEventRecord(ev, st);
+        const unsigned pairs = D.m * D.z;
+        for (unsigned p0 = 0; p0 < pairs; p0 += D.chunk) {
+            const unsigned cnt = std::min(D.chunk, pairs - p0);
+            const unsigned cs = (unsigned)std::min<size_t>(
+                (per / 8 + kGConvThreads - 1) / kGConvThreads, 4096);
+            hipLaunchKernelGGL(dense_scale_pairs, dim3(cs, cnt), dim3(kGConvThreads), 0, st,
+                               D.kpad, rowsPad, p0, static_cast<const __bf16 *>(D.GI), D.Os,
+                               static_cast<__bf16 *>(D.BP));
+            hipLaunchKernelGGL(dense_score_bf16, dim3(D.tiles * D.tiles, cnt), dim3(kGThreads),
+                               kGDlLdsBytes, st, D.r, D.r, D.kpad, D.m, D.z, D.tiles, D.tiles, p0,
+                               per, static_cast<const __bf16 *>(D.U),
+                               static_cast<const __bf16 *>(D.BP), D.partVal, D.partJ);
+        }
+    } else if (D.mode == 2) {
         hipLaunchKernelGGL(dense_convert<__bf16>, dim3(cg), dim3(kGConvThreads), 0, st, D.n,
                            D.kpad, rn, G, static_cast<__bf16 *>(D.GI));
         if (ev)
@@ -1407,12 +1439,26 @@ static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStrea
     D.rpad = D.tiles * kGT;
     D.kpad = (n + kGKPad - 1) / kGKPad * kGKPad;
     const size_t rmz = (size_t)r * m * z;
+    D.blocked = mode == 2 && !dense_legacy_bf16();
+    // the scaled Gamma of as many pairs as fit about 2 GiB (at least one) per chunk;
+    // NOVA_AMD_DENSE_CHUNK_PAIRS caps the pairs per chunk (for tests of the chunking)
+    const size_t pairBytes = esz * (size_t)D.rpad * D.kpad;
+    size_t chunk = std::max<size_t>(1, std::min<size_t>((size_t)m * z,
+                                                        ((size_t)2 << 30) / pairBytes));
+    if (const char *cap = getenv("NOVA_AMD_DENSE_CHUNK_PAIRS")) {
+        const long v = strtol(cap, nullptr, 10);
+        if (v > 0)
+            chunk = std::min<size_t>(chunk, (size_t)v);
+    }
+    D.chunk = D.blocked ? (unsigned)std::min<size_t>(chunk, 65535) : 0u;   // grid.y limit
     const size_t bU = align256(esz * (size_t)m * D.rpad * D.kpad),
                  bG = align256(esz * (size_t)D.rpad * D.kpad),
                  bO = align256(sizeof(float) * (size_t)m * z * D.kpad),
                  bR = align256(sizeof(float) * (size_t)r * m),
-                 bP = align256(sizeof(float) * D.tiles * rmz);
-    int rc = device_alloc(&D.mem, bU + bG + bO + bR + 2 * bP, fn, "the dense-belief workspace");
+                 bP = align256(sizeof(float) * D.tiles * rmz),
+                 bB = align256(pairBytes * D.chunk);
+    int rc = device_alloc(&D.mem, bU + bG + bO + bR + 2 * bP + bB, fn,
+                          "the dense-belief workspace");
     if (rc != NOVA_SUCCESS)
         return rc;
     char *c = static_cast<char *>(D.mem);
@@ -1421,7 +1467,8 @@ static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStrea
     D.Os = reinterpret_cast<float *>(c); c += bO;
     D.Rb = reinterpret_cast<float *>(c); c += bR;
     D.partVal = reinterpret_cast<float *>(c); c += bP;
-    D.partJ = reinterpret_cast<int *>(c);
+    D.partJ = reinterpret_cast<int *>(c); c += bP;
+    D.BP = D.blocked ? c : nullptr;
     g_dense.push_back(D);
 
     // predecessor lists of every action (host), then the device builds
@@ -1482,19 +1529,23 @@ static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStrea
         if (mode == 2) {
             hipFuncSetAttribute((const void *)dense_build_u<__bf16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, ldsBytes);
-            hipFuncSetAttribute((const void *)dense_score<__bf16>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
-            hipLaunchKernelGGL(dense_build_u<__bf16>, dim3(r, m), dim3(kPB), lds, st, n, p->rz,
-                               Dn.rpad, Dn.kpad, p->d_Z, p->d_B, static_cast<unsigned *>(dps),
-                               static_cast<int *>(dpS), static_cast<float *>(dpT),
-                               static_cast<__bf16 *>(Dn.U));
+            if (Dn.blocked)
+                hipFuncSetAttribute((const void *)dense_score_bf16,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGDlLdsBytes);
+            else
+                hipFuncSetAttribute((const void *)dense_score<__bf16>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
+            hipLaunchKernelGGL(dense_build_u<__bf16>, dim3(r, m), dim3(kPB), lds, st, n, m, p->rz,
+                               Dn.rpad, Dn.kpad, Dn.blocked ? 1 : 0, p->d_Z, p->d_B,
+                               static_cast<unsigned *>(dps), static_cast<int *>(dpS),
+                               static_cast<float *>(dpT), static_cast<__bf16 *>(Dn.U));
         } else {
             hipFuncSetAttribute((const void *)dense_build_u<float>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, ldsBytes);
             hipFuncSetAttribute((const void *)dense_score<float>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
-            hipLaunchKernelGGL(dense_build_u<float>, dim3(r, m), dim3(kPB), lds, st, n, p->rz,
-                               Dn.rpad, Dn.kpad, p->d_Z, p->d_B, static_cast<unsigned *>(dps),
+            hipLaunchKernelGGL(dense_build_u<float>, dim3(r, m), dim3(kPB), lds, st, n, m, p->rz,
+                               Dn.rpad, Dn.kpad, 0, p->d_Z, p->d_B, static_cast<unsigned *>(dps),
                                static_cast<int *>(dpS), static_cast<float *>(dpT),
                                static_cast<float *>(Dn.U));
         }

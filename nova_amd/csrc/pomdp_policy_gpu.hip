@@ -9,8 +9,9 @@
 //              block, both operands staged state-major in LDS 32 states at a time.
 //      fp32  - the dense GEMM of dense_gemm.h on the matrix cores (v_mfma_f32_32x32x2_f32):
 //              fp32 products, fp32 sums in the matrix cores' order.
-//      bf16  - the same GEMM on bf16 operands with fp32 accumulation
-//              (v_mfma_f32_32x32x16_bf16): about 3 significant digits per operand.
+//      bf16  - dense_score_bf16 of dense_gemm.h: bf16 operands in the k-blocked layout,
+//              copied straight into LDS, fp32 accumulation (v_mfma_f32_32x32x16_bf16): about
+//              3 significant digits per operand.
 //    All modes write per-(belief, alpha tile) maxima that one merge kernel reduces in tile
 //    order (the earlier tile wins ties, i.e. the first maximiser overall).
 // 2. Belief updates of k (belief, action, observation) triples
@@ -461,10 +462,13 @@ extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsig
     } else {
         void *A = c + 2 * bPart, *Gi = c + 2 * bPart + bA;
         float *ones = reinterpret_cast<float *>(c + 2 * bPart + bA + bG);
-        // the zero padding survives between calls of one shape; another shape clears it
+        // the zero padding of the row-major layouts survives between calls of one shape and
+        // layout; another shape or layout clears it (the k-blocked layout is written whole)
+        const bool blocked = precision == 2 && !dense_legacy_bf16();
+        const int layout = blocked ? 3 : precision;
         const bool same = g_valueShape.mem == mem && g_valueShape.k == k &&
                           g_valueShape.n == n && g_valueShape.r == r &&
-                          g_valueShape.precision == precision;
+                          g_valueShape.precision == layout;
         if (!same) {
             e = hipMemsetAsync(A, 0, bA + bG, st);
             hipLaunchKernelGGL(fill_ones, dim3((kpad + 255) / 256), dim3(256), 0, st, kpad, ones);
@@ -472,7 +476,7 @@ extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsig
             g_valueShape.k = k;
             g_valueShape.n = n;
             g_valueShape.r = r;
-            g_valueShape.precision = precision;
+            g_valueShape.precision = layout;
         }
         const size_t kn = (size_t)k * n, rn = (size_t)r * n;
         const unsigned ga = (unsigned)std::min<size_t>((kn + kGConvThreads - 1) / kGConvThreads,
@@ -480,7 +484,24 @@ extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsig
         const unsigned gg = (unsigned)std::min<size_t>((rn + kGConvThreads - 1) / kGConvThreads,
                                                        65536);
         const dim3 sg(rowTiles * colTiles, 1);
-        if (precision == 2) {
+        if (blocked) {
+            // k-blocked operands, every padding element written by the conversion
+            const size_t ta = (size_t)rowTiles * kGT * kpad, tg = (size_t)colTiles * kGT * kpad;
+            const unsigned ca = (unsigned)std::min<size_t>((ta + kGConvThreads - 1) / kGConvThreads,
+                                                           65536);
+            const unsigned cg = (unsigned)std::min<size_t>((tg + kGConvThreads - 1) / kGConvThreads,
+                                                           65536);
+            hipFuncSetAttribute((const void *)dense_score_bf16,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGDlLdsBytes);
+            hipLaunchKernelGGL(dense_convert_blk, dim3(ca), dim3(kGConvThreads), 0, st, n, kpad,
+                               kn, (size_t)rowTiles * kGT, d_B, static_cast<__bf16 *>(A));
+            hipLaunchKernelGGL(dense_convert_blk, dim3(cg), dim3(kGConvThreads), 0, st, n, kpad,
+                               rn, (size_t)colTiles * kGT, d_Gamma, static_cast<__bf16 *>(Gi));
+            hipLaunchKernelGGL(dense_score_bf16, sg, dim3(kGThreads), kGDlLdsBytes, st, k, r,
+                               kpad, 1u, 1u, rowTiles, colTiles, 0u, (size_t)0,
+                               static_cast<const __bf16 *>(A), static_cast<const __bf16 *>(Gi),
+                               partV, partJ);
+        } else if (precision == 2) {
             hipFuncSetAttribute((const void *)dense_score<__bf16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGLdsBytes);
             hipLaunchKernelGGL(dense_convert<__bf16>, dim3(ga), dim3(kGConvThreads), 0, st, n,

@@ -53,3 +53,36 @@ def test_dense_step_vs_oracle(oracle, mode, label):
         assert verr < 1e-4
     else:
         assert verr < 1e-2
+
+
+def test_dense_bf16_blocked_equals_legacy(monkeypatch):
+    """Dense bf16 PBVI through the chunked scaled-pair path and the k-blocked GEMM equals the
+    row-major kernel that scales Gamma in registers, bit for bit: the next Gamma and every
+    (belief, action, observation) argmax of one backup (ragged r; the 12 pairs in chunks of
+    5, 5 and 2)."""
+    P = dense_pomdp(n=150, m=3, z=4, ns=4, r=270, seed=5)
+    rng = np.random.default_rng(2)
+    Gt = (P["Gamma0"] + rng.uniform(-1, 1, P["Gamma0"].shape)).astype(np.float32)
+    out = {}
+    monkeypatch.setenv("NOVA_AMD_DENSE_CHUNK_PAIRS", "5")
+    set_mode(2)
+    try:
+        for legacy in (False, True):
+            if legacy:
+                monkeypatch.setenv("NOVA_AMD_DENSE_KERNEL", "legacy")
+            else:
+                monkeypatch.delenv("NOVA_AMD_DENSE_KERNEL", raising=False)
+            p = POMDP.from_dict(P)
+            p.initialize_gpu()
+            s = POMDPPBVIGPU(p, GammaInitial=Gt)
+            s.update()
+            js, act = s.argmax_record()
+            out[legacy] = (js.copy(), act.copy(), s.get_policy().Gamma_array().copy())
+            del s, p
+    finally:
+        set_mode(FAST)
+        monkeypatch.delenv("NOVA_AMD_DENSE_KERNEL", raising=False)
+    (j0, a0, G0), (j1, a1, G1) = out[False], out[True]
+    assert np.array_equal(j0, j1) and np.array_equal(a0, a1)
+    assert np.array_equal(G0.view(np.uint32), G1.view(np.uint32))
+    print("dense bf16 blocked == legacy: r=270 n=150 m=3 z=4", flush=True)

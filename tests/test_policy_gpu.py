@@ -157,3 +157,35 @@ def test_belief_update_batch_bitwise(oracle, n, m, z, ns, k):
     assert np.all(status[:5] == 10)
     print("belief update batch: n=%d k=%d bit-identical (%d impossible observations)"
           % (n, k, int((status == 10).sum())), flush=True)
+
+
+def bf16_round(x):
+    """float32 -> nearest bfloat16 (ties to even), returned as float32."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("k,n,r", [(1, 1, 1), (70, 33, 65), (300, 257, 130), (513, 100, 300),
+                                   (1000, 600, 500)])
+def test_bf16_blocked_kernel_equals_legacy(monkeypatch, k, n, r):
+    """The k-blocked direct-to-LDS bf16 GEMM and the row-major register-staged one round the
+    same operands and accumulate the k-steps in the same order: V, action and alpha are
+    bit-identical (ragged k, n and r; 1 to 10 column tiles)."""
+    G, pi, B = instance(k, n, r, seed=11 * k + n)
+    p = policy(G, pi)
+    monkeypatch.delenv("NOVA_AMD_DENSE_KERNEL", raising=False)
+    V, act, j = (x.cpu().numpy() for x in p.value_and_action_batch(B, "bf16", return_alpha=True))
+    monkeypatch.setenv("NOVA_AMD_DENSE_KERNEL", "legacy")
+    Vl, actl, jl = (x.cpu().numpy() for x in p.value_and_action_batch(B, "bf16",
+                                                                      return_alpha=True))
+    monkeypatch.delenv("NOVA_AMD_DENSE_KERNEL")
+    assert np.array_equal(bits(V), bits(Vl)) and np.array_equal(act, actl)
+    assert np.array_equal(j, jl)
+    # against FP64 on the bf16-rounded operands: only fp32 accumulation error remains
+    C = bf16_round(B).astype(np.float64) @ bf16_round(G).astype(np.float64).T
+    mag = np.abs(bf16_round(B)).astype(np.float64) @ np.abs(bf16_round(G)).astype(np.float64).T
+    tol = 1e-6 * n * mag.max(axis=1) + 1e-30
+    assert np.all(np.abs(V - C[np.arange(k), j]) <= tol)
+    assert np.all(C.max(axis=1) - C[np.arange(k), j] <= 2 * tol)
+    print("bf16 blocked == legacy: k=%d n=%d r=%d" % (k, n, r), flush=True)

@@ -1608,8 +1608,12 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
 {
     const unsigned flip = pb->currentHorizon % 2;
     const unsigned jpt = plain_jpt(pb->d_alphaBA);
+    // every device address the captured kernels hold: the dense workspace is allocated apart
+    // from d_alphaBA, so a re-initialize can move it while the other addresses repeat
+    const DenseWs *dense = dense_for(pb->d_alphaBA);
     const void *key[12] = {p->d_S, p->d_T, p->d_O, p->d_R, p->d_Z, p->d_B, pb->d_Gamma,
-                           pb->d_GammaPrime, pb->d_pi, pb->d_alphaBA, nullptr, nullptr};
+                           pb->d_GammaPrime, pb->d_pi, pb->d_alphaBA,
+                           dense ? dense->mem : nullptr, nullptr};
     const unsigned dims[6] = {p->n, p->ns, p->m, p->z, p->r, p->rz};
     const bool hit = g_pgraph.exec && g_pgraph.iters == iters && g_pgraph.flip == flip &&
                      g_pgraph.gamma == p->gamma && g_pgraph.mode == g_pbvi_mode &&
@@ -1623,7 +1627,7 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
     fill_inst(p, pl.host[0], pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
               reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
     pl.host[0].jpt = jpt;
-    pl.dense = dense_for(pb->d_alphaBA);
+    pl.dense = dense;
     if (pl.dense)
         pl.host[0].tiles = pl.dense->tiles;
     int rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
@@ -1704,6 +1708,10 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
                    "fast mode takes any support length."
                  : "r*m*rz*ns above 2^30: beliefs with full support use the dense modes 2/3.");
         return NOVA_ERROR_INVALID_DATA;
+    }
+    if (g_pgraph.exec && pbvi->d_alphaBA != nullptr && g_pgraph.key[9] == pbvi->d_alphaBA) {
+        hipGraphExecDestroy(g_pgraph.exec);      // it holds the addresses freed below
+        g_pgraph.exec = nullptr;
     }
     if (pbvi->d_alphaBA != nullptr)
         dense_release(pbvi->d_alphaBA, fn);

@@ -86,3 +86,31 @@ def test_dense_bf16_blocked_equals_legacy(monkeypatch):
     assert np.array_equal(j0, j1) and np.array_equal(a0, a1)
     assert np.array_equal(G0.view(np.uint32), G1.view(np.uint32))
     print("dense bf16 blocked == legacy: r=270 n=150 m=3 z=4", flush=True)
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_dense_reinitialize_then_run_like_fresh(mode):
+    """A repeated pomdp_pbvi_initialize_gpu on the same solver (its buffers, and the dense
+    workspace, reallocated) followed by a graph run gives the Gamma and pi of a fresh solver:
+    the run must not replay a graph captured with the released addresses."""
+    import ctypes as ct
+    from nova_amd import _abi
+    P = dense_pomdp(n=90, m=2, z=3, ns=3, r=200, seed=9)
+    Gt = P["Gamma0"].astype(np.float32)
+    set_mode(mode)
+    try:
+        p = POMDP.from_dict(P)
+        p.initialize_gpu()
+        s = POMDPPBVIGPU(p, GammaInitial=Gt)
+        s.run(3)                                   # captures and caches the run graph
+        lib = _abi.load()
+        assert lib.pomdp_pbvi_initialize_gpu(ct.byref(p), ct.byref(s)) == 0
+        s.run(3)
+        G1, pi1 = s.get_policy().Gamma_array().copy(), s.get_policy().pi_array().copy()
+        f = POMDPPBVIGPU(p, GammaInitial=Gt)
+        f.run(3)
+        G2, pi2 = f.get_policy().Gamma_array().copy(), f.get_policy().pi_array().copy()
+    finally:
+        set_mode(FAST)
+    assert np.array_equal(G1.view(np.uint32), G2.view(np.uint32)) and np.array_equal(pi1, pi2)
+    print("dense mode %d: re-initialized solver equals a fresh one" % mode, flush=True)

@@ -357,6 +357,7 @@ struct ViSorted {
     const float *owner = nullptr;
     const int *dS = nullptr;
     const float *dT = nullptr;
+    unsigned long long generation = 0;   // model_generation(true) when built
     unsigned n = 0, m = 0, ns = 0, statesPerTile = 0, tileEntries = 0, tiles = 0;
     void *mem = nullptr;
     int *sS = nullptr;
@@ -368,11 +369,14 @@ struct ViSorted {
 };
 static std::vector<ViSorted> g_sorted;
 
+// The layout of this solver and model, unless S or T were uploaded or released through this
+// library since it was built (it holds copies of them): sweeps then use the wave-tiled kernel,
+// which reads d_S and d_T, until mdp_vi_initialize_gpu builds the layout again.
 static const ViSorted *sorted_for(const nova::MDP *mdp, const nova::MDPVIGPU *vi)
 {
     for (const ViSorted &L : g_sorted)
         if (L.owner == vi->d_V && L.dS == mdp->d_S && L.dT == mdp->d_T && L.n == mdp->n &&
-                L.m == mdp->m && L.ns == mdp->ns)
+                L.m == mdp->m && L.ns == mdp->ns && L.generation == model_generation(true))
             return &L;
     return nullptr;
 }
@@ -418,6 +422,7 @@ static int sorted_build(const nova::MDP *mdp, const float *owner, hipStream_t st
     const unsigned n = mdp->n, m = mdp->m, ns = mdp->ns;
     ViSorted L;
     L.owner = owner;
+    L.generation = model_generation(true);
     L.dS = mdp->d_S;
     L.dT = mdp->d_T;
     L.n = n;

@@ -7,13 +7,32 @@
 #include <nova/mdp/utilities/mdp_model_gpu.h>
 #include <nova/pomdp/utilities/pomdp_model_gpu.h>
 
+#include <atomic>
+
 #include "nova_hip.h"
 
 namespace nova_amd {
 
-template <typename T>
-static int upload(T *&dst, const T *src, size_t count, const char *fn, const char *what)
+static std::atomic<unsigned long long> g_genTransitions{1}, g_genAll{1};
+
+unsigned long long model_generation(bool transitionsOnly)
 {
+    return transitionsOnly ? g_genTransitions.load() : g_genAll.load();
+}
+
+void model_changed(bool transitions)
+{
+    if (transitions)
+        g_genTransitions.fetch_add(1);
+    g_genAll.fetch_add(1);
+}
+
+// transitions: the array is S, T or O
+template <typename T>
+static int upload(T *&dst, const T *src, size_t count, const char *fn, const char *what,
+                  bool transitions)
+{
+    model_changed(transitions);
     device_free_if_owned(dst, fn, what);
     dst = nullptr;
     void *p = nullptr;
@@ -32,8 +51,10 @@ static int upload(T *&dst, const T *src, size_t count, const char *fn, const cha
 
 // Releases a d_* field. Only memory this library allocated is freed: a caller may point a
 // d_* field at device memory it owns (for instance a PyTorch tensor) and keep ownership.
-template <typename T> static int release_device(T *&p, const char *fn, const char *what)
+template <typename T> static int release_device(T *&p, const char *fn, const char *what,
+                                                bool transitions)
 {
+    model_changed(transitions);
     int rc = device_free_if_owned(p, fn, what);
     p = nullptr;
     return rc;
@@ -58,7 +79,8 @@ int mdp_initialize_successors_gpu(MDP *mdp)
         NOVA_ERR(fn, "A successor index is outside [0, n).");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return upload(mdp->d_S, mdp->S, (size_t)mdp->n * mdp->m * mdp->ns, fn, "the successor states");
+    return upload(mdp->d_S, mdp->S, (size_t)mdp->n * mdp->m * mdp->ns, fn,
+                  "the successor states", true);
 }
 
 int mdp_uninitialize_successors_gpu(MDP *mdp)
@@ -67,7 +89,8 @@ int mdp_uninitialize_successors_gpu(MDP *mdp)
         NOVA_ERR("mdp_uninitialize_successors_gpu", "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return release_device(mdp->d_S, "mdp_uninitialize_successors_gpu", "the successor states");
+    return release_device(mdp->d_S, "mdp_uninitialize_successors_gpu", "the successor states",
+                          true);
 }
 
 int mdp_initialize_state_transitions_gpu(MDP *mdp)
@@ -77,7 +100,8 @@ int mdp_initialize_state_transitions_gpu(MDP *mdp)
         NOVA_ERR(fn, "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return upload(mdp->d_T, mdp->T, (size_t)mdp->n * mdp->m * mdp->ns, fn, "the state transitions");
+    return upload(mdp->d_T, mdp->T, (size_t)mdp->n * mdp->m * mdp->ns, fn,
+                  "the state transitions", true);
 }
 
 int mdp_uninitialize_state_transitions_gpu(MDP *mdp)
@@ -86,7 +110,8 @@ int mdp_uninitialize_state_transitions_gpu(MDP *mdp)
         NOVA_ERR("mdp_uninitialize_state_transitions_gpu", "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return release_device(mdp->d_T, "mdp_uninitialize_state_transitions_gpu", "the state transitions");
+    return release_device(mdp->d_T, "mdp_uninitialize_state_transitions_gpu",
+                          "the state transitions", true);
 }
 
 int mdp_initialize_rewards_gpu(MDP *mdp)
@@ -96,7 +121,7 @@ int mdp_initialize_rewards_gpu(MDP *mdp)
         NOVA_ERR(fn, "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return upload(mdp->d_R, mdp->R, (size_t)mdp->n * mdp->m, fn, "the rewards");
+    return upload(mdp->d_R, mdp->R, (size_t)mdp->n * mdp->m, fn, "the rewards", false);
 }
 
 int mdp_uninitialize_rewards_gpu(MDP *mdp)
@@ -105,7 +130,7 @@ int mdp_uninitialize_rewards_gpu(MDP *mdp)
         NOVA_ERR("mdp_uninitialize_rewards_gpu", "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return release_device(mdp->d_R, "mdp_uninitialize_rewards_gpu", "the rewards");
+    return release_device(mdp->d_R, "mdp_uninitialize_rewards_gpu", "the rewards", false);
 }
 
 int mdp_initialize_goals_gpu(MDP *mdp)
@@ -115,7 +140,7 @@ int mdp_initialize_goals_gpu(MDP *mdp)
         NOVA_ERR(fn, "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return upload(mdp->d_goals, mdp->goals, (size_t)mdp->ng, fn, "the goals");
+    return upload(mdp->d_goals, mdp->goals, (size_t)mdp->ng, fn, "the goals", false);
 }
 
 int mdp_uninitialize_goals_gpu(MDP *mdp)
@@ -124,7 +149,7 @@ int mdp_uninitialize_goals_gpu(MDP *mdp)
         NOVA_ERR("mdp_uninitialize_goals_gpu", "Invalid input.");
         return NOVA_ERROR_INVALID_DATA;
     }
-    return release_device(mdp->d_goals, "mdp_uninitialize_goals_gpu", "the goals");
+    return release_device(mdp->d_goals, "mdp_uninitialize_goals_gpu", "the goals", false);
 }
 
 int mdp_initialize_gpu(MDP *mdp)
@@ -157,7 +182,7 @@ int mdp_uninitialize_gpu(MDP *mdp)
 
 // ---------------------------------------------------------------------------- POMDP
 
-#define NOVA_POMDP_HELPERS(NAME, FIELD, DFIELD, COUNT, WHAT, EXTRA_CHECK)                  \
+#define NOVA_POMDP_HELPERS(NAME, FIELD, DFIELD, COUNT, WHAT, TRANSITIONS, EXTRA_CHECK)     \
     int pomdp_initialize_##NAME##_gpu(POMDP *pomdp)                                         \
     {                                                                                       \
         const char *fn = "pomdp_initialize_" #NAME "_gpu";                                  \
@@ -166,7 +191,7 @@ int mdp_uninitialize_gpu(MDP *mdp)
             return NOVA_ERROR_INVALID_DATA;                                                 \
         }                                                                                   \
         EXTRA_CHECK                                                                         \
-        return upload(pomdp->DFIELD, pomdp->FIELD, (size_t)(COUNT), fn, WHAT);              \
+        return upload(pomdp->DFIELD, pomdp->FIELD, (size_t)(COUNT), fn, WHAT, TRANSITIONS); \
     }                                                                                       \
     int pomdp_uninitialize_##NAME##_gpu(POMDP *pomdp)                                       \
     {                                                                                       \
@@ -174,29 +199,30 @@ int mdp_uninitialize_gpu(MDP *mdp)
             NOVA_ERR("pomdp_uninitialize_" #NAME "_gpu", "Invalid input.");                 \
             return NOVA_ERROR_INVALID_DATA;                                                 \
         }                                                                                   \
-        return release_device(pomdp->DFIELD, "pomdp_uninitialize_" #NAME "_gpu", WHAT);     \
+        return release_device(pomdp->DFIELD, "pomdp_uninitialize_" #NAME "_gpu", WHAT,       \
+                              TRANSITIONS);                                                 \
     }
 
 NOVA_POMDP_HELPERS(successors, S, d_S, (size_t)pomdp->n * pomdp->m * pomdp->ns,
-    "the successor states",
+    "the successor states", true,
     if (pomdp->ns == 0 ||
             first_bad_successor(pomdp->S, (size_t)pomdp->n * pomdp->m, pomdp->ns, pomdp->n) >= 0) {
         NOVA_ERR(fn, "A successor index is outside [0, n).");
         return NOVA_ERROR_INVALID_DATA;
     })
 NOVA_POMDP_HELPERS(state_transitions, T, d_T, (size_t)pomdp->n * pomdp->m * pomdp->ns,
-    "the state transitions", )
+    "the state transitions", true, )
 NOVA_POMDP_HELPERS(observation_transitions, O, d_O, (size_t)pomdp->m * pomdp->n * pomdp->z,
-    "the observation transitions", )
-NOVA_POMDP_HELPERS(rewards, R, d_R, (size_t)pomdp->n * pomdp->m, "the rewards", )
+    "the observation transitions", true, )
+NOVA_POMDP_HELPERS(rewards, R, d_R, (size_t)pomdp->n * pomdp->m, "the rewards", false, )
 NOVA_POMDP_HELPERS(nonzero_beliefs, Z, d_Z, (size_t)pomdp->r * pomdp->rz,
-    "the belief supports",
+    "the belief supports", false,
     if (pomdp->r == 0 || pomdp->rz == 0 ||
             first_bad_successor(pomdp->Z, pomdp->r, pomdp->rz, pomdp->n) >= 0) {
         NOVA_ERR(fn, "A belief support index is outside [0, n).");
         return NOVA_ERROR_INVALID_DATA;
     })
-NOVA_POMDP_HELPERS(belief_points, B, d_B, (size_t)pomdp->r * pomdp->rz, "the belief values", )
+NOVA_POMDP_HELPERS(belief_points, B, d_B, (size_t)pomdp->r * pomdp->rz, "the belief values", false, )
 
 int pomdp_initialize_gpu(POMDP *pomdp)
 {

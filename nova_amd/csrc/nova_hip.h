@@ -25,6 +25,14 @@ int device_alloc(void **p, size_t bytes, const char *fn, const char *what);
 int device_free(void *p, const char *fn, const char *what);
 int device_free_if_owned(void *p, const char *fn, const char *what);
 
+// Process-wide counts of the model-array uploads and releases made through this library
+// (the *_initialize_*_gpu / *_uninitialize_*_gpu of MDP and POMDP arrays): of the transition
+// arrays S, T and O only (transitionsOnly), or of any array. Data derived from device-resident
+// model arrays and cached by address compares them, so that arrays re-uploaded at a reused
+// address are not mistaken for the ones it was derived from.
+unsigned long long model_generation(bool transitionsOnly);
+void model_changed(bool transitions);
+
 // Checks that every successor index in the valid prefix of each list lies in [0, limit):
 // the kernels gather V[sp]/Gamma[sp] without further checks. Returns the first bad flat index
 // or -1.

@@ -1592,22 +1592,110 @@ struct PbviGraph {
 };
 static PbviGraph g_pgraph;
 
-// jpt chosen for a plain solver at initialize (by the tuner), per workspace pointer
-static std::vector<std::pair<const void *, unsigned>> g_plain_jpt;
+// What a plain solver was initialized with, per workspace pointer: the arithmetic mode (later
+// calls of nova_amd_pbvi_mode do not change it), the scoring variant the tuner chose, the
+// model sizes its buffers were allocated for, and the model generation that its arrays derived
+// from the model at initialize (O*T products, pair lists, dense operands) were built from.
+struct PlainSolver {
+    const void *ws = nullptr;
+    int mode = 0;
+    unsigned jpt = 2;
+    unsigned dims[6] = {};
+    unsigned long long generation = 0;
+};
+static std::vector<PlainSolver> g_plain;
+
+static PlainSolver *plain_for(const void *ws)
+{
+    for (auto &S : g_plain)
+        if (S.ws == ws)
+            return &S;
+    return nullptr;
+}
+
+static void plain_forget(const void *ws)
+{
+    for (size_t x = 0; x < g_plain.size(); x++)
+        if (g_plain[x].ws == ws) {
+            g_plain.erase(g_plain.begin() + x);
+            return;
+        }
+}
 
 static unsigned plain_jpt(const void *ws)
 {
-    for (auto &kv : g_plain_jpt)
-        if (kv.first == ws)
-            return kv.second;
-    return 2;
+    const PlainSolver *S = plain_for(ws);
+    return S ? S->jpt : 2u;
+}
+
+static void plain_dims(const nova::POMDP *p, unsigned (&d)[6])
+{
+    d[0] = p->n; d[1] = p->ns; d[2] = p->m; d[3] = p->z; d[4] = p->r; d[5] = p->rz;
+}
+
+// The solver's record if it was initialized for p's sizes (its buffers hold r x n alphas and
+// r * m * rz * ns pairs of the sizes at initialize), else an error.
+static int plain_sized(const nova::POMDP *p, const nova::POMDPPBVIGPU *pb, const char *fn,
+                       PlainSolver **out)
+{
+    PlainSolver *S = plain_for(pb->d_alphaBA);
+    if (S == nullptr) {
+        NOVA_ERR(fn, "The solver is not initialized (pomdp_pbvi_initialize_gpu).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    unsigned d[6];
+    plain_dims(p, d);
+    if (std::memcmp(d, S->dims, sizeof(d)) != 0) {
+        NOVA_ERR(fn, "The POMDP's sizes changed since pomdp_pbvi_initialize_gpu; initialize "
+                     "the solver again.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    *out = S;
+    return NOVA_SUCCESS;
+}
+
+static int build_ot(const nova::POMDP *p, const PbviInst &I, hipStream_t st);
+static int dense_setup(const nova::POMDP *p, const void *key, int mode, hipStream_t st,
+                       const char *fn);
+
+// plain_sized, and the arrays derived from the model at initialize built again when model
+// arrays were uploaded or released through this library since (the reference's kernels read
+// the model arrays on every update).
+static int plain_check(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, const char *fn,
+                       PlainSolver **out)
+{
+    PlainSolver *S = nullptr;
+    int rc = plain_sized(p, pb, fn, &S);
+    if (rc != NOVA_SUCCESS || S->generation == model_generation(false)) {
+        *out = S;
+        return rc;
+    }
+    hipStream_t st = stream();
+    PbviInst I;
+    fill_inst(p, I, pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
+              reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
+    rc = build_ot(p, I, st);
+    if (rc == NOVA_SUCCESS && dense_mode(S->mode))
+        rc = dense_setup(p, pb->d_alphaBA, S->mode, st, fn);
+    if (rc == NOVA_SUCCESS)
+        rc = hip_fail(hipStreamSynchronize(st), fn, "Failed to rebuild the model products.",
+                      NOVA_ERROR_KERNEL_EXECUTION);
+    if (rc == NOVA_SUCCESS)
+        S->generation = model_generation(false);
+    *out = S;
+    return rc;
 }
 
 static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned iters,
                           hipStream_t st, hipEvent_t t0, hipEvent_t t1, const char *fn)
 {
+    PlainSolver *S = nullptr;
+    int rc = plain_check(p, pb, fn, &S);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    const int mode = S->mode;
     const unsigned flip = pb->currentHorizon % 2;
-    const unsigned jpt = plain_jpt(pb->d_alphaBA);
+    const unsigned jpt = S->jpt;
     // every device address the captured kernels hold: the dense workspace is allocated apart
     // from d_alphaBA, so a re-initialize can move it while the other addresses repeat
     const DenseWs *dense = dense_for(pb->d_alphaBA);
@@ -1616,7 +1704,7 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
                            dense ? dense->mem : nullptr, nullptr};
     const unsigned dims[6] = {p->n, p->ns, p->m, p->z, p->r, p->rz};
     const bool hit = g_pgraph.exec && g_pgraph.iters == iters && g_pgraph.flip == flip &&
-                     g_pgraph.gamma == p->gamma && g_pgraph.mode == g_pbvi_mode &&
+                     g_pgraph.gamma == p->gamma && g_pgraph.mode == mode &&
                      g_pgraph.jpt == jpt && std::memcmp(g_pgraph.key, key, sizeof(key)) == 0 &&
                      std::memcmp(g_pgraph.dims, dims, sizeof(dims)) == 0;
     // The graph's kernels read the descriptor from plan memory when they run, and other calls
@@ -1630,7 +1718,7 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
     pl.dense = dense;
     if (pl.dense)
         pl.host[0].tiles = pl.dense->tiles;
-    int rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
+    rc = plan_upload(pl, plain_plan_mem(p, pb), st, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
     if (!hit) {
@@ -1638,7 +1726,7 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
             hipGraphExecDestroy(g_pgraph.exec);
             g_pgraph.exec = nullptr;
         }
-        rc = plan_graph(pl, iters, flip, g_pbvi_mode, st, &g_pgraph.exec, fn);
+        rc = plan_graph(pl, iters, flip, mode, st, &g_pgraph.exec, fn);
         if (rc != NOVA_SUCCESS)
             return rc;
         std::memcpy(g_pgraph.key, key, sizeof(key));
@@ -1646,7 +1734,7 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
         g_pgraph.iters = iters;
         g_pgraph.flip = flip;
         g_pgraph.gamma = p->gamma;
-        g_pgraph.mode = g_pbvi_mode;
+        g_pgraph.mode = mode;
         g_pgraph.jpt = jpt;
     }
     if (t0) hipEventRecord(t0, st);
@@ -1713,8 +1801,10 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
         hipGraphExecDestroy(g_pgraph.exec);      // it holds the addresses freed below
         g_pgraph.exec = nullptr;
     }
-    if (pbvi->d_alphaBA != nullptr)
+    if (pbvi->d_alphaBA != nullptr) {
         dense_release(pbvi->d_alphaBA, fn);
+        plain_forget(pbvi->d_alphaBA);
+    }
     device_free_if_owned(pbvi->d_Gamma, fn, "Gamma");
     device_free_if_owned(pbvi->d_GammaPrime, fn, "Gamma (prime)");
     device_free_if_owned(pbvi->d_pi, fn, "pi");
@@ -1753,19 +1843,21 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     rc = build_ot(pomdp, I, st);
     if (rc != NOVA_SUCCESS)
         return hip_fail(hipGetLastError(), fn, "Failed to build the O*T products.", rc);
-    if (dense_mode(g_pbvi_mode))
-        return dense_setup(pomdp, pbvi->d_alphaBA, g_pbvi_mode, st, fn);
-    const unsigned jpt = tune_jpt(I, plain_plan_mem(pomdp, pbvi), st, g_pbvi_mode);
-    bool found = false;
-    for (auto &kv : g_plain_jpt)
-        if (kv.first == pbvi->d_alphaBA) {
-            kv.second = jpt;
-            found = true;
-        }
-    if (!found)
-        g_plain_jpt.push_back({pbvi->d_alphaBA, jpt});
-    e = hipStreamSynchronize(st);
-    return hip_fail(e, fn, "Failed to prepare the PBVI workspace.", NOVA_ERROR_KERNEL_EXECUTION);
+    PlainSolver rec;
+    rec.ws = pbvi->d_alphaBA;
+    rec.mode = g_pbvi_mode;
+    plain_dims(pomdp, rec.dims);
+    rec.generation = model_generation(false);
+    if (dense_mode(g_pbvi_mode)) {
+        rc = dense_setup(pomdp, pbvi->d_alphaBA, g_pbvi_mode, st, fn);
+    } else {
+        rec.jpt = tune_jpt(I, plain_plan_mem(pomdp, pbvi), st, g_pbvi_mode);
+        rc = hip_fail(hipStreamSynchronize(st), fn, "Failed to prepare the PBVI workspace.",
+                      NOVA_ERROR_KERNEL_EXECUTION);
+    }
+    if (rc == NOVA_SUCCESS)
+        g_plain.push_back(rec);
+    return rc;
 }
 
 int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
@@ -1782,8 +1874,10 @@ int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
         hipGraphExecDestroy(g_pgraph.exec);
         g_pgraph.exec = nullptr;
     }
-    if (pbvi->d_alphaBA != nullptr)
+    if (pbvi->d_alphaBA != nullptr) {
         dense_release(pbvi->d_alphaBA, fn);
+        plain_forget(pbvi->d_alphaBA);
+    }
     if (device_free_if_owned(pbvi->d_Gamma, fn, "Gamma") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(pbvi->d_GammaPrime, fn, "Gamma (prime)") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(pbvi->d_pi, fn, "pi") != NOVA_SUCCESS) rc = NOVA_ERROR_DEVICE_FREE;
@@ -1806,6 +1900,9 @@ int pomdp_pbvi_get_policy_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, POMDPAlpha
         NOVA_ERR(fn, "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
     }
+    PlainSolver *S = nullptr;
+    if (plain_sized(pomdp, pbvi, fn, &S) != NOVA_SUCCESS)
+        return NOVA_ERROR_INVALID_DATA;
     if (pomdp_alpha_vectors_initialize(policy, pomdp->n, pomdp->m, pomdp->r) != NOVA_SUCCESS) {
         NOVA_ERR(fn, "Could not create the policy.");
         return NOVA_ERROR_POLICY_CREATION;
@@ -1877,6 +1974,9 @@ int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi, int *jst
         NOVA_ERR(fn, "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
     }
+    PlainSolver *S = nullptr;
+    if (plain_sized(pomdp, pbvi, fn, &S) != NOVA_SUCCESS)
+        return NOVA_ERROR_INVALID_DATA;
     PbviInst I;
     inst_ws(pomdp, reinterpret_cast<char *>(pbvi->d_alphaBA), I);
     hipError_t e = hipSuccess;
@@ -1910,18 +2010,22 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
         NOVA_ERR(fn, "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
     }
+    PlainSolver *S = nullptr;
+    int rc = plain_check(pomdp, pbvi, fn, &S);
+    if (rc != NOVA_SUCCESS)
+        return rc;
     hipStream_t st = stream();
     PbviPlan pl;
     pl.count = 1;
     pl.host.resize(1);
     fill_inst(pomdp, pl.host[0], pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
               reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
-    pl.host[0].jpt = plain_jpt(pbvi->d_alphaBA);
-    int rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
+    pl.host[0].jpt = S->jpt;
+    rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
     const unsigned flip = pbvi->currentHorizon % 2;
-    const int mode = g_pbvi_mode;
+    const int mode = S->mode;
     const unsigned c = 1;
     const unsigned *tr = pl.dOff, *be = pl.dOff + (kVariants + 1) * (c + 1);
     hipEvent_t ev[5];
@@ -1989,20 +2093,24 @@ int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned
         NOVA_ERR(fn, "Invalid arguments (recording needs a sparse-belief solver).");
         return NOVA_ERROR_INVALID_DATA;
     }
+    PlainSolver *S = nullptr;
+    int rc = plain_check(pomdp, pbvi, fn, &S);
+    if (rc != NOVA_SUCCESS)
+        return rc;
     hipStream_t st = stream();
     PbviPlan pl;
     pl.count = 1;
     pl.host.resize(1);
     fill_inst(pomdp, pl.host[0], pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
               reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
-    pl.host[0].jpt = plain_jpt(pbvi->d_alphaBA);
+    pl.host[0].jpt = S->jpt;
     pl.host[0].recJ = d_recJ;
     pl.host[0].recA = d_recA;
-    int rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
+    rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
     hipGraphExec_t exec = nullptr;
-    rc = plan_graph(pl, iterations, pbvi->currentHorizon % 2, g_pbvi_mode, st, &exec, fn);
+    rc = plan_graph(pl, iterations, pbvi->currentHorizon % 2, S->mode, st, &exec, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
     hipError_t e = hipGraphLaunch(exec, st);

@@ -266,6 +266,7 @@ std::mutex g_updateMutex;
 // predecessor lists of the last model seen, keyed by its device arrays
 struct PredLists {
     const void *dS = nullptr, *dT = nullptr, *dO = nullptr;
+    unsigned long long generation = 0;      // model_generation(true) when built
     unsigned n = 0, m = 0, ns = 0, z = 0;
     void *mem = nullptr;
     size_t bytes = 0;
@@ -282,12 +283,12 @@ inline size_t up256(size_t x) { return (x + 255) / 256 * 256; }
 // Predecessor lists per action of p's model, (s, T[s,a,l]) for every valid successor entry,
 // grouped by successor s' and ordered by (s, l) within each group, built on the host from p's
 // host arrays; and the observation-major copy of O, built on the device. Kept until another
-// model (other device arrays or sizes) is passed.
+// model (other device arrays or sizes) is passed, or S, T or O are uploaded or released again.
 static int pred_lists(const nova::POMDP *p, const char *fn)
 {
     if (g_pred.mem != nullptr && g_pred.dS == p->d_S && g_pred.dT == p->d_T &&
             g_pred.dO == p->d_O && g_pred.n == p->n && g_pred.m == p->m && g_pred.ns == p->ns &&
-            g_pred.z == p->z)
+            g_pred.z == p->z && g_pred.generation == model_generation(true))
         return NOVA_SUCCESS;
     const unsigned n = p->n, m = p->m, ns = p->ns;
     std::vector<unsigned> pstart((size_t)m * (n + 1), 0);
@@ -362,6 +363,7 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
     g_pred.dS = p->d_S;
     g_pred.dT = p->d_T;
     g_pred.dO = p->d_O;
+    g_pred.generation = model_generation(true);
     g_pred.n = n;
     g_pred.m = m;
     g_pred.ns = ns;

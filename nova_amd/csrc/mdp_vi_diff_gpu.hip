@@ -253,6 +253,7 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
                      unsigned int maxSweeps, unsigned int flags, unsigned int *sweepsDone,
                      double *residual)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_solve_gpu";
     if (mdp == nullptr || mdp->n == 0 || mdp->m == 0 || mdp->ns == 0 || mdp->d_S == nullptr ||
             mdp->d_T == nullptr || mdp->d_R == nullptr || mdp->m > 1024 || vi == nullptr ||
@@ -331,6 +332,7 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
                        float tol, unsigned int maxIters, unsigned int *itersDone,
                        double *residual64)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_adjoint_gpu";
     if (mdp == nullptr || mdp->n == 0 || mdp->m == 0 || mdp->ns == 0 || mdp->d_S == nullptr ||
             mdp->d_T == nullptr || d_pi == nullptr || d_V64 == nullptr || d_rho == nullptr ||

@@ -646,6 +646,7 @@ using namespace nova_amd;
 
 int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_initialize_gpu";
     if (mdp == nullptr || mdp->n == 0 || vi == nullptr) {
         NOVA_ERR(fn, "Invalid arguments.");
@@ -689,6 +690,7 @@ int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
 
 int mdp_vi_uninitialize_gpu(const MDP *mdp, MDPVIGPU *vi)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_uninitialize_gpu";
     if (mdp == nullptr || vi == nullptr) {
         NOVA_ERR(fn, "Invalid arguments.");
@@ -711,6 +713,7 @@ int mdp_vi_uninitialize_gpu(const MDP *mdp, MDPVIGPU *vi)
 
 int mdp_vi_update_gpu(const MDP *mdp, MDPVIGPU *vi)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_update_gpu";
     hipStream_t st = stream();
     const ViPlan plan = vi_plan(mdp, vi);
@@ -731,6 +734,7 @@ int mdp_vi_update_gpu(const MDP *mdp, MDPVIGPU *vi)
 
 int mdp_vi_get_policy_gpu(const MDP *mdp, MDPVIGPU *vi, MDPValueFunction *policy)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_get_policy_gpu";
     if (mdp == nullptr || vi == nullptr || policy == nullptr) {
         NOVA_ERR(fn, "Invalid arguments.");
@@ -750,6 +754,7 @@ int mdp_vi_get_policy_gpu(const MDP *mdp, MDPVIGPU *vi, MDPValueFunction *policy
 
 int mdp_vi_execute_gpu(const MDP *mdp, MDPVIGPU *vi, MDPValueFunction *policy)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_execute_gpu";
     if (mdp == nullptr || mdp->n == 0 || mdp->ns == 0 || mdp->m == 0 || mdp->S == nullptr ||
             mdp->T == nullptr || mdp->R == nullptr || mdp->gamma < 0.0f || mdp->gamma > 1.0f ||
@@ -777,6 +782,7 @@ int mdp_vi_execute_gpu(const MDP *mdp, MDPVIGPU *vi, MDPValueFunction *policy)
 
 int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *kernel)
 {
+    NOVA_AMD_API_LOCK();
     if (mdp == nullptr || vi == nullptr || kernel == nullptr) {
         NOVA_ERR("mdp_vi_sweep_kernel_gpu", "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
@@ -789,6 +795,7 @@ int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *ke
 int mdp_vi_run_gpu(const MDP *mdp, MDPVIGPU *vi, const float *d_V0, unsigned int sweeps,
                    float *elapsedMs)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "mdp_vi_run_gpu";
     if (!mdp_device_ready(mdp) || vi == nullptr || vi->d_V == nullptr ||
             vi->d_VPrime == nullptr || vi->d_pi == nullptr || sweeps == 0) {

@@ -2,6 +2,8 @@
 #ifndef NOVA_AMD_HIP_H
 #define NOVA_AMD_HIP_H
 
+#include <mutex>
+
 #include <hip/hip_runtime.h>
 
 #include "nova_common.h"
@@ -13,6 +15,13 @@ namespace nova_amd {
 // stream PyTorch and plain HIP callers use by default). Every ABI call is synchronous at
 // return: it synchronises this stream before returning.
 hipStream_t stream();
+
+// Serialises the library's GPU entry points, which share per-process state: the library
+// stream, cached layouts, graphs, workspaces and solver records. Recursive, because entry
+// points call one another (execute runs initialize, update and get-policy).
+std::recursive_mutex &api_mutex();
+#define NOVA_AMD_API_LOCK() \
+    const std::lock_guard<std::recursive_mutex> novaAmdApiLock_(nova_amd::api_mutex())
 
 // Reports a HIP failure as "Error[fn]: msg (hip error text)" and returns code, or returns
 // NOVA_SUCCESS when e == hipSuccess.

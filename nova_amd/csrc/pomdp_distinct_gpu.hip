@@ -238,6 +238,7 @@ using namespace nova_amd;
 
 int pomdp_expand_distinct_beliefs_gpu(POMDP *pomdp)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_expand_distinct_beliefs_gpu";
     if (pomdp == nullptr || pomdp->n == 0 || pomdp->ns == 0 || pomdp->m == 0 || pomdp->z == 0 ||
             pomdp->r == 0 || pomdp->rz == 0 || pomdp->S == nullptr || pomdp->T == nullptr ||

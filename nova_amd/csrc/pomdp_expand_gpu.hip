@@ -297,6 +297,7 @@ static bool expand_args_ok(const POMDP *p, unsigned num, float *Bnew)
 int pomdp_expand_random_gpu(const POMDP *pomdp, unsigned int numThreads,
         unsigned int numDesiredBeliefPoints, unsigned int &maxNonZeroValues, float *&Bnew)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_expand_random_gpu";
     (void)numThreads;
     if (!expand_args_ok(pomdp, numDesiredBeliefPoints, Bnew)) {
@@ -316,6 +317,7 @@ int pomdp_expand_random_gpu(const POMDP *pomdp, unsigned int numThreads,
 int pomdp_expand_random_seeded_gpu(const POMDP *pomdp, unsigned int seed,
         unsigned int numDesiredBeliefPoints, unsigned int *maxNonZeroValues, float **Bnew)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_expand_random_seeded_gpu";
     if (maxNonZeroValues == nullptr || Bnew == nullptr ||
             !expand_args_ok(pomdp, numDesiredBeliefPoints, *Bnew)) {

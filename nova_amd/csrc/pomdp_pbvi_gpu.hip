@@ -1776,6 +1776,7 @@ using namespace nova_amd;
 
 int nova_amd_pbvi_mode(int mode)
 {
+    NOVA_AMD_API_LOCK();
     const int old = g_pbvi_mode;
     if (mode >= 0 && mode <= 3)
         g_pbvi_mode = mode;
@@ -1784,6 +1785,7 @@ int nova_amd_pbvi_mode(int mode)
 
 int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_initialize_gpu";
     if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->GammaInitial == nullptr) {
         NOVA_ERR(fn, "Invalid arguments (the POMDP must be resident on the device and "
@@ -1862,6 +1864,7 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
 
 int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_uninitialize_gpu";
     (void)pomdp;
     if (pbvi == nullptr) {
@@ -1890,11 +1893,13 @@ int pomdp_pbvi_uninitialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
 
 int pomdp_pbvi_update_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
 {
+    NOVA_AMD_API_LOCK();
     return pbvi_run_iters(pomdp, pbvi, 1, stream(), nullptr, nullptr, "pomdp_pbvi_update_gpu");
 }
 
 int pomdp_pbvi_get_policy_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, POMDPAlphaVectors *policy)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_get_policy_gpu";
     if (pomdp == nullptr || pbvi == nullptr || policy == nullptr) {
         NOVA_ERR(fn, "Invalid arguments.");
@@ -1918,6 +1923,7 @@ int pomdp_pbvi_get_policy_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, POMDPAlpha
 
 int pomdp_pbvi_execute_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, POMDPAlphaVectors *policy)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_execute_gpu";
     if (!pomdp_device_ready(pomdp) || pomdp->gamma < 0.0f || pomdp->gamma > 1.0f ||
             pomdp->horizon < 1 || pbvi == nullptr || pbvi->GammaInitial == nullptr ||
@@ -1942,6 +1948,7 @@ int pomdp_pbvi_execute_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, POMDPAlphaVec
 int pomdp_pbvi_run_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned int iterations,
                        float *elapsedMs)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_run_gpu";
     if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->d_Gamma == nullptr ||
             pbvi->d_GammaPrime == nullptr || pbvi->d_alphaBA == nullptr || iterations == 0) {
@@ -1969,6 +1976,7 @@ int pomdp_pbvi_run_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned int iter
 int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi, int *jstar,
                           unsigned int *actions)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_argmax_gpu";
     if (pomdp == nullptr || pbvi == nullptr || pbvi->d_alphaBA == nullptr) {
         NOVA_ERR(fn, "Invalid arguments.");
@@ -1994,6 +2002,7 @@ int pomdp_pbvi_argmax_gpu(const POMDP *pomdp, const POMDPPBVIGPU *pbvi, int *jst
 // pomdp_pbvi_update_gpu.
 int pomdp_pbvi_scoring_variant_gpu(const POMDPPBVIGPU *pbvi, unsigned int *variant)
 {
+    NOVA_AMD_API_LOCK();
     if (pbvi == nullptr || pbvi->d_alphaBA == nullptr || variant == nullptr) {
         NOVA_ERR("pomdp_pbvi_scoring_variant_gpu", "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
@@ -2004,6 +2013,7 @@ int pomdp_pbvi_scoring_variant_gpu(const POMDPPBVIGPU *pbvi, unsigned int *varia
 
 int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageMs)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_profile_gpu";
     if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->d_alphaBA == nullptr ||
             stageMs == nullptr) {
@@ -2086,6 +2096,7 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
 int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned int iterations,
         int *d_recJ, int *d_recA)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_run_recorded_gpu";
     if (!pomdp_device_ready(pomdp) || pbvi == nullptr || pbvi->d_alphaBA == nullptr ||
             iterations == 0 || d_recJ == nullptr || d_recA == nullptr ||
@@ -2129,6 +2140,7 @@ int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned
 int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps, POMDPPBVIGPU *const *pbvis,
         unsigned int count, POMDPAlphaVectors *policies, float *elapsedMs)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_batch_execute_gpu";
     if (pomdps == nullptr || pbvis == nullptr || count == 0 || dense_mode(g_pbvi_mode)) {
         NOVA_ERR(fn, "Invalid arguments (batches use the sparse modes 0/1).");
@@ -2281,6 +2293,7 @@ using namespace nova_amd;
 
 int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlphaVectors *policy)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_perseus_execute_gpu";
     if (!pomdp_device_ready(pomdp) || pomdp->gamma < 0.0f || pomdp->gamma > 1.0f ||
             pomdp->horizon < 1 || per == nullptr || per->GammaInitial == nullptr ||

@@ -163,6 +163,7 @@ using namespace nova_amd;
 int pomdp_pbvi_belief_values_gpu(const POMDP *pomdp, const float *d_Gamma, unsigned int rGamma,
         float *d_values, int *d_arg)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_belief_values_gpu";
     if (pomdp == nullptr || pomdp->d_Z == nullptr || pomdp->d_B == nullptr || d_Gamma == nullptr ||
             rGamma == 0 || d_values == nullptr || d_arg == nullptr) {
@@ -181,6 +182,7 @@ int pomdp_pbvi_belief_values_gpu(const POMDP *pomdp, const float *d_Gamma, unsig
 int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations, const int *d_recJ,
         const int *d_recA, const int *d_finalArg, float *d_dR)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_pbvi_replay_grad_gpu";
     if (pomdp == nullptr || pomdp->S == nullptr || pomdp->T == nullptr ||
             pomdp->d_O == nullptr || pomdp->d_Z == nullptr || pomdp->d_B == nullptr ||

@@ -243,6 +243,7 @@ namespace nova {
 
 int pomdp_expand_pema_gpu(POMDP *pomdp, const POMDPAlphaVectors *policy)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "pomdp_expand_pema_gpu";
     if (pomdp == nullptr || pomdp->n == 0 || pomdp->ns == 0 || pomdp->m == 0 ||
             pomdp->z == 0 || pomdp->r == 0 || pomdp->rz == 0 || pomdp->S == nullptr ||

@@ -28,6 +28,12 @@ hipStream_t stream()
     return g_stream[dev];
 }
 
+std::recursive_mutex &api_mutex()
+{
+    static std::recursive_mutex mu;
+    return mu;
+}
+
 int hip_fail(hipError_t e, const char *fn, const char *msg, int code)
 {
     if (e == hipSuccess)

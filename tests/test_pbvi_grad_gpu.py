@@ -32,7 +32,7 @@ def test_replay_vs_finite_differences(oracle):
     L.backward()
     g = R.grad.cpu().numpy()
     L64 = oracle_L(oracle, P, P["R"], 8)
-    assert abs(float(L) - L64) <= 1e-4 * max(1.0, abs(L64))
+    assert abs(float(L.detach()) - L64) <= 1e-4 * max(1.0, abs(L64))
     rng = np.random.default_rng(0)
     h = 1e-3
     errs = []

@@ -19,8 +19,10 @@
 //    sum_s T[s,a,s'] b(s), normalised by its sum over s'. The sum over predecessors runs over a
 //    list ordered by (s, successor slot) -- the order in which the reference's scatter adds to
 //    b'(s') -- and the normaliser is summed serially over s' ascending, so b' equals nova's
-//    bit for bit. The belief is gathered from LDS and O is read from an observation-major copy
-//    built once per model. A triple whose normaliser is below 1e-9 in magnitude keeps the unnormalised
+//    bit for bit. Three kernels: the unnormalised b' (predecessor entries read coalesced, their
+//    products summed per s' from LDS, b gathered from LDS, O from an observation-major copy
+//    built once per model), the normalisers (one serial chain per triple fed through LDS), and
+//    the division. A triple whose normaliser is below 1e-9 in magnitude keeps the unnormalised
 //    b' and reports NOVA_WARNING_INVALID_BELIEF, as the reference returns it.
 #include <algorithm>
 #include <climits>
@@ -145,32 +147,36 @@ __global__ __launch_bounds__(256) void fill_ones(unsigned count, float *__restri
         x[i] = 1.0f;
 }
 
-// One workgroup per triple q. b is staged in LDS; u(s') is summed over the ordered
-// predecessor lists of action a from there, b'(s') = u(s') * O[a,s',o] is read from the
-// observation-major copy Ot[a][o][s'] (coalesced) and written out; b' is then copied back into
-// the LDS buffer, lane 0 sums it serially for the normaliser, and b' is divided by it. When n
-// exceeds the LDS buffer the same steps read b and b' from global memory.
+// Belief updates in three kernels, each sum in the reference's order:
+//   belief_update_batch: one workgroup per triple q; b is staged in LDS. Wave w takes blocks
+//     of 64 consecutive states s'. The block's predecessor entries (contiguous in the lists)
+//     are read 64 at a time, coalesced, and their products T * b(s) formed into the wave's LDS
+//     scratch; lane t then adds the products of s' = block + t in list order, multiplies by
+//     O[a,s',o] (from the observation-major copy Ot[a][o][s']) and writes the unnormalised
+//     b'(s'). When n exceeds the LDS buffer b is read from global memory.
+//   belief_norms: the normaliser of triple q, summed serially over s' by one lane; 16 triples
+//     per workgroup, their rows streamed through double-buffered LDS by the other waves.
+//   belief_divide: b'_q / c_q for the triples whose normaliser is usable.
 constexpr int kUB = 1024;
-constexpr int kUpdLdsFloats = 32 * 1024;
+constexpr int kUpdWaves = kUB / 64;
+constexpr int kUpdScratch = 1024;                // product slots per wave
+constexpr int kUpdLdsFloats = 40 * 1024 - kUpdWaves * kUpdScratch;   // 160 KB in all
 
 __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned m, unsigned z,
-        unsigned k, const unsigned *__restrict__ pstart, const int *__restrict__ pS,
+        const unsigned *__restrict__ pstart, const int *__restrict__ pS,
         const float *__restrict__ pT, const float *__restrict__ Ot, const float *__restrict__ B,
         const unsigned *__restrict__ rowOf, const unsigned *__restrict__ act,
-        const unsigned *__restrict__ obs, float *__restrict__ Bp, int *__restrict__ status,
-        int useLds)
+        const unsigned *__restrict__ obs, float *__restrict__ Bp, int useLds)
 {
-    extern __shared__ float row[];
-    __shared__ float norm;
-    const unsigned q = blockIdx.x, t = threadIdx.x;
+    extern __shared__ float smem[];
+    const unsigned q = blockIdx.x, t = threadIdx.x, lane = t % 64, w = t / 64;
     const unsigned a = act[q], o = obs[q];
+    if (a >= m || o >= z)
+        return;                                  // belief_norms reports it
     const float *b = B + (size_t)(rowOf ? rowOf[q] : q) * n;   // rowOf: belief row of triple q
     float *out = Bp + (size_t)q * n;
-    if (a >= m || o >= z) {
-        if (t == 0)
-            status[q] = NOVA_ERROR_INVALID_DATA;
-        return;
-    }
+    float *row = smem;                                           // n floats when useLds
+    float *scr = smem + (useLds ? n : 0) + w * kUpdScratch;
     if (useLds) {
         for (unsigned s = t; s < n; s += kUB)
             row[s] = b[s];
@@ -179,41 +185,100 @@ __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned 
     const float *src = useLds ? row : b;
     const unsigned *ps = pstart + (size_t)a * (n + 1);
     const float *Oao = Ot + ((size_t)a * z + o) * n;
-    for (unsigned sp = t; sp < n; sp += kUB) {
+    for (unsigned s0 = w * 64; s0 < n; s0 += kUpdWaves * 64) {
+        const unsigned s1 = min(s0 + 64, n), sp = s0 + lane;
+        const unsigned xa = ps[s0], xb = ps[s1];
+        const unsigned lo = sp < n ? ps[sp] : xb, hi = sp < n ? ps[sp + 1] : xb;
         float u = 0.0f;
-        for (unsigned x = ps[sp]; x < ps[sp + 1]; x++)
-            u = __fadd_rn(u, __fmul_rn(pT[x], src[pS[x]]));
-        out[sp] = __fmul_rn(u, Oao[sp]);
+        for (unsigned xs = xa; xs < xb; xs += kUpdScratch) {
+            const unsigned xe = min(xs + (unsigned)kUpdScratch, xb);
+            for (unsigned x = xs + lane; x < xe; x += 64)
+                scr[x - xs] = __fmul_rn(pT[x], src[pS[x]]);
+            // the wave's own LDS writes, then reads by other lanes of the wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const unsigned e1 = min(hi, xe);
+            for (unsigned x = max(lo, xs); x < e1; x++)
+                u = __fadd_rn(u, scr[x - xs]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (sp < n)
+            out[sp] = __fmul_rn(u, Oao[sp]);
     }
-    __syncthreads();
-    if (useLds) {
-        // each lane re-reads the entries it wrote itself
-        for (unsigned sp = t; sp < n; sp += kUB)
-            row[sp] = out[sp];
-        __syncthreads();
-    }
-    if (t == 0) {
-        const float *v = useLds ? row : out;
-        float c = 0.0f;
-#pragma unroll 8
-        for (unsigned sp = 0; sp < n; sp++)
-            c = __fadd_rn(c, v[sp]);
-        norm = c;
-    }
-    __syncthreads();
-    const float c = norm;
-    if ((double)fabsf(c) < NOVA_FLT_ERR_TOL) {
-        if (t == 0)
-            status[q] = NOVA_WARNING_INVALID_BELIEF;
-        return;
-    }
-    for (unsigned sp = t; sp < n; sp += kUB)
-        out[sp] = __fdiv_rn(useLds ? row[sp] : out[sp], c);
-    if (t == 0)
-        status[q] = NOVA_SUCCESS;
 }
 
-// Ot[a][o][s'] = O[a][s'][o]
+constexpr unsigned kNormT = 16;          // triples per workgroup of belief_norms
+constexpr unsigned kNormC = 512;         // states per chunk
+constexpr unsigned kNormLoaders = 256;   // lanes of waves 1-4, 32 values each per chunk
+
+__global__ __launch_bounds__(64 + kNormLoaders) void belief_norms(unsigned n, unsigned m,
+        unsigned z, unsigned k, const unsigned *__restrict__ act,
+        const unsigned *__restrict__ obs, const float *__restrict__ Bp,
+        float *__restrict__ norms, int *__restrict__ status)
+{
+    // waves 1-4 stream chunks of the 16 rows into one buffer (coalesced, all 32 loads of a
+    // lane in flight together) while lanes 0-15 of wave 0 add the other buffer's chunk into
+    // their triple's sum, in order; rows padded to 513 floats so that the 16 summing lanes
+    // read distinct banks
+    constexpr unsigned kPer = kNormT * kNormC / kNormLoaders;
+    __shared__ float buf[2][kNormT][kNormC + 1];
+    const unsigned t = threadIdx.x, q0 = blockIdx.x * kNormT;
+    auto load = [&](unsigned c0, unsigned which) {
+        float v[kPer];
+#pragma unroll
+        for (unsigned u = 0; u < kPer; u++) {
+            const unsigned e = (t - 64) + kNormLoaders * u, j = e / kNormC, i = e % kNormC;
+            v[u] = (q0 + j < k && c0 + i < n) ? Bp[(size_t)(q0 + j) * n + c0 + i] : 0.0f;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < kPer; u++) {
+            const unsigned e = (t - 64) + kNormLoaders * u;
+            buf[which][e / kNormC][e % kNormC] = v[u];
+        }
+    };
+    if (t >= 64)
+        load(0, 0);
+    __syncthreads();
+    float c = 0.0f;
+    unsigned cur = 0;
+    for (unsigned c0 = 0; c0 < n; c0 += kNormC, cur ^= 1) {
+        if (t >= 64 && c0 + kNormC < n)
+            load(c0 + kNormC, cur ^ 1);
+        if (t < kNormT) {
+            const unsigned cnt = min(kNormC, n - c0);
+            for (unsigned i = 0; i < cnt; i++)
+                c = __fadd_rn(c, buf[cur][t][i]);
+        }
+        __syncthreads();
+    }
+    const unsigned q = q0 + t;
+    if (t >= kNormT || q >= k)
+        return;
+    if (act[q] >= m || obs[q] >= z) {
+        status[q] = NOVA_ERROR_INVALID_DATA;
+        return;
+    }
+    norms[q] = c;
+    // a normaliser below the tolerance leaves b' unnormalised (pomdp_model_cpu.cpp:125-130)
+    status[q] = ((double)fabsf(c) < NOVA_FLT_ERR_TOL) ? NOVA_WARNING_INVALID_BELIEF
+                                                      : NOVA_SUCCESS;
+}
+
+__global__ __launch_bounds__(256) void belief_divide(unsigned n, const float *__restrict__ norms,
+        const int *__restrict__ status, float *__restrict__ Bp)
+{
+    const unsigned q = blockIdx.x;
+    if (status[q] != NOVA_SUCCESS)
+        return;
+    const float c = norms[q];
+    float *out = Bp + (size_t)q * n;
+    for (unsigned sp = threadIdx.x; sp < n; sp += 256)
+        out[sp] = __fdiv_rn(out[sp], c);
+}
+
 __global__ __launch_bounds__(256) void observation_major(unsigned n, unsigned z, size_t count,
         const float *__restrict__ O, float *__restrict__ Ot)
 {
@@ -263,6 +328,7 @@ struct DenseShape {
 } g_valueShape;
 
 std::mutex g_updateMutex;
+Arena g_updateArena;                 // the normalisers of a batch of belief updates
 // predecessor lists of the last model seen, keyed by its device arrays
 struct PredLists {
     const void *dS = nullptr, *dT = nullptr, *dO = nullptr;
@@ -398,16 +464,23 @@ int belief_update_rows(const nova::POMDP *pomdp, unsigned k, const float *d_B,
     int rc = pred_lists(pomdp, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
+    void *mem = nullptr;
+    rc = arena_get(g_updateArena, sizeof(float) * (size_t)k, fn, mem);
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    float *norms = static_cast<float *>(mem);
     hipStream_t st = stream();
-    const int useLds = pomdp->n <= (unsigned)kUpdLdsFloats ? 1 : 0;
-    const size_t lds = useLds ? sizeof(float) * pomdp->n : 0;
-    if (useLds)
-        hipFuncSetAttribute((const void *)belief_update_batch,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            kUpdLdsFloats * (int)sizeof(float));
-    hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, pomdp->n, pomdp->m,
-                       pomdp->z, k, g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B,
-                       d_rowOf, d_action, d_observation, d_Bp, d_status, useLds);
+    const unsigned n = pomdp->n;
+    const int useLds = n <= (unsigned)kUpdLdsFloats ? 1 : 0;
+    const size_t lds = sizeof(float) * ((useLds ? n : 0) + (size_t)kUpdWaves * kUpdScratch);
+    hipFuncSetAttribute((const void *)belief_update_batch,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, n, pomdp->m, pomdp->z,
+                       g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B, d_rowOf, d_action,
+                       d_observation, d_Bp, useLds);
+    hipLaunchKernelGGL(belief_norms, dim3((k + kNormT - 1) / kNormT), dim3(64 + kNormLoaders), 0, st, n,
+                       pomdp->m, pomdp->z, k, d_action, d_observation, d_Bp, norms, d_status);
+    hipLaunchKernelGGL(belief_divide, dim3(k), dim3(256), 0, st, n, norms, d_status, d_Bp);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipStreamSynchronize(st);

@@ -189,3 +189,27 @@ def test_bf16_blocked_kernel_equals_legacy(monkeypatch, k, n, r):
     assert np.all(np.abs(V - C[np.arange(k), j]) <= tol)
     assert np.all(C.max(axis=1) - C[np.arange(k), j] <= 2 * tol)
     print("bf16 blocked == legacy: k=%d n=%d r=%d" % (k, n, r), flush=True)
+
+
+def test_belief_update_batch_long_lists_and_global_belief(oracle):
+    """n = 30,000 (past the LDS copy of b: b is read from global memory) and successors skewed
+    onto states 0-63, whose block then holds more predecessor entries than a wave's product
+    scratch (several sub-ranges per block): bit for bit against the oracle's float twin."""
+    n, m, z, ns, k = 30000, 2, 3, 4, 12
+    P = synth.pomdp_arrays(n=n, m=m, z=z, ns=ns, r=2, rz=2, seed=77)
+    P["S"] = P["S"].copy()
+    rng = np.random.default_rng(77)
+    hot = rng.random((n, m)) < 0.2                  # about 24,000 entries onto states 0-63
+    P["S"][:, :, 0] = np.where(hot, rng.integers(0, 64, (n, m)), P["S"][:, :, 0])
+    B = rng.exponential(size=(k, n)).astype(np.float32)
+    B = (B / B.sum(axis=1, keepdims=True)).astype(np.float32)
+    a = rng.integers(0, m, k).astype(np.uint32)
+    o = rng.integers(0, z, k).astype(np.uint32)
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    Bp, status = p.belief_update_batch(B, a, o)
+    Bp, status = Bp.cpu().numpy(), status.cpu().numpy()
+    for i in range(k):
+        rc, bp = oracle.belief_update(P, B[i], int(a[i]), int(o[i]))
+        assert status[i] == rc and np.array_equal(bits(Bp[i]), bits(bp)), i
+    print("belief update batch: n=30000, skewed lists, bit-identical", flush=True)

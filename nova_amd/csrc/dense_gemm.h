@@ -282,17 +282,30 @@ __global__ __launch_bounds__(kGThreads) void dense_score(unsigned rows, unsigned
 }
 
 // Float rows (count / n rows of n) -> bf16 k-blocked [kpad / 32][rowsPad][32]: rows >= count / n
-// and columns >= n are written as zeros, so no earlier clearing is needed.
+// and columns >= n are written as zeros, so no earlier clearing is needed. Each lane forms one
+// 16-byte chunk: 8 consecutive k of one row.
 __global__ __launch_bounds__(kGConvThreads) void dense_convert_blk(unsigned n, unsigned kpad,
         size_t count, size_t rowsPad, const float *__restrict__ G, __bf16 *__restrict__ GB)
 {
-    const size_t rowsValid = count / n, total = rowsPad * kpad;
-    for (size_t x = blockIdx.x * (size_t)kGConvThreads + threadIdx.x; x < total;
-         x += (size_t)gridDim.x * kGConvThreads) {
-        // x enumerates the output in order: block kb, row, k within the block
-        const size_t kb = x / (rowsPad * kGBlk), rem = x % (rowsPad * kGBlk);
-        const size_t row = rem / kGBlk, k = kb * kGBlk + rem % kGBlk;
-        GB[x] = (row < rowsValid && k < n) ? (__bf16)G[row * n + k] : (__bf16)0.0f;
+    const size_t rowsValid = count / n, chunks = rowsPad * kpad / 8;
+    for (size_t c = blockIdx.x * (size_t)kGConvThreads + threadIdx.x; c < chunks;
+         c += (size_t)gridDim.x * kGConvThreads) {
+        // chunk c: elements 8c .. 8c + 7 in order of block kb, row, k within the block
+        const size_t kb = c / (rowsPad * 4), rem = c % (rowsPad * 4);
+        const size_t row = rem / 4, k0 = kb * kGBlk + (rem % 4) * 8;
+        bf16x8_t o;
+        if (row < rowsValid && k0 + 8 <= n) {
+            const float *src = G + row * n + k0;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                o[e] = (__bf16)src[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                o[e] = (row < rowsValid && k0 + e < n) ? (__bf16)G[row * n + k0 + e]
+                                                       : (__bf16)0.0f;
+        }
+        *reinterpret_cast<bf16x8_t *>(GB + 8 * c) = o;
     }
 }
 

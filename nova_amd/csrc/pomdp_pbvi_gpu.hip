@@ -1234,8 +1234,8 @@ static void dense_launch_score(const DenseWs &D, const float *G, hipStream_t st,
         // Gamma once, then per chunk of (a, o) pairs B_p = bf16(Gamma * gamma O[a,.,o]) and
         // the GEMM of the chunk's pairs against U_a
         const size_t rowsPad = D.rpad, per = rowsPad * D.kpad;
-        const unsigned cb = (unsigned)std::min<size_t>((per + kGConvThreads - 1) / kGConvThreads,
-                                                       65536);
+        const unsigned cb = (unsigned)std::min<size_t>(
+            (per / 8 + kGConvThreads - 1) / kGConvThreads, 65536);
         hipLaunchKernelGGL(dense_convert_blk, dim3(cb), dim3(kGConvThreads), 0, st, D.n, D.kpad,
                            rn, rowsPad, G, static_cast<__bf16 *>(D.GI));
         if (ev)

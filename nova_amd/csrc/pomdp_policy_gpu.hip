@@ -562,10 +562,10 @@ extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsig
         if (blocked) {
             // k-blocked operands, every padding element written by the conversion
             const size_t ta = (size_t)rowTiles * kGT * kpad, tg = (size_t)colTiles * kGT * kpad;
-            const unsigned ca = (unsigned)std::min<size_t>((ta + kGConvThreads - 1) / kGConvThreads,
-                                                           65536);
-            const unsigned cg = (unsigned)std::min<size_t>((tg + kGConvThreads - 1) / kGConvThreads,
-                                                           65536);
+            const unsigned ca = (unsigned)std::min<size_t>(
+                (ta / 8 + kGConvThreads - 1) / kGConvThreads, 65536);
+            const unsigned cg = (unsigned)std::min<size_t>(
+                (tg / 8 + kGConvThreads - 1) / kGConvThreads, 65536);
             hipFuncSetAttribute((const void *)dense_score_bf16,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGDlLdsBytes);
             hipLaunchKernelGGL(dense_convert_blk, dim3(ca), dim3(kGConvThreads), 0, st, n, kpad,

@@ -385,13 +385,14 @@ def bench_domains(args):
     """PBVI on nova's shipped domains as its parallel benchmark runs them
     (tests/benchmarks/parallel/parallel.py:64-127): the file's model, 2^k - 1 beliefs added by
     random expansion (here the seeded GPU expansion), 50 iterations from Gamma0 = Rmin/(1-gamma).
-    GPU: the fast-mode solve (wall clock, after one warm-up solve). CPU: the oracle port's
+    GPU: the fast-mode solve (wall clock, after one warm-up solve), and the exact-mode solve
+    (nova's rounding, bit for bit) where the supports fit it. CPU: the oracle port's
     backup of a leading sample of beliefs against the whole initial alpha set on one pinned
     core (per-backup work does not depend on the belief), as backups/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     from nova_amd.pomdp import POMDP
-    from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, set_mode
+    from nova_amd.pomdp_pbvi import EXACT, FAST, POMDPPBVIGPU, set_mode
     o = Oracle()
     set_mode(FAST)
     horizon = 50
@@ -417,11 +418,30 @@ def bench_domains(args):
             s.solve()
             times.append(time.perf_counter() - t0)
             del s
+        # exact mode (nova's rounding, bit for bit) where the supports fit its LDS staging
+        rz, ns = P["Z"].shape[1], P["S"].shape[2]
+        exact_ms = None
+        if rz <= 512 and rz * ns <= 512:
+            set_mode(EXACT)
+            try:
+                for _ in range(2):
+                    s = POMDPPBVIGPU(q, GammaInitial=P["Gamma0"])
+                    t0 = time.perf_counter()
+                    s.solve()
+                    exact_ms = (time.perf_counter() - t0) * 1e3
+                    del s
+            finally:
+                set_mode(FAST)
         del q
         gpu_bps = r * horizon / times[-1]
-        entry = {"n": n, "m": P["S"].shape[1], "z": P["O"].shape[2], "ns": P["S"].shape[2],
-                 "r": r, "rz": P["Z"].shape[1], "gpu_solve_ms": times[-1] * 1e3,
-                 "gpu_backups_per_s": gpu_bps, "scoring_variant": variant}
+        entry = {"n": n, "m": P["S"].shape[1], "z": P["O"].shape[2], "ns": ns,
+                 "r": r, "rz": rz, "gpu_solve_ms": times[-1] * 1e3,
+                 "gpu_backups_per_s": gpu_bps, "scoring_variant": variant,
+                 "gpu_exact_solve_ms": exact_ms,
+                 "gpu_exact_backups_per_s": (r * horizon / (exact_ms * 1e-3)) if exact_ms
+                 else None}
+        if exact_ms is None:
+            entry["gpu_exact_note"] = "supports too long for exact mode (rz*ns > 512)"
         if not args.no_cpu:
             old = os.sched_getaffinity(0)
             os.sched_setaffinity(0, {min(old)})
@@ -438,6 +458,8 @@ def bench_domains(args):
             entry["cpu_baseline"] = {"value": kc / tc, "unit": "backups/s", "cores": 1,
                                      "kind": "port", "sample": "%d of %d beliefs" % (kc, r)}
             entry["gpu_over_cpu"] = gpu_bps / (kc / tc)
+            if exact_ms:
+                entry["gpu_exact_over_cpu"] = entry["gpu_exact_backups_per_s"] / (kc / tc)
         out["domains"][name] = entry
         print("domain %s done" % name, file=sys.stderr, flush=True)
         if name in PEMA_DOMAINS:

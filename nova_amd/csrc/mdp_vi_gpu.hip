@@ -587,15 +587,20 @@ static int vi_run_sweeps(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned swee
             g_graph.exec = nullptr;
         }
         hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        hipStream_t cs = capture_stream();
+        if (cs == nullptr) {
+            NOVA_ERR(fn, "Could not create the graph-capture stream.");
+            return NOVA_ERROR_KERNEL_EXECUTION;
+        }
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess)
             return hip_fail(e, fn, "Failed to begin graph capture.", NOVA_ERROR_KERNEL_EXECUTION);
         for (unsigned k = 0; k < sweeps; k++) {
             const bool even = ((parity + k) % 2) == 0;
             vi_launch(plan, mdp, even ? vi->d_V : vi->d_VPrime, even ? vi->d_VPrime : vi->d_V,
-                      vi->d_pi, k + 1 == sweeps, st);
+                      vi->d_pi, k + 1 == sweeps, cs);
         }
-        e = hipStreamEndCapture(st, &graph);
+        e = hipStreamEndCapture(cs, &graph);
         if (e != hipSuccess)
             return hip_fail(e, fn, "Failed to capture the sweeps.", NOVA_ERROR_KERNEL_EXECUTION);
         e = hipGraphInstantiate(&g_graph.exec, graph, nullptr, nullptr, 0);

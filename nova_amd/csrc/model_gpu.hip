@@ -28,10 +28,13 @@ void model_changed(bool transitions)
 }
 
 // transitions: the array is S, T or O
+// Uploads and releases take the library's lock: their synchronous copies run on the legacy null
+// stream and must not overlap a graph capture by another thread's library call.
 template <typename T>
 static int upload(T *&dst, const T *src, size_t count, const char *fn, const char *what,
                   bool transitions)
 {
+    NOVA_AMD_API_LOCK();
     model_changed(transitions);
     device_free_if_owned(dst, fn, what);
     dst = nullptr;
@@ -54,6 +57,7 @@ static int upload(T *&dst, const T *src, size_t count, const char *fn, const cha
 template <typename T> static int release_device(T *&p, const char *fn, const char *what,
                                                 bool transitions)
 {
+    NOVA_AMD_API_LOCK();
     model_changed(transitions);
     int rc = device_free_if_owned(p, fn, what);
     p = nullptr;

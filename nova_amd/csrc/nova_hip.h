@@ -16,6 +16,12 @@ namespace nova_amd {
 // return: it synchronises this stream before returning.
 hipStream_t stream();
 
+// A non-blocking stream of the current device on which the library only captures graphs (it
+// never executes work): a capture there cannot be invalidated by other threads' work on the
+// legacy null stream, with which the library stream synchronises. The captured graphs are
+// launched on stream().
+hipStream_t capture_stream();
+
 // Serialises the library's GPU entry points, which share per-process state: the library
 // stream, cached layouts, graphs, workspaces and solver records. Recursive, because entry
 // points call one another (execute runs initialize, update and get-policy).

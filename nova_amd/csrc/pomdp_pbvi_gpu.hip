@@ -1312,17 +1312,23 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
                        flip);
 }
 
-// Captures `iters` iterations of a plan into an executable graph.
+// Captures `iters` iterations of a plan into an executable graph (on the capture stream; the
+// caller launches it on the library stream).
 static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mode,
-                      hipStream_t st, hipGraphExec_t *exec, const char *fn)
+                      hipStream_t, hipGraphExec_t *exec, const char *fn)
 {
     hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    hipStream_t cs = capture_stream();
+    if (cs == nullptr) {
+        NOVA_ERR(fn, "Could not create the graph-capture stream.");
+        return NOVA_ERROR_KERNEL_EXECUTION;
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess)
         return hip_fail(e, fn, "Failed to begin graph capture.", NOVA_ERROR_KERNEL_EXECUTION);
     for (unsigned k = 0; k < iters; k++)
-        plan_enqueue(pl, k, flip, mode, st);
-    e = hipStreamEndCapture(st, &graph);
+        plan_enqueue(pl, k, flip, mode, cs);
+    e = hipStreamEndCapture(cs, &graph);
     if (e != hipSuccess)
         return hip_fail(e, fn, "Failed to capture the PBVI iterations.", NOVA_ERROR_KERNEL_EXECUTION);
     e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);

@@ -497,6 +497,7 @@ extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsig
         const float *d_Gamma, const unsigned int *d_pi, unsigned int k, const float *d_B,
         int precision, float *d_V, unsigned int *d_action, int *d_alpha)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "nova_amd_alpha_vectors_value_and_action_gpu";
     if (n == 0 || r == 0 || k == 0 || d_Gamma == nullptr || d_pi == nullptr ||
             d_B == nullptr || d_V == nullptr || d_action == nullptr || precision < 0 ||
@@ -613,6 +614,7 @@ extern "C" int nova_amd_belief_update_batch_gpu(const nova::POMDP *pomdp, unsign
         const float *d_B, const unsigned int *d_action, const unsigned int *d_observation,
         float *d_Bp, int *d_status)
 {
+    NOVA_AMD_API_LOCK();
     const char *fn = "nova_amd_belief_update_batch_gpu";
     if (pomdp == nullptr || pomdp->n == 0 || pomdp->ns == 0 || pomdp->m == 0 ||
             pomdp->z == 0 || pomdp->S == nullptr || pomdp->T == nullptr ||

@@ -28,6 +28,22 @@ hipStream_t stream()
     return g_stream[dev];
 }
 
+hipStream_t capture_stream()
+{
+    static hipStream_t cs[kMaxDevices];
+    static std::once_flag once[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return nullptr;
+    std::call_once(once[dev], [dev]() {
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+            s = nullptr;
+        cs[dev] = s;
+    });
+    return cs[dev];
+}
+
 std::recursive_mutex &api_mutex()
 {
     static std::recursive_mutex mu;

@@ -1,7 +1,8 @@
 """Concurrent calls from several host threads (ctypes releases the GIL during each call): the
 library's GPU entry points share per-process caches and one stream per device and serialise
-on a process-wide lock, so each thread's VI and PBVI solves equal the same solves run one
-after another, bit for bit."""
+on a process-wide lock, and graphs are captured on a stream of their own, which model uploads
+from other threads (synchronous copies on the legacy null stream) cannot invalidate. Each
+thread's VI and PBVI solves equal the same solves run one after another, bit for bit."""
 import threading
 
 import numpy as np
@@ -45,7 +46,7 @@ def test_concurrent_solves_equal_sequential():
 
     def work(k):
         try:
-            for _ in range(2):
+            for _ in range(4):
                 got[("vi", k)] = vi_solve(k)
                 got[("pbvi", k)] = pbvi_solve(k)
         except Exception as e:          # reported by the main thread
@@ -61,4 +62,4 @@ def test_concurrent_solves_equal_sequential():
     for key, (a, b) in want.items():
         ga, gb = got[key]
         assert np.array_equal(bits(ga), bits(a)) and np.array_equal(gb, b), key
-    print("4 threads x (VI + PBVI) x 2: equal to the sequential solves", flush=True)
+    print("4 threads x (VI + PBVI) x 4: equal to the sequential solves", flush=True)

@@ -381,6 +381,70 @@ DOMAINS = [("tiger_95", 4), ("4x3_95", 5), ("shuttle_95", 4), ("paint_95", 4),
            ("tag", 8)]
 
 
+def bench_autograd(args):
+    """The differentiable paths (no counterpart in the reference) on the workloads of their
+    tests. Differentiable VI on vi_diff_n2e5 (n = 200,000, m = 8, ns = 16, gamma = 0.99): the
+    residual-stopped solve to 1e-6 (FP32 sweeps while the residual halves, then FP64), then the
+    adjoint (I - gamma P_pi)^T lambda = rho with dL/dR and dL/dT, each timed after a warm-up;
+    CPU: the oracle's FP64 sweep on one pinned core over 10 sweeps of the same instance. PBVI
+    argmax replay on pbvi_diff_n5000 (n = 5,000, r = 1,024, 30 iterations): the recorded
+    forward run and the backward replay of dL/dR."""
+    import torch
+    from nova_amd import synth
+    from nova_amd.autograd import adjoint, pbvi_objective, pbvi_replay_grad, solve
+    out = {}
+    w = synth.workload("vi_diff_n2e5")
+    dev = torch.device("cuda")
+    R, T, S = (torch.from_numpy(w[k]).to(dev) for k in ("R", "T", "S"))
+    g = float(w["gamma"])
+    n, m, ns = w["S"].shape
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = f()
+        torch.cuda.synchronize()
+        return res, time.perf_counter() - t0
+
+    (V64, pi, sweeps, res), ts = timed(lambda: solve(R, T, S, g))
+    rho = torch.full((n,), 1.0 / n, device=dev)
+    (lam, dR, dT, it, r64), ta = timed(lambda: adjoint(R, T, S, g, pi, V64, rho))
+    entry = {"workload": "vi_diff_n2e5 (n=200000, m=8, ns=16, gamma=0.99)",
+             "solve_ms": ts * 1e3, "solve_sweeps": int(sweeps), "solve_residual": float(res),
+             "solve_state_updates_per_s": sweeps * n / ts,
+             "solve_algorithmic_GBs": vi_bytes_per_sweep(n, m, ns) * sweeps / ts / 1e9,
+             "adjoint_ms": ta * 1e3, "adjoint_iterations": int(it),
+             "adjoint_fp64_residual": float(r64)}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        o = Oracle()
+        V = np.zeros(n)
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            t0 = time.perf_counter()
+            for _ in range(10):
+                V, _ = o.vi_sweep(w["S"], w["T"], w["R"], g, V, dtype=np.float64)
+            tc = time.perf_counter() - t0
+        finally:
+            os.sched_setaffinity(0, old)
+        entry["cpu_baseline"] = {"value": 10 * n / tc, "unit": "state-updates/s (FP64 sweeps)",
+                                 "cores": 1, "kind": "port",
+                                 "sample": "10 FP64 sweeps of the same instance"}
+    out["diff_vi"] = entry
+    print("autograd VI leg done", file=sys.stderr, flush=True)
+    P = synth.workload("pbvi_diff_n5000")
+    res_, tf = timed(lambda: pbvi_objective(P, P["R"]))
+    _, tb = timed(lambda: pbvi_replay_grad(res_))
+    out["pbvi_replay"] = {"workload": "pbvi_diff_n5000 (n=5000, m=8, z=16, ns=8, r=1024, "
+                                      "rz=32, 30 iterations)",
+                          "forward_recorded_ms": tf * 1e3, "backward_replay_ms": tb * 1e3}
+    print("autograd PBVI leg done", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_domains(args):
     """PBVI on nova's shipped domains as its parallel benchmark runs them
     (tests/benchmarks/parallel/parallel.py:64-127): the file's model, 2^k - 1 beliefs added by
@@ -526,6 +590,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-policy", action="store_true", help="skip the policy-execution leg")
     ap.add_argument("--no-domains", action="store_true", help="skip nova's benchmark domains")
+    ap.add_argument("--no-autograd", action="store_true",
+                    help="skip the differentiable-VI and PBVI-replay leg")
     ap.add_argument("--policy-beliefs", type=int, default=16384)
     ap.add_argument("--policy-exact-beliefs", type=int, default=1024)
     args = ap.parse_args()
@@ -556,6 +622,8 @@ def main():
         extras["policy"] = bench_policy(args)
     if not args.no_domains:
         extras["domains"] = bench_domains(args)
+    if not args.no_autograd:
+        extras["autograd"] = bench_autograd(args)
     if extras:
         out["extras"] = extras
     print(json.dumps(out))

@@ -149,10 +149,10 @@ def test_oracle_sample_at_full_size(oracle):
 
 
 @pytest.mark.parametrize("n,m,ns,horizon,seed", [
-    (1000, 8, 16, 40, 31),      # 256 states per tile, four tiles
-    (777, 3, 12, 25, 32),       # a partial last tile
+    (1000, 8, 16, 40, 31),      # 128 states per tile, eight tiles
+    (2000, 3, 12, 25, 32),      # 455 states per tile: a partial fifth tile
     (2000, 1, 64, 10, 33),      # m = 1, wide rows
-    (300, 16, 200, 6, 34),      # m * ns = 3,200: ten states per tile
+    (300, 16, 200, 6, 34),      # m * ns = 3,200: five states per tile
 ])
 def test_sorted_tiles_bitwise_vs_oracle(oracle, monkeypatch, n, m, ns, horizon, seed):
     """The successor-sorted layout, forced at small sizes: same V and pi bit for bit, with

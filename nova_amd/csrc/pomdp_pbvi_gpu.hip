@@ -81,6 +81,7 @@ struct PbviInst {
     int *act;
     int *recJ;             // optional per-iteration record: recJ[(iter*r + i)*z + o] = j*
     int *recA;             // and recA[iter*r + i] = chosen action (-1: none)
+    int fused;             // each iteration is one pbvi_fused launch (small instances)
 };
 
 namespace {
@@ -833,6 +834,141 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
         P.pi[i] = (unsigned)a;
 }
 
+// ------------------------------------------------ small instances: one launch per iteration
+//
+// At nova's smallest domains an iteration's arithmetic takes far less time than launching the
+// four kernels above. For instances within the bounds below one workgroup per belief runs the
+// whole backup: the score of every (observation, alpha) pair, the first maximiser per
+// observation, the action's alpha over all n states and its value on the support, and the
+// winner's write-out. Every sum and product is rounded in nova's order (the exact mode's), so
+// the kernel serves both modes: the exact mode's results are bit for bit those of the four
+// kernels, and the fast mode's stay within float rounding of them, as that mode promises.
+constexpr int kFN = 2048;          // max states: the alpha and the best alpha in LDS
+constexpr int kFZ = 256;           // max observations
+constexpr int kFS = 6144;          // max observations x alphas: the scores in LDS
+constexpr int kFStage = 24576;     // max r*n + (z+1)*n*ns: Gamma, O*T and S of one action in LDS
+
+__global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
+{
+    __shared__ SupportLds L;
+    __shared__ int scratch[kKMax + 1];
+    __shared__ float sc[kFS];
+    __shared__ float alpha[kFN], bestAlpha[kFN];
+    __shared__ int js[kFZ], bestJs[kFZ];
+    __shared__ float actVal;
+    extern __shared__ float stage[];   // Gamma [r][n], then O*T [z][n][ns] and S [n][ns] of a
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r;
+    const float gamma = P.gamma;
+    const unsigned i = blockIdx.x - off[b], t = threadIdx.x;
+    float *G = stage;
+    float *ota = stage + (size_t)r * n;
+    int *Sa = reinterpret_cast<int *>(ota + (size_t)z * n * ns);
+    {
+        const float *__restrict__ Gsrc = src_gamma(P, iter, flip);
+        for (unsigned x = t; x < r * n; x += kPB)
+            G[x] = Gsrc[x];
+    }
+    float bestValue = kLow;            // the same in every lane
+    int bestA = -1;
+    for (unsigned a = 0; a < m; a++) {
+        for (unsigned x = t; x < z * n * ns; x += kPB)
+            ota[x] = P.OT[(size_t)a * z * n * ns + x];
+        for (unsigned x = t; x < n * ns; x += kPB)
+            Sa[x] = P.S[((size_t)(x / ns) * m + a) * ns + x % ns];
+        build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);   // ends with a barrier
+        const int kEnd = L.kEnd;
+        // score of alpha j under observation o: sum_k ((sum_l (O*T) * Gamma[j, sp]) * gamma) * b_k
+        for (unsigned x = t; x < z * r; x += kPB) {
+            const unsigned o = x / r, j = x % r;
+            const float *ot = ota + (size_t)o * n * ns;
+            const float *Gj = G + (size_t)j * n;
+            float value = 0.0f;
+            for (int k = 0; k < kEnd; k++) {
+                float vt = 0.0f;
+                for (int u = L.startK[k]; u < L.startK[k + 1]; u++)
+                    vt = __fadd_rn(vt, __fmul_rn(ot[L.otU[u]], Gj[L.spU[u]]));
+                value = __fadd_rn(value, __fmul_rn(__fmul_rn(vt, gamma), L.bK[k]));
+            }
+            sc[x] = value;
+        }
+        __syncthreads();
+        // first maximiser per observation by strict '>' from -1e10 (none: alpha 0)
+        for (unsigned o = t; o < z; o += kPB) {
+            float best = kLow;
+            int bj = 0;
+            for (unsigned j = 0; j < r; j++) {
+                if (sc[o * r + j] > best) {
+                    best = sc[o * r + j];
+                    bj = (int)j;
+                }
+            }
+            js[o] = bj;
+            P.jstar[((size_t)i * m + a) * z + o] = bj;
+        }
+        __syncthreads();
+        // alpha(s) = R[s,a] + sum_o (gamma * sum_l (O*T) * Gamma[j*_o, sp]), o ascending
+        for (unsigned s = t; s < n; s += kPB) {
+            const int *rw = Sa + (size_t)s * ns;
+            float al = P.R[(size_t)s * m + a];
+            for (unsigned o = 0; o < z; o++) {
+                const float *Gj = G + (size_t)js[o] * n;
+                const float *ot = ota + ((size_t)o * n + s) * ns;
+                float vt = 0.0f;
+                for (unsigned l = 0; l < ns; l++) {
+                    const int sp = rw[l];
+                    if (sp < 0)
+                        break;
+                    vt = __fadd_rn(vt, __fmul_rn(ot[l], Gj[sp]));
+                }
+                al = __fadd_rn(al, __fmul_rn(vt, gamma));
+            }
+            alpha[s] = al;
+        }
+        __syncthreads();
+        if (t == 0) {
+            float v = 0.0f;
+            for (int k = 0; k < kEnd; k++)
+                v = __fadd_rn(v, __fmul_rn(alpha[L.sK[k]], L.bK[k]));
+            P.aval[(size_t)i * m + a] = v;
+            actVal = v;
+        }
+        __syncthreads();
+        const float v = actVal;
+        if (v > bestValue) {
+            bestValue = v;
+            bestA = (int)a;
+            for (unsigned s = t; s < n; s += kPB)
+                bestAlpha[s] = alpha[s];
+            for (unsigned o = t; o < z; o += kPB)
+                bestJs[o] = js[o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        P.act[i] = bestA;
+        if (bestA >= 0)
+            P.pi[i] = (unsigned)bestA;
+        if (P.recJ != nullptr)
+            P.recA[(size_t)iter * r + i] = bestA;
+    }
+    if (P.recJ != nullptr)
+        for (unsigned o = t; o < z; o += kPB)
+            P.recJ[((size_t)iter * r + i) * z + o] = (bestA >= 0) ? bestJs[o] : -1;
+    if (bestA >= 0) {
+        // no action beat -1e10: Gamma'[i] and pi[i] keep their values
+        float *out = dst_gamma(P, iter, flip) + (size_t)i * n;
+        for (unsigned s = t; s < n; s += kPB)
+            out[s] = bestAlpha[s];
+    }
+}
+
 // ------------------------------------------------------------ dense beliefs (MFMA)
 //
 // With dense beliefs (support = all n states) the scoring of observation o under action a is
@@ -1094,6 +1230,31 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
     inst_ws(p, ws, I);
 }
 
+// Whether a PBVI instance runs its iterations as single pbvi_fused launches: sparse modes, the
+// kernel's LDS bounds and the exact mode's support staging, and at most 256 (fast mode) or
+// 1024 (exact mode) scoring multiply-adds per lane and iteration, m * ceil(z*r / 256) * rz*ns.
+// Measured on nova's domains, the one launch takes 14-21 us per iteration at tiger, paint and
+// shuttle (24-28 us with four kernels, fast mode) and 61 us at 4x3 (35 us fast, 83 us exact);
+// at aloha and tiger_grid it is 3-5x slower than fast mode's four kernels.
+// NOVA_AMD_PBVI_FUSED=0 turns it off; another value replaces the limit of both modes.
+static bool fused_fits(const PbviInst &I, int mode)
+{
+    if (mode != 0 && mode != 1)
+        return false;
+    double limit = (mode == 1) ? 1024.0 : 256.0;
+    if (const char *f = std::getenv("NOVA_AMD_PBVI_FUSED"))
+        limit = std::atof(f);
+    if (limit <= 0.0)
+        return false;
+    const size_t zr = (size_t)I.z * I.r;
+    const double perLane = (double)I.m * (double)((zr + kPB - 1) / kPB) * I.rz * I.ns;
+    const double stage = (double)I.r * I.n + (I.z + 1.0) * I.n * I.ns;
+    return I.rb == I.r && I.n <= (unsigned)kFN && I.z <= (unsigned)kFZ &&
+           (size_t)I.z * I.r <= (size_t)kFS && I.rz <= (unsigned)kKMax &&
+           (size_t)I.rz * I.ns <= (size_t)kUMax && stage <= (double)kFStage &&
+           perLane <= limit;
+}
+
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
 struct DenseWs;
 
@@ -1105,8 +1266,9 @@ struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
     unsigned count = 0;
     PbviInst *dInst = nullptr;       // device copy of the descriptors
-    unsigned *dOff = nullptr;        // [tr | sc(jpt=1) | sc(2) | sc(4) | be], count+1 each
-    unsigned totTr = 0, totSc[kVariants] = {}, totBe = 0;
+    unsigned *dOff = nullptr;        // [tr | sc(jpt=1) | ... | sc(32) | be | fu], count+1 each
+    unsigned totTr = 0, totSc[kVariants] = {}, totBe = 0, totFu = 0;
+    size_t fuLds = 0;                // dynamic LDS of pbvi_fused: the largest fused instance's
     size_t ldsBytes = 0;
     unsigned maxHorizon = 0;
     std::vector<PbviInst> host;
@@ -1142,15 +1304,17 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
 static size_t plan_bytes(unsigned count)
 {
     return align256(sizeof(PbviInst) * count) +
-           align256(sizeof(unsigned) * (kVariants + 2) * (count + 1));
+           align256(sizeof(unsigned) * (kVariants + 3) * (count + 1));
 }
 
 static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
 {
     const unsigned c = pl.count;
-    // [tr | score offsets of each variant | be], c + 1 entries each
-    std::vector<unsigned> off((kVariants + 2) * (size_t)(c + 1), 0);
+    // [tr | score offsets of each variant | be | fu], c + 1 entries each; a fused instance
+    // has workgroups in fu only, every other instance in the first four stages only
+    std::vector<unsigned> off((kVariants + 3) * (size_t)(c + 1), 0);
     unsigned *tr = &off[0], *be = &off[(kVariants + 1) * (c + 1)];
+    unsigned *fu = &off[(kVariants + 2) * (c + 1)];
     unsigned *scs[kVariants];
     for (int v = 0; v < kVariants; v++)
         scs[v] = &off[(1 + v) * (c + 1)];
@@ -1159,11 +1323,17 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
     for (unsigned b = 0; b < c; b++) {
         PbviInst &I = pl.host[b];
         I.tiles = tiles_for(I.r, I.jpt);
-        tr[b + 1] = tr[b] + ((I.n + 63) / 64) * (I.rS / 64);
+        const bool four = !I.fused;
+        tr[b + 1] = tr[b] + (four ? ((I.n + 63) / 64) * (I.rS / 64) : 0u);
         for (int v = 0; v < kVariants; v++)
-            scs[v][b + 1] = scs[v][b] + ((jpt_slot(I.jpt) == v) ? I.rb * I.m * I.tiles : 0u);
-        be[b + 1] = be[b] + I.rb;
-        if (I.useLds)
+            scs[v][b + 1] = scs[v][b] +
+                            ((four && jpt_slot(I.jpt) == v) ? I.rb * I.m * I.tiles : 0u);
+        be[b + 1] = be[b] + (four ? I.rb : 0u);
+        fu[b + 1] = fu[b] + (four ? 0u : I.rb);
+        if (!four)
+            pl.fuLds = std::max(pl.fuLds, sizeof(float) * ((size_t)I.r * I.n +
+                                                           (size_t)(I.z + 1) * I.n * I.ns));
+        if (I.useLds && four)
             pl.ldsBytes = std::max(pl.ldsBytes, sizeof(float) * (size_t)I.n);
         if (I.horizon != kNoHorizon)
             pl.maxHorizon = std::max(pl.maxHorizon, I.horizon);
@@ -1172,6 +1342,14 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
     for (int v = 0; v < kVariants; v++)
         pl.totSc[v] = scs[v][c];
     pl.totBe = be[c];
+    pl.totFu = fu[c];
+    hipError_t ea = hipSuccess;
+    if (pl.totFu > 0)
+        ea = hipFuncSetAttribute((const void *)pbvi_fused,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 4 * kFStage);
+    if (ea != hipSuccess)
+        return hip_fail(ea, fn, "Failed to set up the one-launch PBVI kernel.",
+                        NOVA_ERROR_KERNEL_EXECUTION);
     pl.dInst = static_cast<PbviInst *>(dst);
     pl.dOff = reinterpret_cast<unsigned *>(static_cast<char *>(dst) +
                                            align256(sizeof(PbviInst) * c));
@@ -1295,6 +1473,12 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     }
     const unsigned c = pl.count;
     const unsigned *tr = pl.dOff, *be = pl.dOff + (kVariants + 1) * (c + 1);
+    const unsigned *fu = pl.dOff + (kVariants + 2) * (c + 1);
+    if (pl.totFu > 0)
+        hipLaunchKernelGGL(pbvi_fused, dim3(pl.totFu), dim3(kPB), pl.fuLds, st, pl.dInst, fu, c,
+                           iter, flip);
+    if (pl.totBe == 0)
+        return;
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
     static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32};
     for (int v = 0; v < kVariants; v++) {
@@ -1399,6 +1583,7 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
         pl.count = 1;
         I.jpt = jpt;
         I.horizon = kNoHorizon;
+        I.fused = 0;                           // the tuner times the four-kernel path
         pl.host.assign(1, I);
         if (plan_upload(pl, planMem, st, "pbvi tuner") != NOVA_SUCCESS)
             continue;
@@ -1595,6 +1780,7 @@ struct PbviGraph {
     float gamma = 0.0f;
     int mode = -1;
     unsigned jpt = 0;
+    int fused = -1;
 };
 static PbviGraph g_pgraph;
 
@@ -1709,10 +1895,6 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
                            pb->d_GammaPrime, pb->d_pi, pb->d_alphaBA,
                            dense ? dense->mem : nullptr, nullptr};
     const unsigned dims[6] = {p->n, p->ns, p->m, p->z, p->r, p->rz};
-    const bool hit = g_pgraph.exec && g_pgraph.iters == iters && g_pgraph.flip == flip &&
-                     g_pgraph.gamma == p->gamma && g_pgraph.mode == mode &&
-                     g_pgraph.jpt == jpt && std::memcmp(g_pgraph.key, key, sizeof(key)) == 0 &&
-                     std::memcmp(g_pgraph.dims, dims, sizeof(dims)) == 0;
     // The graph's kernels read the descriptor from plan memory when they run, and other calls
     // (the tuner, profiling, recorded runs) reuse that memory: upload it on every run.
     PbviPlan pl;
@@ -1721,6 +1903,12 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
     fill_inst(p, pl.host[0], pb->d_Gamma, pb->d_GammaPrime, pb->d_pi,
               reinterpret_cast<char *>(pb->d_alphaBA), kNoHorizon);
     pl.host[0].jpt = jpt;
+    pl.host[0].fused = (dense == nullptr && fused_fits(pl.host[0], mode)) ? 1 : 0;
+    const bool hit = g_pgraph.exec && g_pgraph.iters == iters && g_pgraph.flip == flip &&
+                     g_pgraph.gamma == p->gamma && g_pgraph.mode == mode &&
+                     g_pgraph.jpt == jpt && g_pgraph.fused == pl.host[0].fused &&
+                     std::memcmp(g_pgraph.key, key, sizeof(key)) == 0 &&
+                     std::memcmp(g_pgraph.dims, dims, sizeof(dims)) == 0;
     pl.dense = dense;
     if (pl.dense)
         pl.host[0].tiles = pl.dense->tiles;
@@ -1739,6 +1927,7 @@ static int pbvi_run_iters(const nova::POMDP *p, nova::POMDPPBVIGPU *pb, unsigned
         std::memcpy(g_pgraph.dims, dims, sizeof(dims));
         g_pgraph.iters = iters;
         g_pgraph.flip = flip;
+        g_pgraph.fused = pl.host[0].fused;
         g_pgraph.gamma = p->gamma;
         g_pgraph.mode = mode;
         g_pgraph.jpt = jpt;
@@ -2037,6 +2226,8 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
     fill_inst(pomdp, pl.host[0], pbvi->d_Gamma, pbvi->d_GammaPrime, pbvi->d_pi,
               reinterpret_cast<char *>(pbvi->d_alphaBA), kNoHorizon);
     pl.host[0].jpt = S->jpt;
+    pl.host[0].fused = (dense_for(pbvi->d_alphaBA) == nullptr &&
+                        fused_fits(pl.host[0], S->mode)) ? 1 : 0;
     rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
@@ -2068,6 +2259,26 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
             hipEventDestroy(x);
         if (e != hipSuccess)
             return hip_fail(e, fn, "Failed to run the profiled iteration.", NOVA_ERROR_KERNEL_EXECUTION);
+        pbvi->currentHorizon++;
+        return NOVA_SUCCESS;
+    }
+    if (pl.totFu > 0) {
+        // a small instance: the whole iteration is one launch, reported as the score stage
+        hipEventRecord(ev[0], st);
+        hipEventRecord(ev[1], st);
+        hipLaunchKernelGGL(pbvi_fused, dim3(pl.totFu), dim3(kPB), pl.fuLds, st, pl.dInst,
+                           pl.dOff + (kVariants + 2) * (c + 1), c, 0u, flip);
+        hipEventRecord(ev[2], st);
+        hipEventRecord(ev[3], st);
+        hipEventRecord(ev[4], st);
+        hipError_t e = hipStreamSynchronize(st);
+        for (int k = 0; k < 4; k++)
+            hipEventElapsedTime(&stageMs[k], ev[k], ev[k + 1]);
+        for (auto &x : ev)
+            hipEventDestroy(x);
+        if (e != hipSuccess)
+            return hip_fail(e, fn, "Failed to run the profiled iteration.",
+                            NOVA_ERROR_KERNEL_EXECUTION);
         pbvi->currentHorizon++;
         return NOVA_SUCCESS;
     }
@@ -2123,6 +2334,7 @@ int pomdp_pbvi_run_recorded_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, unsigned
     pl.host[0].jpt = S->jpt;
     pl.host[0].recJ = d_recJ;
     pl.host[0].recA = d_recA;
+    pl.host[0].fused = fused_fits(pl.host[0], S->mode) ? 1 : 0;
     rc = plan_upload(pl, plain_plan_mem(pomdp, pbvi), st, fn);
     if (rc != NOVA_SUCCESS)
         return rc;
@@ -2177,6 +2389,7 @@ int pomdp_pbvi_batch_execute_gpu(const POMDP *const *pomdps, POMDPPBVIGPU *const
                       pbvis[b]->d_pi, reinterpret_cast<char *>(pbvis[b]->d_alphaBA),
                       pomdps[b]->horizon);
             pl.host[b].jpt = plain_jpt(pbvis[b]->d_alphaBA);
+            pl.host[b].fused = fused_fits(pl.host[b], g_pbvi_mode) ? 1 : 0;
         }
         rc = plan_upload(pl, planMem, st, fn);
     }

@@ -145,9 +145,11 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
     """From the same Gamma (the oracle's after t iterations), one fast GPU iteration chooses
     the oracle's alphas except at near-ties; for the tuned scoring kernel and for each forced
     one (1 and 4 alphas per lane on the vector ALUs; 8, 16, 32 = the matrix-core kernel with 4
-    waves, 8 waves, 8 waves and a deeper prefetch)."""
+    waves, 8 waves, 8 waves and a deeper prefetch). A forced variant also turns off the
+    one-launch path of small instances, which would otherwise run instead of it."""
     if variant != "tuned":
         monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
+        monkeypatch.setenv("NOVA_AMD_PBVI_FUSED", "0")
     P = synth.pomdp_arrays(**_params(params))
     if rag:
         P = ragged(P, params["seed"])
@@ -186,6 +188,7 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
 def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag, variant):
     if variant != "tuned":
         monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
+        monkeypatch.setenv("NOVA_AMD_PBVI_FUSED", "0")
     P = synth.pomdp_arrays(**_params(params))
     if rag:
         P = ragged(P, params["seed"])

@@ -846,7 +846,7 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
 constexpr int kFN = 2048;          // max states: the alpha and the best alpha in LDS
 constexpr int kFZ = 256;           // max observations
 constexpr int kFS = 6144;          // max observations x alphas: the scores in LDS
-constexpr int kFStage = 24576;     // max r*n + (z+1)*n*ns: Gamma, O*T and S of one action in LDS
+constexpr int kFStage = 24576;     // max words of Gamma, O*T, S, R and the belief in LDS
 
 __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
@@ -857,7 +857,7 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
     __shared__ float alpha[kFN], bestAlpha[kFN];
     __shared__ int js[kFZ], bestJs[kFZ];
     __shared__ float actVal;
-    extern __shared__ float stage[];   // Gamma [r][n], then O*T [z][n][ns] and S [n][ns] of a
+    extern __shared__ float stage[];   // Gamma, O*T, S, R and belief i: fused_stage_words
     if (blockIdx.x >= off[count])
         return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
@@ -867,22 +867,41 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
     const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r;
     const float gamma = P.gamma;
     const unsigned i = blockIdx.x - off[b], t = threadIdx.x;
+    // one round of loads stages everything the iteration reads: Gamma [r][n], O*T [m][z][n][ns],
+    // S [m][n][ns] (action-major), R [n][m] and belief i's (Z, B)
+    const unsigned rz = P.rz, zn = z * n * ns, sn = n * ns;
     float *G = stage;
-    float *ota = stage + (size_t)r * n;
-    int *Sa = reinterpret_cast<int *>(ota + (size_t)z * n * ns);
+    float *OTall = G + (size_t)r * n;
+    int *Sall = reinterpret_cast<int *>(OTall + (size_t)m * zn);
+    float *Rall = reinterpret_cast<float *>(Sall + (size_t)m * sn);
+    int *Zi = reinterpret_cast<int *>(Rall + (size_t)n * m);
+    float *Bi = reinterpret_cast<float *>(Zi + rz);
     {
         const float *__restrict__ Gsrc = src_gamma(P, iter, flip);
         for (unsigned x = t; x < r * n; x += kPB)
             G[x] = Gsrc[x];
+        for (unsigned x = t; x < m * zn; x += kPB)
+            OTall[x] = P.OT[x];
+        for (unsigned x = t; x < m * sn; x += kPB) {
+            const unsigned a = x / sn, e = x % sn;
+            Sall[x] = P.S[((size_t)(e / ns) * m + a) * ns + e % ns];
+        }
+        for (unsigned x = t; x < n * m; x += kPB)
+            Rall[x] = P.R[x];
+        for (unsigned k = t; k < rz; k += kPB) {
+            Zi[k] = P.Z[(size_t)i * rz + k];
+            Bi[k] = P.B[(size_t)i * rz + k];
+        }
     }
     float bestValue = kLow;            // the same in every lane
     int bestA = -1;
     for (unsigned a = 0; a < m; a++) {
-        for (unsigned x = t; x < z * n * ns; x += kPB)
-            ota[x] = P.OT[(size_t)a * z * n * ns + x];
-        for (unsigned x = t; x < n * ns; x += kPB)
-            Sa[x] = P.S[((size_t)(x / ns) * m + a) * ns + x % ns];
-        build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);   // ends with a barrier
+        const float *ota = OTall + (size_t)a * zn;
+        const int *Sa = Sall + (size_t)a * sn;
+        // the support's pairs from the staged rows (belief 0 of one action: i = 0, m = 1,
+        // a = 0); build_support reads its inputs only after its first barrier, which orders
+        // the staging above before them
+        build_support(L, 0, 0, 1, ns, rz, Sa, Zi, Bi, scratch);   // ends with a barrier
         const int kEnd = L.kEnd;
         // score of alpha j under observation o: sum_k ((sum_l (O*T) * Gamma[j, sp]) * gamma) * b_k
         for (unsigned x = t; x < z * r; x += kPB) {
@@ -916,7 +935,7 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
         // alpha(s) = R[s,a] + sum_o (gamma * sum_l (O*T) * Gamma[j*_o, sp]), o ascending
         for (unsigned s = t; s < n; s += kPB) {
             const int *rw = Sa + (size_t)s * ns;
-            float al = P.R[(size_t)s * m + a];
+            float al = Rall[(size_t)s * m + a];
             for (unsigned o = 0; o < z; o++) {
                 const float *Gj = G + (size_t)js[o] * n;
                 const float *ot = ota + ((size_t)o * n + s) * ns;
@@ -1230,6 +1249,13 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
     inst_ws(p, ws, I);
 }
 
+// Words of LDS that pbvi_fused stages for an instance: Gamma, O*T, S, R and one belief.
+static double fused_stage_words(const PbviInst &I)
+{
+    return (double)I.r * I.n + (double)I.m * (I.z + 1.0) * I.n * I.ns + (double)I.n * I.m +
+           2.0 * I.rz;
+}
+
 // Whether a PBVI instance runs its iterations as single pbvi_fused launches: sparse modes, the
 // kernel's LDS bounds and the exact mode's support staging, and at most 256 (fast mode) or
 // 1024 (exact mode) scoring multiply-adds per lane and iteration, m * ceil(z*r / 256) * rz*ns.
@@ -1248,7 +1274,7 @@ static bool fused_fits(const PbviInst &I, int mode)
         return false;
     const size_t zr = (size_t)I.z * I.r;
     const double perLane = (double)I.m * (double)((zr + kPB - 1) / kPB) * I.rz * I.ns;
-    const double stage = (double)I.r * I.n + (I.z + 1.0) * I.n * I.ns;
+    const double stage = fused_stage_words(I);
     return I.rb == I.r && I.n <= (unsigned)kFN && I.z <= (unsigned)kFZ &&
            (size_t)I.z * I.r <= (size_t)kFS && I.rz <= (unsigned)kKMax &&
            (size_t)I.rz * I.ns <= (size_t)kUMax && stage <= (double)kFStage &&
@@ -1331,8 +1357,7 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
         be[b + 1] = be[b] + (four ? I.rb : 0u);
         fu[b + 1] = fu[b] + (four ? 0u : I.rb);
         if (!four)
-            pl.fuLds = std::max(pl.fuLds, sizeof(float) * ((size_t)I.r * I.n +
-                                                           (size_t)(I.z + 1) * I.n * I.ns));
+            pl.fuLds = std::max(pl.fuLds, sizeof(float) * (size_t)fused_stage_words(I));
         if (I.useLds && four)
             pl.ldsBytes = std::max(pl.ldsBytes, sizeof(float) * (size_t)I.n);
         if (I.horizon != kNoHorizon)

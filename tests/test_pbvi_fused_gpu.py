@@ -37,7 +37,7 @@ def fused_fits(P):
     z = P["O"].shape[2]
     r, rz = P["Z"].shape
     return (n <= 2048 and z <= 256 and z * r <= 6144 and rz <= 512 and rz * ns <= 512
-            and r * n + (z + 1) * n * ns <= 24576)
+            and r * n + m * (z + 1) * n * ns + n * m + 2 * rz <= 24576)
 
 
 def ragged(P, seed):
@@ -61,8 +61,8 @@ CASES = {
     "tiny": dict(n=5, m=2, z=3, ns=5, r=7, rz=5, seed=32, gamma=0.9),
     "tiger_like": dict(n=2, m=3, z=2, ns=2, r=16, rz=2, seed=5, gamma=0.95),
     "one_alpha": dict(n=12, m=1, z=1, ns=3, r=1, rz=4, seed=6, gamma=0.9),
-    "mid": dict(n=100, m=3, z=20, ns=6, r=60, rz=12, seed=31, gamma=0.95),
-    "n_above_block": dict(n=600, m=2, z=4, ns=3, r=20, rz=8, seed=7, gamma=0.9),
+    "mid": dict(n=60, m=3, z=12, ns=6, r=60, rz=12, seed=31, gamma=0.95),
+    "n_above_block": dict(n=300, m=2, z=4, ns=3, r=20, rz=8, seed=7, gamma=0.9),
 }
 
 

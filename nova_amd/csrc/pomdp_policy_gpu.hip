@@ -166,10 +166,14 @@ __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned 
         const unsigned *__restrict__ pstart, const int *__restrict__ pS,
         const float *__restrict__ pT, const float *__restrict__ Ot, const float *__restrict__ B,
         const unsigned *__restrict__ rowOf, const unsigned *__restrict__ act,
-        const unsigned *__restrict__ obs, float *__restrict__ Bp, int useLds)
+        const unsigned *__restrict__ obs, float *__restrict__ Bp, int useLds,
+        const unsigned *__restrict__ order)
 {
     extern __shared__ float smem[];
-    const unsigned q = blockIdx.x, t = threadIdx.x, lane = t % 64, w = t / 64;
+    // order: the triples grouped by action, so that the workgroups running together on an XCD
+    // stream the same action's predecessor list and find it in their L2
+    const unsigned q = order ? order[blockIdx.x] : blockIdx.x;
+    const unsigned t = threadIdx.x, lane = t % 64, w = t / 64;
     const unsigned a = act[q], o = obs[q];
     if (a >= m || o >= z)
         return;                                  // belief_norms reports it
@@ -208,6 +212,35 @@ __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned 
         if (sp < n)
             out[sp] = __fmul_rn(u, Oao[sp]);
     }
+}
+
+// The k triples grouped by action (a counting sort in one workgroup; invalid actions last).
+// The order within an action is whatever the atomics give: each triple's result does not
+// depend on when it runs.
+constexpr unsigned kOrderMaxM = 4096;
+
+__global__ __launch_bounds__(1024) void order_by_action(unsigned k, unsigned m,
+        const unsigned *__restrict__ act, unsigned *__restrict__ order)
+{
+    __shared__ unsigned cnt[kOrderMaxM + 1];
+    const unsigned t = threadIdx.x;
+    for (unsigned x = t; x <= m; x += 1024)
+        cnt[x] = 0;
+    __syncthreads();
+    for (unsigned q = t; q < k; q += 1024)
+        atomicAdd(&cnt[min(act[q], m)], 1u);
+    __syncthreads();
+    if (t == 0) {
+        unsigned run = 0;
+        for (unsigned x = 0; x <= m; x++) {
+            const unsigned c = cnt[x];
+            cnt[x] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (unsigned q = t; q < k; q += 1024)
+        order[atomicAdd(&cnt[min(act[q], m)], 1u)] = q;
 }
 
 constexpr unsigned kNormT = 16;          // triples per workgroup of belief_norms
@@ -328,7 +361,7 @@ struct DenseShape {
 } g_valueShape;
 
 std::mutex g_updateMutex;
-Arena g_updateArena;                 // the normalisers of a batch of belief updates
+Arena g_updateArena;                 // the normalisers and action order of a batch of belief updates
 // predecessor lists of the last model seen, keyed by its device arrays
 struct PredLists {
     const void *dS = nullptr, *dT = nullptr, *dO = nullptr;
@@ -465,19 +498,26 @@ int belief_update_rows(const nova::POMDP *pomdp, unsigned k, const float *d_B,
     if (rc != NOVA_SUCCESS)
         return rc;
     void *mem = nullptr;
-    rc = arena_get(g_updateArena, sizeof(float) * (size_t)k, fn, mem);
+    const size_t normBytes = (sizeof(float) * (size_t)k + 255) / 256 * 256;
+    rc = arena_get(g_updateArena, normBytes + sizeof(unsigned) * (size_t)k, fn, mem);
     if (rc != NOVA_SUCCESS)
         return rc;
     float *norms = static_cast<float *>(mem);
+    unsigned *order = pomdp->m <= kOrderMaxM
+                      ? reinterpret_cast<unsigned *>(static_cast<char *>(mem) + normBytes)
+                      : nullptr;
     hipStream_t st = stream();
     const unsigned n = pomdp->n;
+    if (order)
+        hipLaunchKernelGGL(order_by_action, dim3(1), dim3(1024), 0, st, k, pomdp->m, d_action,
+                           order);
     const int useLds = n <= (unsigned)kUpdLdsFloats ? 1 : 0;
     const size_t lds = sizeof(float) * ((useLds ? n : 0) + (size_t)kUpdWaves * kUpdScratch);
     hipFuncSetAttribute((const void *)belief_update_batch,
                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, n, pomdp->m, pomdp->z,
                        g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B, d_rowOf, d_action,
-                       d_observation, d_Bp, useLds);
+                       d_observation, d_Bp, useLds, order);
     hipLaunchKernelGGL(belief_norms, dim3((k + kNormT - 1) / kNormT), dim3(64 + kNormLoaders), 0, st, n,
                        pomdp->m, pomdp->z, k, d_action, d_observation, d_Bp, norms, d_status);
     hipLaunchKernelGGL(belief_divide, dim3(k), dim3(256), 0, st, n, norms, d_status, d_Bp);

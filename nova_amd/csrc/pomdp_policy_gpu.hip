@@ -162,8 +162,10 @@ constexpr int kUpdWaves = kUB / 64;
 constexpr int kUpdScratch = 1024;                // product slots per wave
 constexpr int kUpdLdsFloats = 40 * 1024 - kUpdWaves * kUpdScratch;   // 160 KB in all
 
+// IDX: the predecessor's index type, 16 bits for n <= 65,536 (6 bytes per list entry, not 8)
+template <typename IDX>
 __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned m, unsigned z,
-        const unsigned *__restrict__ pstart, const int *__restrict__ pS,
+        const unsigned *__restrict__ pstart, const IDX *__restrict__ pS,
         const float *__restrict__ pT, const float *__restrict__ Ot, const float *__restrict__ B,
         const unsigned *__restrict__ rowOf, const unsigned *__restrict__ act,
         const unsigned *__restrict__ obs, float *__restrict__ Bp, int useLds,
@@ -370,7 +372,8 @@ struct PredLists {
     void *mem = nullptr;
     size_t bytes = 0;
     unsigned *pstart = nullptr;
-    int *pS = nullptr;
+    int *pS = nullptr;                  // predecessors when n > 65,536
+    unsigned short *pS16 = nullptr;     // predecessors when n <= 65,536
     float *pT = nullptr;
     float *Ot = nullptr;      // O observation-major: Ot[a][o][s'] = O[a][s'][o]
 } g_pred;
@@ -426,7 +429,12 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
         device_free(g_pred.mem, fn, "predecessor lists");
         g_pred = PredLists();
     }
-    const size_t b0 = up256(sizeof(unsigned) * pstart.size()), b1 = up256(sizeof(int) * pS.size());
+    const bool narrow = n <= 65536;
+    std::vector<unsigned short> pS16(narrow ? pS.size() : 0);
+    for (size_t x = 0; x < pS16.size(); x++)
+        pS16[x] = (unsigned short)pS[x];
+    const size_t idxBytes = narrow ? sizeof(unsigned short) : sizeof(int);
+    const size_t b0 = up256(sizeof(unsigned) * pstart.size()), b1 = up256(idxBytes * pS.size());
     const size_t b2 = up256(sizeof(float) * pT.size());
     const size_t otCount = (size_t)m * p->z * n;
     const size_t bytes = b0 + b1 + b2 + up256(sizeof(float) * otCount);
@@ -438,7 +446,8 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
     hipError_t e = hipMemcpy(c, pstart.data(), sizeof(unsigned) * pstart.size(),
                              hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(c + b0, pS.data(), sizeof(int) * pS.size(), hipMemcpyHostToDevice);
+        e = hipMemcpy(c + b0, narrow ? (const void *)pS16.data() : (const void *)pS.data(),
+                      idxBytes * pS.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c + b0 + b1, pT.data(), sizeof(float) * pT.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -470,7 +479,10 @@ static int pred_lists(const nova::POMDP *p, const char *fn)
     g_pred.mem = mem;
     g_pred.bytes = bytes;
     g_pred.pstart = reinterpret_cast<unsigned *>(c);
-    g_pred.pS = reinterpret_cast<int *>(c + b0);
+    if (narrow)
+        g_pred.pS16 = reinterpret_cast<unsigned short *>(c + b0);
+    else
+        g_pred.pS = reinterpret_cast<int *>(c + b0);
     g_pred.pT = reinterpret_cast<float *>(c + b0 + b1);
     g_pred.Ot = Ot;
     return NOVA_SUCCESS;
@@ -513,11 +525,19 @@ int belief_update_rows(const nova::POMDP *pomdp, unsigned k, const float *d_B,
                            order);
     const int useLds = n <= (unsigned)kUpdLdsFloats ? 1 : 0;
     const size_t lds = sizeof(float) * ((useLds ? n : 0) + (size_t)kUpdWaves * kUpdScratch);
-    hipFuncSetAttribute((const void *)belief_update_batch,
-                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL(belief_update_batch, dim3(k), dim3(kUB), lds, st, n, pomdp->m, pomdp->z,
-                       g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B, d_rowOf, d_action,
-                       d_observation, d_Bp, useLds, order);
+    if (g_pred.pS16 != nullptr) {
+        hipFuncSetAttribute((const void *)belief_update_batch<unsigned short>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipLaunchKernelGGL(belief_update_batch<unsigned short>, dim3(k), dim3(kUB), lds, st, n,
+                           pomdp->m, pomdp->z, g_pred.pstart, g_pred.pS16, g_pred.pT, g_pred.Ot,
+                           d_B, d_rowOf, d_action, d_observation, d_Bp, useLds, order);
+    } else {
+        hipFuncSetAttribute((const void *)belief_update_batch<int>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipLaunchKernelGGL(belief_update_batch<int>, dim3(k), dim3(kUB), lds, st, n, pomdp->m,
+                           pomdp->z, g_pred.pstart, g_pred.pS, g_pred.pT, g_pred.Ot, d_B,
+                           d_rowOf, d_action, d_observation, d_Bp, useLds, order);
+    }
     hipLaunchKernelGGL(belief_norms, dim3((k + kNormT - 1) / kNormT), dim3(64 + kNormLoaders), 0, st, n,
                        pomdp->m, pomdp->z, k, d_action, d_observation, d_Bp, norms, d_status);
     hipLaunchKernelGGL(belief_divide, dim3(k), dim3(256), 0, st, n, norms, d_status, d_Bp);

@@ -13,7 +13,8 @@ import ctypes as ct
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libnova.so")
+# NOVA_AMD_LIB names another build of the library (the host-sanitizer build of the tests)
+LIB_PATH = os.environ.get("NOVA_AMD_LIB") or os.path.join(HERE, "lib", "libnova.so")
 
 c_float_p = ct.POINTER(ct.c_float)
 c_int_p = ct.POINTER(ct.c_int)
@@ -211,6 +212,7 @@ EXTRA_SIGNATURES = {
     "nova_amd_pema_mode": [ct.c_int],
     "pomdp_expand_pema_gpu": [_POMDP, _PAV],
     "mdp_vi_sweep_kernel_gpu": [_MDP, _VIG, _P(ct.c_uint)],
+    "mdp_vi_batch_execute_gpu": [ct.c_void_p, ct.c_void_p, U, ct.c_void_p, _P(ct.c_float)],
     "mdp_vi_solve_gpu": [_MDP, _VIG, ct.c_void_p, ct.c_double, U, U, _P(ct.c_uint),
                          _P(ct.c_double)],
     "mdp_vi_adjoint_gpu": [_MDP, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
@@ -230,10 +232,11 @@ def load():
     # When it is loaded first, libnova.so's dependency on libamdhip64.so.7 binds to that same
     # runtime; loading libnova.so first would bring in /opt/rocm's copy and torch would then
     # start a second HIP runtime in the process, which finds no GPU. So torch goes first.
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    if not os.environ.get("NOVA_AMD_NO_TORCH"):
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     if not os.path.exists(LIB_PATH):
         raise ImportError("nova_amd: %s is missing; build it with `python -m nova_amd.build`"
                           % LIB_PATH)

@@ -73,9 +73,46 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+# Host sanitizers (AddressSanitizer, UndefinedBehaviorSanitizer) on the host translation
+# units: libnova_san.so links them, built with the sanitizers, to the device translation
+# units' objects of the plain build (device code is never sanitized). For tests of the *_cpu
+# entry points with the sanitizer runtime preloaded.
+SAN_FLAGS = ["-fsanitize=address", "-fsanitize=undefined", "-fno-sanitize-recover=undefined",
+             "-fno-omit-frame-pointer", "-g"]
+SAN_LIB = os.path.join(LIBDIR, "libnova_san.so")
+
+
+def build_sanitized(jobs=None):
+    cpp, hip = sources()
+    hipObjs = [os.path.join(BUILD, os.path.basename(src) + ".o") for src in hip]
+    if not all(os.path.exists(o) for o in hipObjs):
+        build(jobs=jobs)
+    sdir = os.path.join(BUILD, "san")
+    os.makedirs(sdir, exist_ok=True)
+    cmds, objs = [], []
+    for src in cpp:
+        obj = os.path.join(sdir, os.path.basename(src) + ".o")
+        cmds.append([CXX] + CXX_FLAGS + SAN_FLAGS + ["-c", src, "-o", obj])
+        objs.append(obj)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    host_san = []
+    for f in SAN_FLAGS[:3]:
+        host_san += ["-Xarch_host", f]
+    tmp = SAN_LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH] + host_san + ["-o", tmp] +
+         objs + hipObjs)
+    os.replace(tmp, SAN_LIB)
+    return SAN_LIB
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--sanitize" in sys.argv:
+        print(build_sanitized())
+    else:
+        print(build(verbose="-v" in sys.argv))

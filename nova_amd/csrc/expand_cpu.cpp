@@ -93,7 +93,7 @@ int pomdp_expand_random_cpu(POMDP *pomdp, unsigned int numBeliefsToAdd)
                 }
             }
             float *bp = nullptr;
-            pomdp_belief_update_cpu(pomdp, b.data(), a, o, bp);
+            nova_amd::belief_update_host(pomdp, b.data(), a, o, bp, true);
             std::memcpy(b.data(), bp, sizeof(float) * n);
             delete[] bp;
             std::memcpy(&Bnew[(size_t)added * n], b.data(), sizeof(float) * n);
@@ -126,7 +126,7 @@ int pomdp_expand_distinct_beliefs_cpu(POMDP *pomdp)
                 if (nova_amd::prob_obs(pomdp, b.data(), a, o) == 0.0f)
                     continue;
                 float *bp = nullptr;
-                const int rc = pomdp_belief_update_cpu(pomdp, b.data(), a, o, bp);
+                const int rc = nova_amd::belief_update_host(pomdp, b.data(), a, o, bp, true);
                 if (rc != NOVA_SUCCESS) {
                     delete[] bp;
                     continue;
@@ -263,7 +263,7 @@ int pomdp_expand_pema_cpu(POMDP *pomdp, const POMDPAlphaVectors *policy)
                     }
                 }
                 float *bp = nullptr;
-                if (pomdp_belief_update_cpu(pomdp, b, a, o, bp) != NOVA_SUCCESS) {
+                if (nova_amd::belief_update_host(pomdp, b, a, o, bp, true) != NOVA_SUCCESS) {
                     delete[] bp;
                     continue;
                 }
@@ -302,7 +302,7 @@ int pomdp_expand_pema_cpu(POMDP *pomdp, const POMDPAlphaVectors *policy)
         }
         if (aStarValue > bound) {
             float *bp = nullptr;
-            if (pomdp_belief_update_cpu(pomdp, b, aStar, oStar[aStar], bp) == NOVA_SUCCESS)
+            if (nova_amd::belief_update_host(pomdp, b, aStar, oStar[aStar], bp, true) == NOVA_SUCCESS)
                 std::memcpy(Bnew.data(), bp, sizeof(float) * n);
             else
                 std::memcpy(Bnew.data(), b, sizeof(float) * n);

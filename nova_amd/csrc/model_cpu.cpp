@@ -115,6 +115,19 @@ int pomdp_initialize_cpu(POMDP *pomdp, unsigned int n, unsigned int ns, unsigned
 int pomdp_belief_update_cpu(const POMDP *pomdp, const float *b, unsigned int a, unsigned int o,
         float *&bp)
 {
+    return nova_amd::belief_update_host(pomdp, b, a, o, bp, false);
+}
+
+} // namespace nova
+
+namespace nova_amd {
+
+// The body of pomdp_belief_update_cpu. quiet: the impossible-observation warning is returned
+// without its stderr line (the expansions call it for every (action, observation) and skip
+// those, so the line says nothing to their callers); results are the same either way.
+int belief_update_host(const nova::POMDP *pomdp, const float *b, unsigned int a, unsigned int o,
+        float *&bp, bool quiet)
+{
     if (pomdp == nullptr || pomdp->n == 0 || pomdp->ns == 0 || pomdp->m == 0 || pomdp->z == 0 ||
             pomdp->S == nullptr || pomdp->T == nullptr || pomdp->O == nullptr ||
             pomdp->R == nullptr || bp != nullptr) {
@@ -140,14 +153,19 @@ int pomdp_belief_update_cpu(const POMDP *pomdp, const float *b, unsigned int a, 
     }
     bp = out;
     if (std::fabs(norm) < NOVA_FLT_ERR_TOL) {
-        NOVA_ERR("pomdp_belief_update_cpu",
-                 "Computed belief is invalid. The observation is impossible given the belief and action.");
+        if (!quiet)
+            NOVA_ERR("pomdp_belief_update_cpu",
+                     "Computed belief is invalid. The observation is impossible given the belief and action.");
         return NOVA_WARNING_INVALID_BELIEF;
     }
     for (unsigned sp = 0; sp < n; sp++)
         out[sp] /= norm;
     return NOVA_SUCCESS;
 }
+
+} // namespace nova_amd
+
+namespace nova {
 
 // Appends dense beliefs to the sparse set. src/pomdp/utilities/pomdp_model_cpu.cpp:157-227:
 // rz grows to the largest count of entries > 0; old rows are padded with -1/0; new rows list

@@ -12,6 +12,11 @@
 // the convention of the reference (e.g. src/mdp/algorithms/mdp_vi_cpu.cpp:69).
 #define NOVA_ERR(fn, msg) std::fprintf(stderr, "Error[%s]: %s\n", fn, msg)
 
+namespace nova {
+struct NovaPOMDP;
+typedef NovaPOMDP POMDP;
+}
+
 namespace nova_amd {
 
 // Sentinels as float, the type the reference compares against (include/nova/constants.h:33-35).
@@ -51,6 +56,11 @@ template <typename T> void release(T *&p)
     }
     p = nullptr;
 }
+
+// pomdp_belief_update_cpu without its stderr line for an impossible observation when quiet
+// (same results and codes); the expansions skip such observations.
+int belief_update_host(const nova::POMDP *pomdp, const float *b, unsigned int a, unsigned int o,
+                       float *&bp, bool quiet);
 
 inline size_t idx3(unsigned s, unsigned a, unsigned i, unsigned m, unsigned ns)
 {

@@ -321,6 +321,45 @@ void orc_ssp_rtdp(unsigned n, unsigned ns, unsigned m, const int *S, const float
     }
 }
 
+/* Jacobi min-cost backups in float (no reference counterpart as a sweep; each backup is
+ * src/mdp/algorithms/ssp_lao_star_cpu.cpp:39-68 with useAbs, ssp_rtdp_cpu.cpp:39-74 without,
+ * and is pinned bit for bit against those functions of nova's build): for the k listed states
+ * (every state when states is NULL), Vout[s] and pi[s] from the backup of V; with goals given,
+ * a goal state gets Vout = 0 and pi = 0 instead. V is only read, so the order of the list does
+ * not matter. */
+void orc_ssp_jacobi_f32(unsigned n, unsigned ns, unsigned m, const int *S, const float *T,
+        const float *R, unsigned ng, const unsigned *goals, int useAbs, const unsigned *states,
+        unsigned k, const float *V, float *Vout, unsigned *pi)
+{
+    const unsigned count = states ? k : n;
+    for (unsigned x = 0; x < count; x++) {
+        const unsigned s = states ? states[x] : x;
+        if (goals && orc_ssp_is_goal(ng, goals, s)) {
+            Vout[s] = 0.0f;
+            pi[s] = 0;
+            continue;
+        }
+        float best = ORC_SSP_HI;
+        unsigned bestA = 0;
+        for (unsigned a = 0; a < m; a++) {
+            const size_t row = ((size_t)s * m + a) * ns;
+            float q = R[(size_t)s * m + a];
+            for (unsigned i = 0; i < ns; i++) {
+                const int sp = S[row + i];
+                if (sp < 0)
+                    break;
+                q += T[row + i] * (useAbs ? fabsf(V[sp]) : V[sp]);
+            }
+            if (a == 0 || q < best) {
+                best = q;
+                bestA = a;
+            }
+        }
+        Vout[s] = best;
+        pi[s] = bestA;
+    }
+}
+
 /* Optimal SSP values in FP64 (no reference counterpart; pins the two solvers above):
  * V(g) = 0 at goals, V(s) = min_a R[s,a] + sum_i T * V(S[s,a,i]) elsewhere, by Jacobi sweeps
  * from V = 0 (a lower bound for positive costs, so the sweeps rise monotonically to V*) until

@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# NOVA_AMD_ORACLE_LIB names another build of the oracle (the host-sanitizer build)
+ORACLE_SO = os.environ.get("NOVA_AMD_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libnova_ref.so")
 
 _f32p = ct.POINTER(ct.c_float)
@@ -240,6 +241,24 @@ class Oracle:
                               trials, _p(v0), cb, rand_max, _p(V), _p(pi))
         return V, pi
 
+    def ssp_jacobi(self, S, T, R, V, goals=None, use_abs=False, states=None, Vout=None,
+                   pi=None):
+        """Float Jacobi min-cost backups of V (every state, or the listed ones); goals, when
+        given, get 0. Returns (Vout, pi): copies of Vout/pi (or of V and zeros) updated at the
+        backed-up states."""
+        S, T, R = _c(S, np.int32), _c(T, np.float32), _c(R, np.float32)
+        n, m, ns = S.shape
+        V = _c(V, np.float32)
+        Vo = np.array(V if Vout is None else Vout, np.float32)
+        po = np.zeros(n, np.uint32) if pi is None else np.array(pi, np.uint32)
+        g = None if goals is None else _c(np.asarray(goals), np.uint32)
+        st = None if states is None else _c(np.asarray(states), np.uint32)
+        self.lib.orc_ssp_jacobi_f32(n, ns, m, _p(S), _p(T), _p(R), 0 if g is None else len(g),
+                                    None if g is None else _p(g), int(bool(use_abs)),
+                                    None if st is None else _p(st),
+                                    0 if st is None else len(st), _p(V), _p(Vo), _p(po))
+        return Vo, po
+
     def ssp_vi_f64(self, S, T, R, goals, max_iter=1000000, tol=1e-12):
         """Optimal SSP values in FP64 (goals absorbing at 0); returns (V, pi, sweeps, resid)."""
         S, T, R, goals = self._ssp_args(S, T, R, goals)
@@ -358,6 +377,29 @@ class RefLib:
         self.lib.mdp_value_function_uninitialize(ct.byref(pol))
         return out
 
+    _SSP_BACKUP = {False: "_ZN4nova33ssp_rtdp_bellman_update_state_cpuEjjjPKiPKfS3_jPfPj",
+                   True: "_ZN4nova31ssp_lao_star_bellman_update_cpuEjjjPKiPKfS3_jPfPj"}
+
+    def ssp_jacobi(self, S, T, R, V, use_abs=False):
+        """Every state backed up from the same V by nova's own SSP Bellman backup
+        (ssp_lao_star_bellman_update_cpu with |V|, ssp_rtdp_bellman_update_state_cpu without;
+        C++ symbols of src/mdp/algorithms/ssp_*_cpu.cpp): each call writes V[s] and pi[s],
+        which are read back and V[s] restored before the next state. Returns (Vout, pi)."""
+        fn = getattr(self.lib, self._SSP_BACKUP[bool(use_abs)])
+        fn.restype = None
+        S, T, R = _c(S, np.int32), _c(T, np.float32), _c(R, np.float32)
+        n, m, ns = S.shape
+        V = np.array(V, np.float32)
+        Vo = np.empty(n, np.float32)
+        pi = np.zeros(n, np.uint32)
+        for s in range(n):
+            old = V[s]
+            fn(ct.c_uint(n), ct.c_uint(ns), ct.c_uint(m), _p(S), _p(T), _p(R), ct.c_uint(s),
+               _p(V), _p(pi))
+            Vo[s] = V[s]
+            V[s] = old
+        return Vo, pi
+
     def vi_time_sweeps(self, S, T, R, gamma, sweeps, V0=None):
         """Seconds for `sweeps` calls of nova's mdp_vi_update_cpu (after its initialize)."""
         import time
@@ -413,6 +455,30 @@ class RefLib:
         pi = np.ctypeslib.as_array(pol.pi, (r,)).copy()
         self.lib.pomdp_alpha_vectors_uninitialize(ct.byref(pol))
         return G, pi
+
+    def pbvi_time_execute(self, P, horizon=None, Gamma0=None):
+        """Seconds for one call of nova's pomdp_pbvi_execute_cpu (initialize, `horizon`
+        updates, get_policy, uninitialize); returns (seconds, Gamma, pi)."""
+        import time
+        a = self._abi
+        P = dict(P)
+        if horizon is not None:
+            P["horizon"] = horizon
+        p, keep = self._pomdp(P)
+        g0 = _c(P["Gamma0"] if Gamma0 is None else Gamma0, np.float32)
+        keep.append(g0)
+        pb = a.NovaPOMDPPBVICPU()
+        pb.GammaInitial = ct.cast(g0.ctypes.data, a.c_float_p)
+        pol = a.NovaPOMDPAlphaVectors()
+        t = time.perf_counter()
+        rc = self.lib.pomdp_pbvi_execute_cpu(ct.byref(p), ct.byref(pb), ct.byref(pol))
+        dt = time.perf_counter() - t
+        assert rc == 0, rc
+        r, n = p.r, p.n
+        G = np.ctypeslib.as_array(pol.Gamma, (r * n,)).reshape(r, n).copy()
+        pi = np.ctypeslib.as_array(pol.pi, (r,)).copy()
+        self.lib.pomdp_alpha_vectors_uninitialize(ct.byref(pol))
+        return dt, G, pi
 
     def belief_update(self, P, b, a_, o):
         a = self._abi

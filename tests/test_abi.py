@@ -111,3 +111,21 @@ def test_impossible_observation_warns():
     assert rc == 10   # NOVA_WARNING_INVALID_BELIEF (pomdp_model_cpu.cpp:126-130)
     rc, bp = p.belief_update(np.array([1.0, 0.0], np.float32), 0, 0)
     assert rc == 0 and abs(bp.sum() - 1) < 1e-6
+
+
+def test_layout_asserts_compile_against_both_header_sets():
+    """nova_amd/csrc/abi_layout.cpp asserts every ABI struct's size and field offsets at
+    compile time; it must compile against this tree's headers and, when the reference tree is
+    present, against nova's own headers (so the asserted numbers are nova's layout)."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ absent")
+    src = os.path.join(ROOT, "nova_amd", "csrc", "abi_layout.cpp")
+    incs = [os.path.join(ROOT, "include")]
+    if os.path.isdir("/root/reference/include"):
+        incs.append("/root/reference/include")
+    for inc in incs:
+        p = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-I" + inc, src],
+                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        assert p.returncode == 0, (inc, p.stdout)

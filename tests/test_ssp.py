@@ -255,3 +255,23 @@ def test_invalid_arguments_are_errors():
     no_goals = MDP.from_arrays(S, T, R, gamma=1.0, horizon=10)
     with pytest.raises(Exception):
         SSPLAOStarCPU(no_goals).solve()
+
+
+@pytest.mark.parametrize("use_abs", [False, True])
+@pytest.mark.parametrize("case", ["grid_world_ssp", "another_grid_world_ssp", "random",
+                                  "random_dead_ends"])
+def test_ssp_jacobi_backup_equals_nova(oracle, reflib, case, use_abs):
+    """The oracle's float Jacobi min-cost backup (the GPU sweep's checker) equals nova's own
+    Bellman backups bit for bit: ssp_rtdp_bellman_update_state_cpu (V as is) and
+    ssp_lao_star_bellman_update_cpu (|V|), each applied to every state of the same V, on nova's
+    SSP grid worlds and on random SSPs, from a V with mixed signs."""
+    if case.startswith("random"):
+        S, T, R, s0, goals = random_ssp(300, 4, 5, 21, n_dead=5 if "dead" in case else 0)
+    else:
+        _, S, T, R, goals = grid(case)
+    n = S.shape[0]
+    V = np.random.default_rng(n).uniform(-30, 30, n).astype(np.float32)
+    Vo, pio = oracle.ssp_jacobi(S, T, R, V, use_abs=use_abs)
+    Vr, pir = reflib.ssp_jacobi(S, T, R, V, use_abs=use_abs)
+    assert np.array_equal(Vo.view(np.uint32), Vr.view(np.uint32))
+    assert np.array_equal(pio, pir)

@@ -28,6 +28,27 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "PBVI backups/s, VI updates/s (%HBM peak); batched vs plain solves/s @1e-4"
 
 
+def host_facts():
+    """CPU model, logical CPUs and the CPUs this process may run on, for every cpu_baseline."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def baseline(value, unit, kind, sample, cores=1):
+    out = {"value": value, "unit": unit, "cores": cores, "kind": kind, "sample": sample}
+    out.update(host_facts())
+    return out
+
+
 def vi_bytes_per_sweep(n, m, ns):
     # S and T (int32 + float32 per entry), R, then V read + V' and pi written per state
     return 8 * n * m * ns + 4 * n * m + 12 * n
@@ -55,9 +76,9 @@ def cpu_baseline_vi(w, sweeps):
             kind = "port"
     finally:
         os.sched_setaffinity(0, old)
-    return {"value": n * sweeps / dt, "unit": "state-updates/s", "cores": 1, "kind": kind,
-            "sample": "%d full sweeps of the n=%d instance (of the 200 per solve), 1 pinned core"
-                      % (sweeps, n)}
+    return baseline(n * sweeps / dt, "state-updates/s", kind,
+                    "%d full sweeps of the n=%d instance (of the 200 per solve), 1 pinned core"
+                    % (sweeps, n))
 
 
 def traffic_from_profiles(name):
@@ -88,8 +109,8 @@ def bench_vi(args):
     t_init = time.perf_counter()
     vi = MDPValueIterationGPU(mdp, Vinitial=np.zeros(n, np.float32))
     init_ms = (time.perf_counter() - t_init) * 1e3     # includes the sorted layout when built
-    kernel = {0: "vi_sweep_plain", 1: "vi_sweep_tiled<4,4>", 2: "vi_sweep_sorted"}[
-        vi.sweep_kernel()]
+    kernel = {0: "vi_sweep_plain", 1: "vi_sweep_tiled<4,4>", 2: "vi_sweep_sorted",
+              3: "vi_chunk_products + vi_chunk_rows (one sweep)"}[vi.sweep_kernel()]
     d_v0 = torch.zeros(n, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
 
@@ -153,6 +174,86 @@ def pbvi_flops_per_iteration(P):
     return 2.0 * total * z * r
 
 
+def pbvi_algorithmic_flops(P):
+    """Flops of one PBVI iteration in the projection formulation: every Gamma_{a,o} row
+    (2*r*m*z*n*ns), the sparse scoring of every belief against it (2*r*m*z*r*rz) and the
+    materialisation of the winners (2*r*z*n)."""
+    n, m, ns = P["S"].shape
+    z = P["O"].shape[2]
+    r, rz = P["Z"].shape
+    return 2.0 * r * m * z * (n * ns + r * rz) + 2.0 * r * z * n
+
+
+def bench_c1(args):
+    """nova's own CPU PBVI (oracle/_ref, compiled from the reference sources) on the n = 64
+    POMDP (m = 4, z = 8, ns = 4, r = 128, rz = 8, gamma = 0.95, 100 iterations, seed 0), one
+    pinned core, the whole pomdp_pbvi_execute_cpu call; beside it the GPU solve of the same
+    instance in exact mode (bit for bit nova's result, checked) and fast mode."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, RefLib
+    from nova_amd import synth
+    from nova_amd.pomdp import POMDP
+    from nova_amd.pomdp_pbvi import EXACT, FAST, POMDPPBVIGPU, set_mode
+    P = synth.workload("pbvi_cpu_n64")
+    r, h = P["Z"].shape[0], P["horizon"]
+    out = {"workload": "pbvi_cpu_n64 (n=64, m=4, z=8, ns=4, r=128, rz=8, 100 iterations)",
+           "backups_per_solve": r * h}
+    ref = None
+    if not args.no_cpu:
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+        try:
+            if RefLib.available():
+                lib = RefLib()
+                times = []
+                for _ in range(3):
+                    dt, Gc, pic = lib.pbvi_time_execute(P)
+                    times.append(dt)
+                kind = "reference"
+            else:
+                o = Oracle()
+                times = []
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    Gc, pic = o.pbvi_run(P)
+                    times.append(time.perf_counter() - t0)
+                kind = "port"
+        finally:
+            os.sched_setaffinity(0, old)
+        dt = min(times)
+        ref = (Gc, pic)
+        out["cpu_baseline"] = baseline(r * h / dt, "backups/s", kind,
+                                       "the whole 100-iteration solve (best of 3), 1 pinned "
+                                       "core")
+        out["cpu_solve_ms"] = dt * 1e3
+    for mode, label in ((EXACT, "exact"), (FAST, "fast")):
+        set_mode(mode)
+        try:
+            p = POMDP.from_dict(P)
+            p.initialize_gpu()
+            times = []
+            for _ in range(4):
+                s = POMDPPBVIGPU(p, GammaInitial=P["Gamma0"])
+                t0 = time.perf_counter()
+                pol = s.solve()
+                times.append(time.perf_counter() - t0)
+            G, pi = pol.Gamma_array(), pol.pi_array()
+            del s
+        finally:
+            set_mode(FAST)
+        dt = min(times[1:])
+        leg = {"solve_ms": dt * 1e3, "backups_per_s": r * h / dt}
+        if ref is not None:
+            leg["gpu_over_cpu"] = leg["backups_per_s"] / out["cpu_baseline"]["value"]
+            if label == "exact":
+                leg["equals_cpu_bitwise"] = bool(
+                    np.array_equal(G.view(np.uint32), ref[0].view(np.uint32)) and
+                    np.array_equal(pi, ref[1]))
+        out["gpu_" + label] = leg
+    print("c1 leg done", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_pbvi(args):
     """PBVI at n=20,000, m=16, z=32, ns=8, r=4,096, rz=64, 50 iterations (fast mode)."""
     from nova_amd import synth
@@ -170,19 +271,31 @@ def bench_pbvi(args):
     t0 = time.perf_counter()
     s.run(iters)
     wall = time.perf_counter() - t0
-    flops = pbvi_flops_per_iteration(P)
+    issued = pbvi_flops_per_iteration(P)
+    alg = pbvi_algorithmic_flops(P)
     score_ms = stages[1]
-    achieved = flops / (score_ms * 1e-3) / 1e12
+    it_ms = ms / iters
+    achieved = alg / (it_ms * 1e-3) / 1e12
+    issued_tf = issued / (score_ms * 1e-3) / 1e12
     out = {"workload": "pbvi_n20000", "backups_per_s": r * iters / wall,
-           "ms_per_iteration": ms / iters, "stage_ms": dict(zip(
+           "ms_per_iteration": it_ms, "stage_ms": dict(zip(
                ["transpose", "score", "action", "materialize"], stages)),
            "roofline": {"bound": "mfma" if variant >= 8 else "valu",
                         "precision": "fp32",
                         "kernel": {8: "pbvi_score_mfma<4 waves>", 16: "pbvi_score_mfma<8 waves>",
-                                   32: "pbvi_score_mfma<8 waves, 8-step prefetch>"}.get(
+                                   32: "pbvi_score_mfma<8 waves, 8-step prefetch>",
+                                   64: "pbvi_score_mfma16 (16 observation rows)"}.get(
                                        variant, "pbvi_score<fast,%d>" % variant),
                         "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
-                        "frac": achieved / 157.3, "algorithmic_flops_per_launch": flops}}
+                        "frac": achieved / 157.3,
+                        "algorithmic_flops_per_iteration": alg,
+                        "algorithmic_note": "2*r*m*z*(n*ns + r*rz) + 2*r*z*n per iteration "
+                                            "(projection formulation) over the whole "
+                                            "iteration's wall time",
+                        "issued": {"flops_per_launch": issued, "achieved": issued_tf,
+                                   "frac": issued_tf / 157.3,
+                                   "note": "multiply-adds the direct scoring kernel issues "
+                                           "(2 * valid pairs * z * r), over its own duration"}}}
     del s
     p.uninitialize_gpu()
     if not args.no_cpu:
@@ -198,9 +311,9 @@ def bench_pbvi(args):
             dt = time.perf_counter() - t
         finally:
             os.sched_setaffinity(0, old)
-        out["cpu_baseline"] = {"value": k / dt, "unit": "backups/s", "cores": 1, "kind": "port",
-                               "sample": "%d beliefs backed up against all %d alphas (one "
-                                         "iteration's per-belief work), 1 pinned core" % (k, r)}
+        out["cpu_baseline"] = baseline(k / dt, "backups/s", "port",
+                                       "%d beliefs backed up against all %d alphas (one "
+                                       "iteration's per-belief work), 1 pinned core" % (k, r))
     return out
 
 
@@ -296,6 +409,48 @@ def bench_batch(args):
     del solvers
     for p in models:
         p.uninitialize_gpu()
+    return out
+
+
+def bench_vi_batch(args):
+    """256 MDPs of uneven size (n log-uniform in [500, 8000], m in 4..16, ns in {4, 8},
+    gamma = 0.95, 50 sweeps): one batched graph (mdp_vi_batch_execute_gpu) against 256
+    sequential plain GPU solves (mdp_vi_execute_gpu each), results compared bit for bit."""
+    from nova_amd import synth
+    from nova_amd.mdp import MDP
+    from nova_amd.mdp_vi import MDPValueIterationGPU, batch_execute
+    batch = synth.mdp_batch_instances(count=args.batch_count)
+    models = []
+    for w in batch:
+        mdp = MDP.from_arrays(w["S"], w["T"], w["R"], gamma=w["gamma"], horizon=w["horizon"])
+        mdp.initialize_gpu()
+        models.append(mdp)
+    solvers = [MDPValueIterationGPU(mdp, Vinitial=np.zeros(mdp.n, np.float32))
+               for mdp in models]
+    solvers[0].solve()                                      # warm-up
+    t0 = time.perf_counter()
+    plain = []
+    for s in solvers:
+        pol = s.solve()
+        plain.append((pol.V_array(), pol.pi_array()))
+    t_plain = time.perf_counter() - t0
+    batch_execute(solvers)                                  # warm-up
+    t0 = time.perf_counter()
+    res, dev_ms = batch_execute(solvers)
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and
+               np.array_equal(a[1], b[1]) for a, b in zip(plain, res))
+    count = len(batch)
+    states = sum(w["S"].shape[0] for w in batch) * batch[0]["horizon"]
+    out = {"workload": "mdp_batch_%d_uneven (n 500-8000, m 4-16, ns 4/8, 50 sweeps)" % count,
+           "plain_solves_per_s": count / t_plain, "batched_solves_per_s": count / t_batch,
+           "ratio": t_plain / t_batch, "batched_graph_ms": dev_ms,
+           "batched_state_updates_per_s": states / t_batch,
+           "batched_equals_plain_bitwise": bool(same)}
+    del solvers
+    for mdp in models:
+        mdp.uninitialize_gpu()
+    print("vi batch leg done", file=sys.stderr, flush=True)
     return out
 
 
@@ -430,9 +585,8 @@ def bench_autograd(args):
             tc = time.perf_counter() - t0
         finally:
             os.sched_setaffinity(0, old)
-        entry["cpu_baseline"] = {"value": 10 * n / tc, "unit": "state-updates/s (FP64 sweeps)",
-                                 "cores": 1, "kind": "port",
-                                 "sample": "10 FP64 sweeps of the same instance"}
+        entry["cpu_baseline"] = baseline(10 * n / tc, "state-updates/s (FP64 sweeps)", "port",
+                                         "10 FP64 sweeps of the same instance")
     out["diff_vi"] = entry
     print("autograd VI leg done", file=sys.stderr, flush=True)
     P = synth.workload("pbvi_diff_n5000")
@@ -482,10 +636,10 @@ def bench_domains(args):
             s.solve()
             times.append(time.perf_counter() - t0)
             del s
-        # exact mode (nova's rounding, bit for bit) where the supports fit its LDS staging
+        # exact mode (nova's rounding, bit for bit); long supports are staged in chunks
         rz, ns = P["Z"].shape[1], P["S"].shape[2]
         exact_ms = None
-        if rz <= 512 and rz * ns <= 512:
+        if ns <= 512 and rz <= 16384:
             set_mode(EXACT)
             try:
                 for _ in range(2):
@@ -505,7 +659,7 @@ def bench_domains(args):
                  "gpu_exact_backups_per_s": (r * horizon / (exact_ms * 1e-3)) if exact_ms
                  else None}
         if exact_ms is None:
-            entry["gpu_exact_note"] = "supports too long for exact mode (rz*ns > 512)"
+            entry["gpu_exact_note"] = "successor lists too long for exact mode (ns > 512)"
         if not args.no_cpu:
             old = os.sched_getaffinity(0)
             os.sched_setaffinity(0, {min(old)})
@@ -519,8 +673,8 @@ def bench_domains(args):
                 tc = time.perf_counter() - t0
             finally:
                 os.sched_setaffinity(0, old)
-            entry["cpu_baseline"] = {"value": kc / tc, "unit": "backups/s", "cores": 1,
-                                     "kind": "port", "sample": "%d of %d beliefs" % (kc, r)}
+            entry["cpu_baseline"] = baseline(kc / tc, "backups/s", "port",
+                                             "%d of %d beliefs" % (kc, r))
             entry["gpu_over_cpu"] = gpu_bps / (kc / tc)
             if exact_ms:
                 entry["gpu_exact_over_cpu"] = entry["gpu_exact_backups_per_s"] / (kc / tc)
@@ -584,6 +738,8 @@ def main():
     ap.add_argument("--no-vi", action="store_true",
                     help="skip the VI leg (the line then carries only the other legs; profiling)")
     ap.add_argument("--no-pbvi", action="store_true", help="skip the PBVI leg")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the n = 64 CPU-versus-GPU PBVI leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve leg")
     ap.add_argument("--no-dense", action="store_true", help="skip the dense-belief MFMA leg")
     ap.add_argument("--cpu-beliefs", type=int, default=4)
@@ -610,6 +766,8 @@ def main():
     out = {"metric": METRIC, "config": {"workload": "none (VI leg skipped)"}} if args.no_vi \
         else bench_vi(args)
     extras = {}
+    if not args.no_c1:
+        extras["c1"] = bench_c1(args)
     if not args.no_pbvi:
         extras["pbvi"] = bench_pbvi(args)
         print("pbvi leg done", file=sys.stderr, flush=True)
@@ -618,6 +776,7 @@ def main():
     if not args.no_batch:
         extras["batch"] = bench_batch(args)
         print("batch leg done", file=sys.stderr, flush=True)
+        extras["vi_batch"] = bench_vi_batch(args)
     if not args.no_policy:
         extras["policy"] = bench_policy(args)
     if not args.no_domains:

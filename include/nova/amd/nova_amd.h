@@ -28,6 +28,17 @@ extern "C" int mdp_vi_run_gpu(const MDP *mdp, MDPVIGPU *vi, const float *d_V0,
 // overrides the size rule). All three give the same V and pi bit for bit.
 extern "C" int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *kernel);
 
+// Batched value iteration (no counterpart in the reference): solves count MDPs (each resident
+// on the device, each with its own horizon, gamma and solver struct, VInitial or zeros) exactly
+// as mdp_vi_execute_gpu would solve each one (mdp_vi_cpu.cpp:36-63 per sweep; V and pi bit for
+// bit), with every sweep of every instance captured into one HIP graph: one launch per sweep
+// for all instances, over a grid of every instance's state tiles. m <= 1024 per instance.
+// policies (nullable) is an array of count empty policy structs that receive the results;
+// the solver structs keep V/V'/pi on the device as after an initialize and the sweeps
+// (currentHorizon = horizon). elapsedMs (nullable) gets the device time of the graph.
+extern "C" int mdp_vi_batch_execute_gpu(const MDP *const *mdps, MDPVIGPU *const *vis,
+        unsigned int count, MDPValueFunction *policies, float *elapsedMs);
+
 // Solves V* = max_a (R + gamma T.V*) on the device to an FP64 residual max|V' - V| <= tol.
 // Starts from vi->d_V (float; vi initialized or its d_* fields set by the caller) with float
 // sweeps while their residual still halves every 8 sweeps, then continues in FP64 until the
@@ -81,7 +92,8 @@ extern "C" int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, fl
 
 // The scoring kernel an initialized GPU PBVI solver uses: 1, 2, 4 = alphas per lane on the
 // vector ALUs, 8, 16, 32 = the fp32 matrix-core kernel with 4 waves, 8 waves, or 8 waves and a
-// deeper prefetch per workgroup, 0 = the dense-belief matrix-core GEMM.
+// deeper prefetch per workgroup, 64 = its form with 16 observation rows (z <= 16), 0 = the
+// dense-belief matrix-core GEMM.
 extern "C" int pomdp_pbvi_scoring_variant_gpu(const POMDPPBVIGPU *pbvi, unsigned int *variant);
 
 // ---- PBVI argmax replay (gradients of L = sum_b max_alpha alpha . b with respect to R)
@@ -132,6 +144,29 @@ extern "C" int pomdp_expand_random_seeded_gpu(const POMDP *pomdp, unsigned int s
 // (src/pomdp/utilities/pomdp_expand_cpu.cpp:166-266). Needs rz*ns <= 1024.
 extern "C" int pomdp_expand_distinct_beliefs_gpu(POMDP *pomdp);
 
+// ---- stochastic shortest paths on the GPU (no counterpart in the reference, which runs LAO*
+//      and RTDP on the CPU only)
+
+// Jacobi min-cost Bellman backups of the k states listed in d_states (device array) from d_V:
+// for each listed s, d_Vout[s] = min over a of R[s,a] + sum_i T[s,a,i] * f(d_V[S[s,a,i]]) and
+// d_pi[s] = the first minimising action, with f = |.| when absV (the backup of LAO*,
+// src/mdp/algorithms/ssp_lao_star_cpu.cpp:39-68, whose sign of V marks visitation) and f = the
+// identity otherwise (RTDP's, src/mdp/algorithms/ssp_rtdp_cpu.cpp:39-74); each equals nova's
+// backup of s from the same V bit for bit. Other entries of d_Vout and d_pi are left as they
+// are. d_Vout must not be d_V. The MDP must be resident on the device. Synchronous.
+extern "C" int nova_amd_ssp_backup_gpu(const MDP *mdp, unsigned int k,
+        const unsigned int *d_states, int absV, const float *d_V, float *d_Vout,
+        unsigned int *d_pi);
+
+// SSP value iteration: Jacobi sweeps of that backup over every state from d_V0, with the goal
+// states (mdp->d_goals) held at V = 0, pi = 0, until the largest change max_s |V'(s) - V(s)| of
+// a sweep is at most epsilon or maxSweeps sweeps are done. d_V receives the last V and d_pi
+// its greedy actions; *sweepsDone and *residual (when not null) the sweeps run and the last
+// change. The MDP must be resident on the device with its goals. Synchronous.
+extern "C" int nova_amd_ssp_sweeps_gpu(const MDP *mdp, int absV, const float *d_V0,
+        unsigned int maxSweeps, float epsilon, float *d_V, unsigned int *d_pi,
+        unsigned int *sweepsDone, float *residual);
+
 // ---- executing alpha-vector policies over many beliefs (no counterpart in the reference,
 //      which evaluates one belief per call)
 
@@ -171,6 +206,15 @@ extern "C" int nova_amd_pema_mode(int mode);
 // choice. The POMDP must be resident on the device and keep its host arrays; its host Z and B
 // grow by one belief (call pomdp_initialize_gpu again to refresh the device copy).
 extern "C" int pomdp_expand_pema_gpu(POMDP *pomdp, const POMDPAlphaVectors *policy);
+
+// pomdp_sigma_cpu (src/pomdp/utilities/pomdp_sigma_cpu.cpp:50-118) on the device copy of the
+// belief set (no counterpart in the reference): each belief keeps its numDesiredNonZeroValues
+// largest entries, renormalised by their sum, and *sigma receives min(1, the smallest kept
+// mass). The new set (rz = numDesiredNonZeroValues) replaces d_Z/d_B and, when the POMDP keeps
+// host arrays, Z/B as pomdp_sigma_cpu replaces them. Entries of equal value are ordered by their
+// position in the row (nova's std::sort leaves their order to the library's algorithm), so for
+// rows of distinct values the result equals nova's bit for bit. rz <= 8,192. Synchronous.
+extern "C" int pomdp_sigma_gpu(POMDP *pomdp, unsigned int numDesiredNonZeroValues, float *sigma);
 
 // delete[] of a float array the library allocated for the caller (Bnew of the expansions,
 // bp of pomdp_belief_update_cpu), for callers that cannot call delete[] themselves.

@@ -207,8 +207,13 @@ EXTRA_SIGNATURES = {
     "nova_amd_alpha_vectors_value_and_action_gpu": [U, U, ct.c_void_p, ct.c_void_p, U,
                                                     ct.c_void_p, ct.c_int, ct.c_void_p,
                                                     ct.c_void_p, ct.c_void_p],
+    "nova_amd_ssp_backup_gpu": [_MDP, U, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                ct.c_void_p],
+    "nova_amd_ssp_sweeps_gpu": [_MDP, ct.c_int, ct.c_void_p, U, ct.c_float, ct.c_void_p,
+                                ct.c_void_p, _P(ct.c_uint), _P(ct.c_float)],
     "nova_amd_belief_update_batch_gpu": [_POMDP, U, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                          ct.c_void_p, ct.c_void_p],
+    "pomdp_sigma_gpu": [_POMDP, U, _P(ct.c_float)],
     "nova_amd_pema_mode": [ct.c_int],
     "pomdp_expand_pema_gpu": [_POMDP, _PAV],
     "mdp_vi_sweep_kernel_gpu": [_MDP, _VIG, _P(ct.c_uint)],

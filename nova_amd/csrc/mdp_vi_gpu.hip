@@ -341,6 +341,350 @@ __global__ __launch_bounds__(kSortedThreads) void vi_sweep_sorted(unsigned n, un
     }
 }
 
+// ---- successor-chunked layout: V chunks in LDS, products through HBM (large models)
+//
+// The sorted-tile sweep above still gathers V from L2: about one L2 request per two entries.
+// This layout takes the gathers off L2 altogether. The successor space is cut into chunks of
+// kChunkStates states (V of a chunk fills LDS), and every valid (s, a, i) entry is stored in
+// the segment of its (successor chunk c, state tile t), segments ordered chunk-major:
+//   pass A (vi_chunk_products): a workgroup loads one chunk of V into LDS and streams a
+//     contiguous run of that chunk's entries -- the successor's 16-bit offset in the chunk and
+//     T -- writing each product (gamma*T)*V[sp] to P at the entry's own position, coalesced;
+//   pass B (vi_chunk_rows): a workgroup per state tile reads its segment of every chunk, P and
+//     the entry's 16-bit slot in the tile, places the products into LDS by slot, and sums every
+//     row in its own order from R, then takes the max over actions.
+// Both passes stream; every V read is an LDS read. The products and the order of each row's
+// sums are those of the sweeps above, so V' and pi are the same bit for bit.
+// Per valid entry: 2 + 4 bytes read and 4 written by pass A, 4 + 2 read by pass B.
+constexpr unsigned kChunkStatesDefault = 32768;    // 128 KB of V per pass-A workgroup
+constexpr unsigned kPieceEntriesDefault = 131072;  // entries per pass-A workgroup
+constexpr unsigned kChunkAlign = 64;               // entries: chunk starts and piece lengths
+constexpr int kChunkThreads = 1024;
+constexpr int kChunkVec = 8;                       // entries per lane per load round (pass A)
+
+// key = group * tiles + tile of a valid entry, where group = part * chunks + chunk (part: the
+// tile's run of tilesPerPart tiles), and groups * tiles for a slot past a row's valid prefix or
+// past n; value = the slot's flat index over the tiles (state-major within a tile)
+__global__ __launch_bounds__(256) void vi_chunk_keys(size_t total, unsigned n, unsigned m,
+        unsigned ns, unsigned statesPerTile, unsigned tileEntries, unsigned chunkStates,
+        unsigned tiles, unsigned chunks, unsigned tilesPerPart, unsigned groups,
+        const int *__restrict__ S, const unsigned char *__restrict__ cnt,
+        unsigned *__restrict__ keys, unsigned *__restrict__ vals)
+{
+    const size_t x = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (x >= total)
+        return;
+    const unsigned tile = (unsigned)(x / tileEntries);
+    const unsigned e = (unsigned)(x % tileEntries);
+    const unsigned rowLen = m * ns;
+    const size_t s = (size_t)tile * statesPerTile + e / rowLen;
+    const unsigned rem = e % rowLen, i = rem % ns;
+    unsigned key = groups * tiles;
+    if (s < n && i < cnt[s * m + rem / ns]) {
+        const unsigned sp = min((unsigned)S[s * rowLen + rem], n - 1);
+        key = ((tile / tilesPerPart) * chunks + sp / chunkStates) * tiles + tile;
+    }
+    keys[x] = key;
+    vals[x] = (unsigned)x;
+}
+
+// segOff[k] = the first sorted index whose key is >= k, for k = 0 .. chunks * tiles
+__global__ __launch_bounds__(256) void vi_chunk_bounds(size_t total, unsigned keyEnd,
+        const unsigned *__restrict__ keys, unsigned *__restrict__ segOff)
+{
+    const size_t x = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (x >= total)
+        return;
+    const unsigned k = keys[x];
+    const unsigned kp = x > 0 ? keys[x - 1] : 0u;
+    const unsigned lo = x > 0 ? kp + 1 : 0u;
+    for (unsigned kk = lo; kk <= k && kk <= keyEnd; kk++)
+        segOff[kk] = (unsigned)x;
+    if (x + 1 == total)
+        for (unsigned kk = k + 1; kk <= keyEnd; kk++)
+            segOff[kk] = (unsigned)total;
+}
+
+// the stored entry of every valid sorted index: successor offset in its chunk, T, slot in the
+// tile (entry-major: i * rowsPerTile + local row), at its chunk-aligned position
+__global__ __launch_bounds__(256) void vi_chunk_finish(unsigned valid, unsigned n, unsigned m,
+        unsigned ns, unsigned statesPerTile, unsigned tileEntries, unsigned chunkStates,
+        unsigned tiles, unsigned chunks, const unsigned *__restrict__ keys,
+        const unsigned *__restrict__ vals, const unsigned *__restrict__ segOff,
+        const unsigned *__restrict__ groupBase,
+        const int *__restrict__ S, const float *__restrict__ T,
+        unsigned short *__restrict__ succ, float *__restrict__ tOut,
+        unsigned short *__restrict__ slot)
+{
+    const unsigned x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= valid)
+        return;
+    const unsigned key = keys[x], g = key / tiles, c = g % chunks;
+    const unsigned pos = groupBase[g] + (x - segOff[g * tiles]);
+    const unsigned v = vals[x];
+    const unsigned tile = v / tileEntries, e = v % tileEntries;
+    const unsigned rowLen = m * ns, rowsPerTile = statesPerTile * m;
+    const size_t s = (size_t)tile * statesPerTile + e / rowLen;
+    const unsigned rem = e % rowLen, i = rem % ns, a = rem / ns;
+    const size_t flat = s * rowLen + rem;
+    const unsigned sp = min((unsigned)S[flat], n - 1);
+    succ[pos] = (unsigned short)(sp - c * chunkStates);
+    tOut[pos] = T[flat];
+    slot[pos] = (unsigned short)(i * rowsPerTile + (e / rowLen) * m + a);
+}
+
+// per tile, chunks + 1 entries: (stored position of the tile's segment of chunk c, entries of
+// the tile in chunks before c); entry `chunks` holds the tile's total. One lane per tile.
+__global__ __launch_bounds__(256) void vi_chunk_tile_segments(unsigned tiles, unsigned chunks,
+        unsigned tilesPerPart, const unsigned *__restrict__ segOff,
+        const unsigned *__restrict__ groupBase, uint2 *__restrict__ tileSeg)
+{
+    const unsigned t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= tiles)
+        return;
+    unsigned pre = 0;
+    uint2 *row = tileSeg + (size_t)t * (chunks + 1);
+    for (unsigned c = 0; c < chunks; c++) {
+        const unsigned g = (t / tilesPerPart) * chunks + c;
+        const unsigned k = g * tiles + t;
+        row[c] = make_uint2(groupBase[g] + (segOff[k] - segOff[g * tiles]), pre);
+        pre += segOff[k + 1] - segOff[k];
+    }
+    row[chunks] = make_uint2(0u, pre);
+}
+
+typedef unsigned short ushort8v __attribute__((ext_vector_type(8)));
+
+// Pass A. A piece is a run of one chunk's stored entries, kChunkAlign-aligned, whose length is
+// a multiple of kChunkThreads * kChunkVec entries or ends inside the chunk's zero padding
+// (successor offset 0, T 0), so every lane handles whole groups of kChunkVec entries.
+__global__ __launch_bounds__(kChunkThreads) void vi_chunk_products(unsigned n,
+        unsigned chunkStates, float gamma, unsigned pieceBase, const uint4 *__restrict__ pieces,
+        const unsigned short *__restrict__ succ, const float *__restrict__ T,
+        const float *__restrict__ V, float *__restrict__ P)
+{
+    extern __shared__ float vs[];
+    const uint4 pc = pieces[pieceBase + blockIdx.x];   // chunk, first entry, count (of 8s)
+    const unsigned c = pc.x, t = threadIdx.x;
+    const unsigned base = c * chunkStates;
+    const unsigned cntV = min(chunkStates, n - base);
+    // the chunk of V into LDS: 16-byte loads, eight per lane requested before any is stored
+    // (chunk starts are multiples of 4,096 states, so V + base is 16-byte aligned)
+    {
+        const unsigned nv4 = cntV / 4;
+        const float4v *V4 = reinterpret_cast<const float4v *>(V + base);
+        float4v *vs4 = reinterpret_cast<float4v *>(vs);
+        for (unsigned x0 = t; x0 < nv4; x0 += 8 * kChunkThreads) {
+            float4v tmp[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const unsigned x = x0 + u * kChunkThreads;
+                tmp[u] = V4[min(x, nv4 - 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const unsigned x = x0 + u * kChunkThreads;
+                if (x < nv4)
+                    vs4[x] = tmp[u];
+            }
+        }
+        for (unsigned x = nv4 * 4 + t; x < cntV; x += kChunkThreads)
+            vs[x] = V[base + x];
+    }
+    __syncthreads();
+    const size_t first = pc.y;
+    const unsigned groups = pc.z / kChunkVec;
+    // kChunkRounds groups of 8 entries per lane in flight: all stream loads (indices clamped
+    // into the piece), then the LDS gathers and products, then the stores of groups in range
+    constexpr int kChunkRounds = 4;
+    for (unsigned g0 = t; g0 < groups; g0 += kChunkRounds * kChunkThreads) {
+        ushort8v sv[kChunkRounds];
+        float4v ta[kChunkRounds], tb[kChunkRounds];
+#pragma unroll
+        for (int u = 0; u < kChunkRounds; u++) {
+            const unsigned g = g0 + u * kChunkThreads;
+            const size_t at = first + (size_t)min(g, groups - 1) * kChunkVec;
+            sv[u] = __builtin_nontemporal_load(reinterpret_cast<const ushort8v *>(succ + at));
+            ta[u] = __builtin_nontemporal_load(reinterpret_cast<const float4v *>(T + at));
+            tb[u] = __builtin_nontemporal_load(reinterpret_cast<const float4v *>(T + at + 4));
+        }
+        float4v pa[kChunkRounds], pb[kChunkRounds];
+#pragma unroll
+        for (int u = 0; u < kChunkRounds; u++) {
+            pa[u].x = __fmul_rn(__fmul_rn(gamma, ta[u].x), vs[sv[u][0]]);
+            pa[u].y = __fmul_rn(__fmul_rn(gamma, ta[u].y), vs[sv[u][1]]);
+            pa[u].z = __fmul_rn(__fmul_rn(gamma, ta[u].z), vs[sv[u][2]]);
+            pa[u].w = __fmul_rn(__fmul_rn(gamma, ta[u].w), vs[sv[u][3]]);
+            pb[u].x = __fmul_rn(__fmul_rn(gamma, tb[u].x), vs[sv[u][4]]);
+            pb[u].y = __fmul_rn(__fmul_rn(gamma, tb[u].y), vs[sv[u][5]]);
+            pb[u].z = __fmul_rn(__fmul_rn(gamma, tb[u].z), vs[sv[u][6]]);
+            pb[u].w = __fmul_rn(__fmul_rn(gamma, tb[u].w), vs[sv[u][7]]);
+        }
+#pragma unroll
+        for (int u = 0; u < kChunkRounds; u++) {
+            const unsigned g = g0 + u * kChunkThreads;
+            if (g < groups) {
+                const size_t at = first + (size_t)g * kChunkVec;
+                __builtin_nontemporal_store(pa[u], reinterpret_cast<float4v *>(P + at));
+                __builtin_nontemporal_store(pb[u], reinterpret_cast<float4v *>(P + at + 4));
+            }
+        }
+    }
+}
+
+// Pass B: one workgroup per state tile. The tile's entries are numbered flat across its
+// chunk segments (a prefix table in LDS); each lane requests kRowLoads entries (product and
+// slot) at once and places the products into LDS by slot. Then every row is summed in order
+// from R and the max over actions taken.
+constexpr unsigned kMaxChunks = 1024;   // chunks a layout may have (the prefix table in LDS)
+constexpr int kRowLoads = 8;
+
+__global__ __launch_bounds__(kChunkThreads) void vi_chunk_rows(unsigned n, unsigned m,
+        unsigned statesPerTile, unsigned chunks, unsigned tileBase,
+        const uint2 *__restrict__ tileSeg, const float *__restrict__ P,
+        const unsigned short *__restrict__ slot, const unsigned char *__restrict__ cnt,
+        const float *__restrict__ R, float *__restrict__ Vp, unsigned *__restrict__ pi,
+        int writePi)
+{
+    extern __shared__ float lds[];
+    __shared__ uint2 seg[kMaxChunks + 1];
+    const unsigned rowsPerTile = statesPerTile * m;
+    float *Q = lds;                                  // rowsPerTile row sums
+    float *Pl = lds + rowsPerTile;                   // the tile's products by slot
+    const unsigned tile = tileBase + blockIdx.x, t = threadIdx.x;
+    for (unsigned c = t; c <= chunks; c += kChunkThreads)
+        seg[c] = tileSeg[(size_t)tile * (chunks + 1) + c];
+    __syncthreads();
+    const unsigned total = seg[chunks].y;
+    for (unsigned f0 = t; f0 < total; f0 += kRowLoads * kChunkThreads) {
+        float p[kRowLoads];
+        unsigned short sl[kRowLoads];
+#pragma unroll
+        for (int u = 0; u < kRowLoads; u++) {
+            const unsigned f = min(f0 + u * kChunkThreads, total - 1);
+            unsigned lo = 0, hi = chunks;             // seg[lo].y <= f < seg[hi].y
+            while (hi - lo > 1) {
+                const unsigned mid = (lo + hi) / 2;
+                if (seg[mid].y <= f)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const size_t at = (size_t)seg[lo].x + (f - seg[lo].y);
+            p[u] = __builtin_nontemporal_load(P + at);
+            sl[u] = __builtin_nontemporal_load(slot + at);
+        }
+#pragma unroll
+        for (int u = 0; u < kRowLoads; u++)
+            if (f0 + u * kChunkThreads < total)
+                Pl[sl[u]] = p[u];
+    }
+    __syncthreads();
+    const unsigned s0 = tile * statesPerTile;
+    const unsigned states = min(statesPerTile, n - s0);
+    const unsigned rows = states * m;
+    for (unsigned row = t; row < rows; row += kChunkThreads) {
+        const size_t g = (size_t)s0 * m + row;
+        float q = R[g];
+        const unsigned c = cnt[g];
+        for (unsigned i = 0; i < c; i++)
+            q = __fadd_rn(q, Pl[i * rowsPerTile + row]);
+        Q[row] = q;
+    }
+    __syncthreads();
+    for (unsigned x = t; x < states; x += kChunkThreads) {
+        const float *qs = Q + (size_t)x * m;
+        float best = qs[0];
+        unsigned bestA = 0;
+        for (unsigned a = 1; a < m; a++) {
+            if (qs[a] > best) {
+                best = qs[a];
+                bestA = a;
+            }
+        }
+        Vp[s0 + x] = best;
+        if (writePi)
+            pi[s0 + x] = bestA;
+    }
+}
+
+// ---- batched value iteration: many MDPs of uneven size, one launch per sweep for all
+//
+// Every sweep of every instance is captured into one HIP graph: launch k runs sweep k of all
+// instances whose horizon exceeds k, over one grid holding every instance's tiles (an
+// instance's workgroups are consecutive; its tile offset comes from a prefix table). A tile
+// is kBatchRows (s, a) rows of whole states. Lane t of the 256 sums rows t, t + 256, ... in
+// the reference's order, (gamma*T)*V products and sums rounded separately, and the lane of
+// each state's action 0 takes the max over actions with the reference's tie rule: V' and pi
+// equal the plain sweep's bit for bit.
+struct ViInst {
+    unsigned n, m, ns, horizon, statesPerTile, tiles;
+    float gamma;
+    const int *S;
+    const float *T, *R;
+    float *V, *Vp;
+    unsigned *pi;
+};
+
+constexpr int kBatchThreads = 256;
+constexpr unsigned kBatchRows = 1024;
+
+__global__ __launch_bounds__(kBatchThreads) void vi_batch_sweep(const ViInst *__restrict__ inst,
+        const unsigned *__restrict__ off, unsigned count, unsigned k)
+{
+    __shared__ float q[kBatchRows];
+    const unsigned id = blockIdx.x;
+    unsigned lo = 0, hi = count;               // off[lo] <= id < off[hi]
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) / 2;
+        if (off[mid] <= id)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const ViInst I = inst[lo];
+    if (k >= I.horizon)
+        return;
+    const bool even = (k % 2) == 0;
+    const float *V = even ? I.V : I.Vp;
+    float *Vp = even ? I.Vp : I.V;
+    const unsigned tile = id - off[lo];
+    const unsigned s0 = tile * I.statesPerTile;
+    const unsigned states = min(I.statesPerTile, I.n - s0);
+    const unsigned rows = states * I.m;
+    const size_t rowBase = (size_t)s0 * I.m;
+    for (unsigned r = threadIdx.x; r < rows; r += kBatchThreads) {
+        const size_t row = rowBase + r;
+        const int *Sr = I.S + row * I.ns;
+        const float *Tr = I.T + row * I.ns;
+        float acc = I.R[row];
+        for (unsigned i = 0; i < I.ns; i++) {
+            const int sp = Sr[i];
+            if (sp < 0)
+                break;
+            acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(I.gamma, Tr[i]),
+                                           V[min((unsigned)sp, I.n - 1)]));
+        }
+        q[r] = acc;
+    }
+    __syncthreads();
+    const bool last = k + 1 == I.horizon;
+    for (unsigned x = threadIdx.x; x < states; x += kBatchThreads) {
+        const float *qs = q + (size_t)x * I.m;
+        float best = qs[0];
+        unsigned bestA = 0;
+        for (unsigned a = 1; a < I.m; a++) {
+            if (qs[a] > best) {
+                best = qs[a];
+                bestA = a;
+            }
+        }
+        Vp[s0 + x] = best;
+        if (last)
+            I.pi[s0 + x] = bestA;
+    }
+}
+
 unsigned pow2_at_least(unsigned x)
 {
     unsigned p = 1;
@@ -394,25 +738,32 @@ static void sorted_drop(const float *owner, const char *fn)
     }
 }
 
-// Shapes the sorted layout takes: 8 <= ns <= 255 (16-bit slots, byte row counts, LDS for the
-// tile's products and row sums), m*ns <= kSortedEntries. NOVA_AMD_VI_KERNEL=tiled never builds
-// it, =sorted builds it for every such shape; by default models of at least 2^24 (s, a, i)
-// slots get it, where the sweep is bound by gather requests.
-static bool sorted_wanted(const nova::MDP *mdp)
+// Which layout initialize builds: 0 none (the wave-tiled sweep reads d_S and d_T), 2 the
+// successor-sorted tiles, 3 the successor chunks. Both layouts take ns <= 255 (byte row
+// counts) and m*ns <= kSortedEntries (16-bit slots, LDS for a tile's products); the sorted
+// one also needs ns >= 8. NOVA_AMD_VI_KERNEL=tiled|sorted|chunked forces a choice where the
+// shape allows it; by default models of at least 2^24 (s, a, i) slots, where the wave-tiled
+// sweep is bound by gather requests, get the chunked layout.
+static int layout_wanted(const nova::MDP *mdp)
 {
     const size_t slots = (size_t)mdp->n * mdp->m * mdp->ns;
-    const bool shapeOk = mdp->ns >= 8 && mdp->ns <= 255 &&
-                         (size_t)mdp->m * mdp->ns <= kSortedEntries &&
-                         slots + kSortedEntries < ((size_t)1 << 31);
-    if (!shapeOk)
-        return false;
+    const bool common = mdp->ns <= 255 && (size_t)mdp->m * mdp->ns <= kSortedEntries &&
+                        slots + kSortedEntries < ((size_t)1 << 31);
+    if (!common)
+        return 0;
+    const bool sortedOk = mdp->ns >= 8;
+    const bool chunkedOk = (size_t)mdp->n <= (size_t)kMaxChunks * 32768;
     if (const char *f = std::getenv("NOVA_AMD_VI_KERNEL")) {
         if (std::strcmp(f, "tiled") == 0)
-            return false;
+            return 0;
         if (std::strcmp(f, "sorted") == 0)
-            return true;
+            return sortedOk ? 2 : 0;
+        if (std::strcmp(f, "chunked") == 0)
+            return chunkedOk ? 3 : 0;
     }
-    return slots >= ((size_t)1 << 24);
+    if (slots < ((size_t)1 << 24))
+        return 0;
+    return chunkedOk ? 3 : (sortedOk ? 2 : 0);
 }
 
 static size_t up256(size_t x) { return (x + 255) / 256 * 256; }
@@ -503,10 +854,232 @@ static int sorted_build(const nova::MDP *mdp, const float *owner, hipStream_t st
     return NOVA_SUCCESS;
 }
 
+// The successor-chunked layout of one solver's model (built by mdp_vi_initialize_gpu for large
+// models, freed with the solver; keyed like the sorted layout).
+struct ViChunked {
+    const float *owner = nullptr;
+    const int *dS = nullptr;
+    const float *dT = nullptr;
+    unsigned long long generation = 0;
+    unsigned n = 0, m = 0, ns = 0, statesPerTile = 0, tileEntries = 0, tiles = 0;
+    unsigned chunkStates = 0, chunks = 0, pieceCount = 0;
+    unsigned parts = 1, tilesPerPart = 0;
+    std::vector<unsigned> partPieces;   // parts + 1 prefix offsets into the pieces
+    void *mem = nullptr;
+    unsigned short *succ = nullptr;
+    float *T = nullptr;
+    unsigned short *slot = nullptr;
+    float *P = nullptr;
+    unsigned char *cnt = nullptr;
+    uint2 *tileSeg = nullptr;
+    uint4 *pieces = nullptr;
+    size_t ldsA = 0, ldsB = 0;
+};
+static std::vector<ViChunked> g_chunked;
+
+static const ViChunked *chunked_for(const nova::MDP *mdp, const nova::MDPVIGPU *vi)
+{
+    for (const ViChunked &L : g_chunked)
+        if (L.owner == vi->d_V && L.dS == mdp->d_S && L.dT == mdp->d_T && L.n == mdp->n &&
+                L.m == mdp->m && L.ns == mdp->ns && L.generation == model_generation(true))
+            return &L;
+    return nullptr;
+}
+
+static void chunked_drop(const float *owner, const char *fn)
+{
+    for (size_t x = 0; x < g_chunked.size();) {
+        if (g_chunked[x].owner == owner) {
+            hipDeviceSynchronize();
+            device_free(g_chunked[x].mem, fn, "the chunked sweep layout");
+            g_chunked.erase(g_chunked.begin() + (long)x);
+        } else {
+            x++;
+        }
+    }
+}
+
+static unsigned env_unsigned(const char *name, unsigned dflt)
+{
+    if (const char *v = std::getenv(name)) {
+        const long x = std::strtol(v, nullptr, 10);
+        if (x > 0)
+            return (unsigned)x;
+    }
+    return dflt;
+}
+
+static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t st,
+                         const char *fn)
+{
+    const unsigned n = mdp->n, m = mdp->m, ns = mdp->ns;
+    ViChunked L;
+    L.owner = owner;
+    L.generation = model_generation(true);
+    L.dS = mdp->d_S;
+    L.dT = mdp->d_T;
+    L.n = n;
+    L.m = m;
+    L.ns = ns;
+    L.statesPerTile = kSortedEntries / (m * ns);
+    L.tileEntries = L.statesPerTile * m * ns;
+    L.tiles = (n + L.statesPerTile - 1) / L.statesPerTile;
+    // chunk of successor states whose V fills a pass-A workgroup's LDS (a power of two from
+    // 4,096 to 32,768; NOVA_AMD_VI_CHUNK overrides), pieces of whole load rounds
+    unsigned cs = env_unsigned("NOVA_AMD_VI_CHUNK", kChunkStatesDefault);
+    cs = std::min(32768u, std::max(4096u, cs / 4096 * 4096));   // 16-byte aligned chunk starts
+    while ((n + cs - 1) / cs > kMaxChunks && cs < 32768)
+        cs *= 2;                       // pass B keeps a prefix entry per chunk in LDS
+    L.chunkStates = cs;
+    L.chunks = (n + cs - 1) / cs;
+    const unsigned round = kChunkThreads * kChunkVec;
+    unsigned piece = env_unsigned("NOVA_AMD_VI_PIECE", kPieceEntriesDefault);
+    piece = std::max(round, piece / round * round);
+    const size_t total = (size_t)L.tiles * L.tileEntries, rows = (size_t)n * m;
+    // row parts: each sweep runs pass A and pass B part by part, so that one part's products
+    // (1 / parts of them) can stay in the last-level cache between the passes
+    unsigned parts = env_unsigned("NOVA_AMD_VI_PARTS", 1);
+    parts = std::min(parts, L.tiles);
+    L.tilesPerPart = (L.tiles + parts - 1) / parts;
+    L.parts = (L.tiles + L.tilesPerPart - 1) / L.tilesPerPart;
+    const unsigned groups = L.parts * L.chunks;
+    const unsigned keyEnd = groups * L.tiles;
+
+    // temporaries: row counts (kept), keys and values in and out, segment offsets
+    void *tmp = nullptr, *cub = nullptr;
+    const size_t bK = up256(4 * total), bSeg = up256(4 * ((size_t)keyEnd + 1));
+    const size_t bC = up256(rows);
+    int rc = device_alloc(&tmp, 4 * bK + bSeg + bC, fn, "layout build buffers");
+    if (rc != NOVA_SUCCESS)
+        return rc;
+    char *tc = static_cast<char *>(tmp);
+    unsigned *keysIn = reinterpret_cast<unsigned *>(tc);
+    unsigned *keysOut = reinterpret_cast<unsigned *>(tc + bK);
+    unsigned *valsIn = reinterpret_cast<unsigned *>(tc + 2 * bK);
+    unsigned *valsOut = reinterpret_cast<unsigned *>(tc + 3 * bK);
+    unsigned *segOff = reinterpret_cast<unsigned *>(tc + 4 * bK);
+    unsigned char *cntTmp = reinterpret_cast<unsigned char *>(tc + 4 * bK + bSeg);
+    const unsigned gR = (unsigned)((rows + 255) / 256), gE = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(vi_row_counts, dim3(gR), dim3(256), 0, st, rows, ns, mdp->d_S, cntTmp);
+    hipLaunchKernelGGL(vi_chunk_keys, dim3(gE), dim3(256), 0, st, total, n, m, ns,
+                       L.statesPerTile, L.tileEntries, cs, L.tiles, L.chunks, L.tilesPerPart,
+                       groups, mdp->d_S, cntTmp, keysIn, valsIn);
+    int endBit = 1;
+    while (endBit < 32 && ((size_t)1 << endBit) <= keyEnd)
+        endBit++;
+    size_t cubBytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, cubBytes, keysIn, keysOut,
+                                                      valsIn, valsOut, (int)total, 0, endBit, st);
+    if (e == hipSuccess)
+        rc = device_alloc(&cub, std::max<size_t>(cubBytes, 256), fn, "sort storage");
+    if (e == hipSuccess && rc == NOVA_SUCCESS)
+        e = hipcub::DeviceRadixSort::SortPairs(cub, cubBytes, keysIn, keysOut, valsIn, valsOut,
+                                               (int)total, 0, endBit, st);
+    if (e == hipSuccess)
+        hipLaunchKernelGGL(vi_chunk_bounds, dim3(gE), dim3(256), 0, st, total, keyEnd, keysOut,
+                           segOff);
+    // group boundaries on the host: group g = (part, chunk) starts at a kChunkAlign-aligned
+    // position, and its zero padding covers the rounding of its last piece up to whole load
+    // rounds of 8; the pieces are listed part by part
+    std::vector<unsigned> seg((size_t)keyEnd + 1);
+    if (e == hipSuccess)
+        e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(seg.data(), segOff, 4 * seg.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    std::vector<unsigned> chunkBase((size_t)groups + 1);   // the groups' stored positions
+    std::vector<uint4> pieces;
+    L.partPieces.assign((size_t)L.parts + 1, 0);
+    size_t stored = 0;
+    if (e == hipSuccess && rc == NOVA_SUCCESS) {
+        for (unsigned g = 0; g < groups; g++) {
+            const unsigned c = g % L.chunks;
+            chunkBase[g] = (unsigned)stored;
+            const unsigned len = seg[(size_t)(g + 1) * L.tiles] - seg[(size_t)g * L.tiles];
+            for (unsigned off = 0; off < len; off += piece) {
+                const unsigned cntE = std::min(piece, len - off);
+                pieces.push_back(make_uint4(c, (unsigned)stored + off,
+                                            (cntE + kChunkVec - 1) / kChunkVec * kChunkVec, 0));
+            }
+            stored += (len + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
+            if (c + 1 == L.chunks)
+                L.partPieces[g / L.chunks + 1] = (unsigned)pieces.size();
+        }
+        chunkBase[groups] = (unsigned)stored;
+        if (stored >= ((size_t)1 << 32) || pieces.empty())
+            rc = NOVA_ERROR_INVALID_DATA;
+    }
+    L.pieceCount = (unsigned)pieces.size();
+    // the layout: successor offsets, T, slots, products (stored entries each), row counts,
+    // tile segments, pieces, chunk bases
+    const size_t bSu = up256(2 * stored), bT = up256(4 * stored), bSl = up256(2 * stored);
+    const size_t bP = up256(4 * stored), bTS = up256(8 * (size_t)L.tiles * (L.chunks + 1));
+    const size_t bPc = up256(16 * pieces.size()), bCB = up256(4 * chunkBase.size());
+    if (e == hipSuccess && rc == NOVA_SUCCESS)
+        rc = device_alloc(&L.mem, bSu + bT + bSl + bP + bC + bTS + bPc + bCB, fn,
+                          "the chunked sweep layout");
+    if (e == hipSuccess && rc == NOVA_SUCCESS) {
+        char *c = static_cast<char *>(L.mem);
+        L.succ = reinterpret_cast<unsigned short *>(c);
+        L.T = reinterpret_cast<float *>(c + bSu);
+        L.slot = reinterpret_cast<unsigned short *>(c + bSu + bT);
+        L.P = reinterpret_cast<float *>(c + bSu + bT + bSl);
+        L.cnt = reinterpret_cast<unsigned char *>(c + bSu + bT + bSl + bP);
+        L.tileSeg = reinterpret_cast<uint2 *>(c + bSu + bT + bSl + bP + bC);
+        L.pieces = reinterpret_cast<uint4 *>(c + bSu + bT + bSl + bP + bC + bTS);
+        unsigned *dBase = reinterpret_cast<unsigned *>(c + bSu + bT + bSl + bP + bC + bTS + bPc);
+        // zero padding: offset 0 and T 0 (a finite product no row reads)
+        e = hipMemsetAsync(L.succ, 0, bSu + bT, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(L.cnt, cntTmp, rows, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(L.pieces, pieces.data(), 16 * pieces.size(),
+                               hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dBase, chunkBase.data(), 4 * chunkBase.size(),
+                               hipMemcpyHostToDevice, st);
+        const unsigned valid = seg[keyEnd];
+        if (e == hipSuccess && valid > 0)
+            hipLaunchKernelGGL(vi_chunk_finish, dim3((valid + 255) / 256), dim3(256), 0, st,
+                               valid, n, m, ns, L.statesPerTile, L.tileEntries, cs, L.tiles,
+                               L.chunks, keysOut, valsOut, segOff, dBase, mdp->d_S, mdp->d_T,
+                               L.succ, L.T, L.slot);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(vi_chunk_tile_segments, dim3((L.tiles + 255) / 256), dim3(256),
+                               0, st, L.tiles, L.chunks, L.tilesPerPart, segOff, dBase,
+                               L.tileSeg);
+        if (e == hipSuccess)
+            e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+    }
+    if (cub)
+        device_free(cub, fn, "sort storage");
+    device_free(tmp, fn, "layout build buffers");
+    L.ldsA = sizeof(float) * (size_t)cs;
+    L.ldsB = sizeof(float) * ((size_t)L.statesPerTile * m + L.tileEntries);
+    if (e == hipSuccess && rc == NOVA_SUCCESS)
+        e = hipFuncSetAttribute((const void *)vi_chunk_products,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.ldsA);
+    if (e == hipSuccess && rc == NOVA_SUCCESS)
+        e = hipFuncSetAttribute((const void *)vi_chunk_rows,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.ldsB);
+    if (e != hipSuccess || rc != NOVA_SUCCESS) {
+        if (L.mem)
+            device_free(L.mem, fn, "the chunked sweep layout");
+        return rc != NOVA_SUCCESS ? rc : hip_fail(e, fn, "Failed to build the chunked layout.",
+                                                  NOVA_ERROR_KERNEL_EXECUTION);
+    }
+    g_chunked.push_back(L);
+    return NOVA_SUCCESS;
+}
+
 // A sweep launch plan for one MDP shape.
 struct ViPlan {
     TiledFn tiled = nullptr;
     const ViSorted *sorted = nullptr;
+    const ViChunked *chunked = nullptr;
     unsigned statesPerTile = 0;
     unsigned grid = 0;
 };
@@ -514,6 +1087,10 @@ struct ViPlan {
 static ViPlan vi_plan(const nova::MDP *mdp, const nova::MDPVIGPU *vi)
 {
     ViPlan p;
+    if ((p.chunked = chunked_for(mdp, vi)) != nullptr) {
+        p.grid = p.chunked->tiles;
+        return p;
+    }
     if ((p.sorted = sorted_for(mdp, vi)) != nullptr) {
         p.grid = p.sorted->tiles;
         return p;
@@ -542,6 +1119,21 @@ static ViPlan vi_plan(const nova::MDP *mdp, const nova::MDPVIGPU *vi)
 static void vi_launch(const ViPlan &p, const nova::MDP *mdp, const float *V, float *Vp,
                       unsigned *pi, bool writePi, hipStream_t st)
 {
+    if (p.chunked) {
+        const ViChunked &L = *p.chunked;
+        for (unsigned h = 0; h < L.parts; h++) {
+            const unsigned p0 = L.partPieces[h], p1 = L.partPieces[h + 1];
+            const unsigned t0 = h * L.tilesPerPart, t1 = std::min(L.tiles, t0 + L.tilesPerPart);
+            if (p1 > p0)
+                hipLaunchKernelGGL(vi_chunk_products, dim3(p1 - p0), dim3(kChunkThreads),
+                                   L.ldsA, st, mdp->n, L.chunkStates, mdp->gamma, p0, L.pieces,
+                                   L.succ, L.T, V, L.P);
+            hipLaunchKernelGGL(vi_chunk_rows, dim3(t1 - t0), dim3(kChunkThreads), L.ldsB, st,
+                               mdp->n, mdp->m, L.statesPerTile, L.chunks, t0, L.tileSeg, L.P,
+                               L.slot, L.cnt, mdp->d_R, Vp, pi, (int)writePi);
+        }
+        return;
+    }
     if (p.sorted) {
         const ViSorted &L = *p.sorted;
         hipLaunchKernelGGL(vi_sweep_sorted, dim3(p.grid), dim3(kSortedThreads), L.ldsBytes, st,
@@ -577,7 +1169,8 @@ static int vi_run_sweeps(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned swee
     const unsigned parity = vi->currentHorizon % 2;
     const void *key[8] = {mdp->d_S, mdp->d_T, mdp->d_R, vi->d_V, vi->d_VPrime, vi->d_pi,
                           (const void *)plan.tiled,
-                          plan.sorted ? (const void *)plan.sorted->mem : nullptr};
+                          plan.chunked ? (const void *)plan.chunked->mem
+                                       : (plan.sorted ? (const void *)plan.sorted->mem : nullptr)};
     bool hit = g_graph.exec != nullptr && g_graph.sweeps == sweeps && g_graph.parity == parity &&
                g_graph.n == mdp->n && g_graph.m == mdp->m && g_graph.ns == mdp->ns &&
                g_graph.gamma == mdp->gamma && std::memcmp(g_graph.key, key, sizeof(key)) == 0;
@@ -643,21 +1236,17 @@ static bool mdp_device_ready(const nova::MDP *mdp)
            mdp->d_T != nullptr && mdp->d_R != nullptr;
 }
 
-} // namespace nova_amd
-
-namespace nova {
-
-using namespace nova_amd;
-
-int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
+// mdp_vi_initialize_gpu's work: V, V' (from VInitial or zeros) and pi on the device, and, when
+// layout, the sweep layout the model's size calls for.
+static int vi_state_init(const nova::MDP *mdp, nova::MDPVIGPU *vi, const char *fn,
+                         bool layout)
 {
-    NOVA_AMD_API_LOCK();
-    const char *fn = "mdp_vi_initialize_gpu";
     if (mdp == nullptr || mdp->n == 0 || vi == nullptr) {
         NOVA_ERR(fn, "Invalid arguments.");
         return NOVA_ERROR_INVALID_DATA;
     }
     sorted_drop(vi->d_V, fn);
+    chunked_drop(vi->d_V, fn);
     device_free_if_owned(vi->d_V, fn, "the value function");
     device_free_if_owned(vi->d_VPrime, fn, "the value function (prime)");
     device_free_if_owned(vi->d_pi, fn, "the policy (pi)");
@@ -688,9 +1277,26 @@ int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
         e = hipMemset(vi->d_pi, 0, n * sizeof(unsigned));
     rc = hip_fail(e, fn, "Failed to copy memory from host to device for V/pi.",
                   NOVA_ERROR_MEMCPY_TO_DEVICE);
-    if (rc == NOVA_SUCCESS && mdp->d_S != nullptr && mdp->d_T != nullptr && sorted_wanted(mdp))
-        rc = sorted_build(mdp, vi->d_V, stream(), fn);
+    if (layout && rc == NOVA_SUCCESS && mdp->d_S != nullptr && mdp->d_T != nullptr) {
+        const int want = layout_wanted(mdp);
+        if (want == 2)
+            rc = sorted_build(mdp, vi->d_V, stream(), fn);
+        else if (want == 3)
+            rc = chunked_build(mdp, vi->d_V, stream(), fn);
+    }
     return rc;
+}
+
+} // namespace nova_amd
+
+namespace nova {
+
+using namespace nova_amd;
+
+int mdp_vi_initialize_gpu(const MDP *mdp, MDPVIGPU *vi)
+{
+    NOVA_AMD_API_LOCK();
+    return vi_state_init(mdp, vi, "mdp_vi_initialize_gpu", true);
 }
 
 int mdp_vi_uninitialize_gpu(const MDP *mdp, MDPVIGPU *vi)
@@ -704,6 +1310,7 @@ int mdp_vi_uninitialize_gpu(const MDP *mdp, MDPVIGPU *vi)
     vi->currentHorizon = 0;
     int rc = NOVA_SUCCESS;
     sorted_drop(vi->d_V, fn);
+    chunked_drop(vi->d_V, fn);
     if (device_free_if_owned(vi->d_V, fn, "the value function") != NOVA_SUCCESS)
         rc = NOVA_ERROR_DEVICE_FREE;
     if (device_free_if_owned(vi->d_VPrime, fn, "the value function (prime)") != NOVA_SUCCESS)
@@ -785,6 +1392,116 @@ int mdp_vi_execute_gpu(const MDP *mdp, MDPVIGPU *vi, MDPValueFunction *policy)
     return (rc != NOVA_SUCCESS) ? rc : rc2;
 }
 
+int mdp_vi_batch_execute_gpu(const MDP *const *mdps, MDPVIGPU *const *vis, unsigned int count,
+                             MDPValueFunction *policies, float *elapsedMs)
+{
+    NOVA_AMD_API_LOCK();
+    const char *fn = "mdp_vi_batch_execute_gpu";
+    if (mdps == nullptr || vis == nullptr || count == 0) {
+        NOVA_ERR(fn, "Invalid arguments.");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    for (unsigned b = 0; b < count; b++) {
+        const MDP *mdp = mdps[b];
+        if (!mdp_device_ready(mdp) || vis[b] == nullptr || mdp->horizon < 1 ||
+                mdp->m > kBatchRows || mdp->gamma < 0.0f || mdp->gamma > 1.0f) {
+            NOVA_ERR(fn, "Invalid instance (a device-resident MDP with m <= 1024, horizon >= 1, "
+                         "gamma in [0, 1], and its solver struct).");
+            return NOVA_ERROR_INVALID_DATA;
+        }
+    }
+    int rc = NOVA_SUCCESS;
+    std::vector<ViInst> host(count);
+    std::vector<unsigned> off(count + 1, 0);
+    unsigned maxH = 0;
+    for (unsigned b = 0; b < count && rc == NOVA_SUCCESS; b++) {
+        const MDP *mdp = mdps[b];
+        rc = vi_state_init(mdp, vis[b], fn, false);
+        ViInst &I = host[b];
+        I.n = mdp->n;
+        I.m = mdp->m;
+        I.ns = mdp->ns;
+        I.horizon = mdp->horizon;
+        I.statesPerTile = kBatchRows / mdp->m;
+        I.tiles = (mdp->n + I.statesPerTile - 1) / I.statesPerTile;
+        I.gamma = mdp->gamma;
+        I.S = mdp->d_S;
+        I.T = mdp->d_T;
+        I.R = mdp->d_R;
+        I.V = vis[b]->d_V;
+        I.Vp = vis[b]->d_VPrime;
+        I.pi = vis[b]->d_pi;
+        off[b + 1] = off[b] + I.tiles;
+        maxH = std::max(maxH, I.horizon);
+        if ((size_t)off[b] + I.tiles >= ((size_t)1 << 31))
+            rc = NOVA_ERROR_INVALID_DATA;
+    }
+    void *mem = nullptr;
+    const size_t bI = up256(sizeof(ViInst) * count), bO = up256(sizeof(unsigned) * (count + 1));
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc(&mem, bI + bO, fn, "the batch descriptors");
+    hipStream_t st = stream();
+    hipGraphExec_t exec = nullptr;
+    if (rc == NOVA_SUCCESS) {
+        ViInst *dInst = static_cast<ViInst *>(mem);
+        unsigned *dOff = reinterpret_cast<unsigned *>(static_cast<char *>(mem) + bI);
+        hipError_t e = hipMemcpyAsync(dInst, host.data(), sizeof(ViInst) * count,
+                                      hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dOff, off.data(), sizeof(unsigned) * (count + 1),
+                               hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+        hipStream_t cs = capture_stream();
+        hipGraph_t graph = nullptr;
+        if (e == hipSuccess && cs == nullptr)
+            e = hipErrorInvalidValue;
+        if (e == hipSuccess)
+            e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (unsigned k = 0; k < maxH; k++)
+                hipLaunchKernelGGL(vi_batch_sweep, dim3(off[count]), dim3(kBatchThreads), 0, cs,
+                                   dInst, dOff, count, k);
+            e = hipStreamEndCapture(cs, &graph);
+        }
+        if (e == hipSuccess) {
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+        }
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (e == hipSuccess && elapsedMs) {
+            hipEventCreate(&t0);
+            hipEventCreate(&t1);
+            hipEventRecord(t0, st);
+        }
+        if (e == hipSuccess)
+            e = hipGraphLaunch(exec, st);
+        if (e == hipSuccess && elapsedMs)
+            hipEventRecord(t1, st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+        if (elapsedMs) {
+            float ms = 0.0f;
+            if (e == hipSuccess && t0 && t1)
+                hipEventElapsedTime(&ms, t0, t1);
+            *elapsedMs = ms;
+        }
+        if (t0) hipEventDestroy(t0);
+        if (t1) hipEventDestroy(t1);
+        if (exec)
+            hipGraphExecDestroy(exec);
+        rc = hip_fail(e, fn, "Failed to run the batch graph.", NOVA_ERROR_KERNEL_EXECUTION);
+    }
+    if (mem)
+        device_free(mem, fn, "the batch descriptors");
+    for (unsigned b = 0; b < count && rc == NOVA_SUCCESS; b++) {
+        vis[b]->currentHorizon = mdps[b]->horizon;
+        if (policies)
+            rc = mdp_vi_get_policy_gpu(mdps[b], vis[b], &policies[b]);
+    }
+    return rc;
+}
+
 int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *kernel)
 {
     NOVA_AMD_API_LOCK();
@@ -793,7 +1510,7 @@ int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *ke
         return NOVA_ERROR_INVALID_DATA;
     }
     const ViPlan p = vi_plan(mdp, vi);
-    *kernel = p.sorted ? 2u : (p.tiled ? 1u : 0u);
+    *kernel = p.chunked ? 3u : (p.sorted ? 2u : (p.tiled ? 1u : 0u));
     return NOVA_SUCCESS;
 }
 

@@ -89,6 +89,7 @@ namespace {
 constexpr int kPB = 256;            // lanes per workgroup
 constexpr int kUMax = 512;          // max valid (k, l) pairs of one belief-action in LDS
 constexpr int kKMax = 512;          // max belief support length handled in LDS
+constexpr int kActMax = 16384;      // exact mode: max support length (the action kernel's LDS)
 constexpr int kZC = 16;             // observations per scoring chunk
 constexpr int kLdsRowFloats = 36 * 1024;  // Gamma row / h vector staged in LDS (144 KB)
 constexpr unsigned kNoHorizon = 0xffffffffu;
@@ -299,6 +300,68 @@ __device__ void build_support(SupportLds &L, unsigned i, unsigned a, unsigned m,
     __syncthreads();
 }
 
+// Fills the SupportLds with the support entries [k0, k1) of belief i under action a (k1 at most
+// the valid length, k1 - k0 <= kKMax, and their valid pairs at most kUMax): the exact mode's
+// chunks of a long support. Entry k of the chunk is support entry k0 + k.
+__device__ void build_support_range(SupportLds &L, unsigned i, unsigned a, unsigned m,
+        unsigned ns, unsigned rz, int k0, int k1, const int *__restrict__ S,
+        const int *__restrict__ Z, const float *__restrict__ B, int *cnt)
+{
+    const int t = (int)threadIdx.x, kc = k1 - k0;
+    __syncthreads();                  // the previous chunk's readers are done
+    for (int k = t; k < kc; k += kPB) {
+        const int s = Z[(size_t)i * rz + k0 + k];
+        L.sK[k] = s;
+        L.bK[k] = B[(size_t)i * rz + k0 + k];
+        const size_t row = ((size_t)s * m + a) * ns;
+        int c = 0;
+        while (c < (int)ns && S[row + c] >= 0)
+            c++;
+        cnt[k] = c;
+    }
+    if (t == 0)
+        L.kEnd = kc;
+    __syncthreads();
+    for (int k = t; k <= kc; k += kPB)
+        L.startK[k] = (k == 0) ? 0 : cnt[k - 1];
+    __syncthreads();
+    for (int off = 1; off <= kc; off <<= 1) {
+        for (int k = t; k <= kc; k += kPB)
+            cnt[k] = (k >= off) ? L.startK[k] + L.startK[k - off] : L.startK[k];
+        __syncthreads();
+        for (int k = t; k <= kc; k += kPB)
+            L.startK[k] = cnt[k];
+        __syncthreads();
+    }
+    if (t == 0)
+        L.uEnd = L.startK[kc];
+    for (int x = t; x < kc * (int)ns; x += kPB) {
+        const int k = x / (int)ns, l = x % (int)ns;
+        const int u = L.startK[k] + l;
+        if (u < L.startK[k + 1]) {
+            const size_t row = ((size_t)L.sK[k] * m + a) * ns;
+            L.spU[u] = S[row + l];
+            L.otU[u] = L.sK[k] * (int)ns + l;
+        }
+    }
+    __syncthreads();
+}
+
+// Valid support length of belief i (the first Z < 0, else rz), computed by all kPB lanes.
+__device__ int support_length(unsigned i, unsigned rz, const int *__restrict__ Z, int *slot)
+{
+    if (threadIdx.x == 0)
+        *slot = (int)rz;
+    __syncthreads();
+    for (unsigned k = threadIdx.x; k < rz; k += kPB)
+        if (Z[(size_t)i * rz + k] < 0)
+            atomicMin(slot, (int)k);
+    __syncthreads();
+    const int v = *slot;
+    __syncthreads();
+    return v;
+}
+
 template <bool EXACT, int JPT>
 __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
@@ -325,7 +388,17 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
     int kEnd, uEnd;
-    if (EXACT) {
+    // exact mode with a support whose pairs pass the LDS: its entries are staged in chunks of
+    // kc entries (at most kUMax pairs each) inside the observation loop, the sums over the
+    // chunks kept in order
+    const bool chunked = EXACT && (P.rz > (unsigned)kKMax || P.rz * ns > (unsigned)kUMax);
+    int kFull = 0, kc = 0;
+    if (EXACT && chunked) {
+        kFull = support_length(i, P.rz, P.Z, &scratch[0]);
+        kc = min(kKMax, kUMax / (int)ns);
+        kEnd = 0;
+        uEnd = 0;
+    } else if (EXACT) {
         build_support(L, i, a, m, ns, P.rz, P.S, P.Z, P.B, scratch);
         kEnd = L.kEnd;
         uEnd = L.uEnd;
@@ -352,6 +425,22 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
 
     for (unsigned o0 = 0; o0 < z; o0 += kZC) {
         const int zc = min((int)kZC, (int)(z - o0));
+        float val[kZC][JPT];
+#pragma unroll
+        for (int oo = 0; oo < kZC; oo++)
+#pragma unroll
+            for (int q = 0; q < JPT; q++)
+                val[oo][q] = 0.0f;
+        // support chunks (one unless chunked)
+        for (int k0 = 0; k0 < (chunked ? kFull : 1); k0 += (chunked ? kc : 1)) {
+        if (chunked) {
+            build_support_range(L, i, a, m, ns, P.rz, k0, min(kFull, k0 + kc), P.S, P.Z, P.B,
+                                scratch);
+            kEnd = L.kEnd;
+            uEnd = L.uEnd;
+        } else {
+            __syncthreads();          // the previous observation chunk's readers of w are done
+        }
         // stage the chunk's factors: exact -> O*T; fast -> (gamma*b_k*T) * O
         for (int x = t; x < uEnd * kZC; x += kPB) {
             const int u = x / kZC, oo = x % kZC;
@@ -366,13 +455,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
         }
         __syncthreads();
 
-        float val[kZC][JPT];
         float vtk[EXACT ? kZC : 1][JPT];
-#pragma unroll
-        for (int oo = 0; oo < kZC; oo++)
-#pragma unroll
-            for (int q = 0; q < JPT; q++)
-                val[oo][q] = 0.0f;
 
         // exact: per support entry k, its pairs (rounded as the reference); fast: all pairs
         const int kLoops = EXACT ? kEnd : (uEnd > 0 ? 1 : 0);
@@ -416,6 +499,7 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
                     }
             }
         }
+        }   // support chunks
 
         // per lane: first maximiser over its alphas (visited in increasing j)
         float bv[kZC];
@@ -623,6 +707,153 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     }
 }
 
+// The same scoring for instances with at most 16 observations, on v_mfma_f32_16x16x4_f32:
+// rows are 16 observations (the 32-row kernel above would multiply half its rows by zero),
+// columns 16 alphas per block, the reduction over the pairs four at a time. Each of the 8 waves
+// owns 64 alphas of the kMTile tile as four 16 x 16 blocks. Operand layout of the instruction:
+// lane l supplies A[l % 16][l / 16] (observation, pair) and B[l / 16][l % 16] (pair, alpha),
+// and holds D[4 * (l / 16) + v][l % 16] in element v of its accumulator.
+constexpr int kW16Chunk = 256;       // support pairs per staged W chunk
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+template <int PS>
+__global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter)
+{
+    constexpr int NW = 8, NB = 4, NT = NW * 64;
+    static_assert(NW * NB * 16 == kMTile, "a workgroup covers one alpha tile");
+    constexpr int STEP = 8 * PS;              // pairs per loop step (two trips of PS MFMAs)
+    static_assert(kW16Chunk % STEP == 0, "whole loop steps per chunk");
+    __shared__ int spU[kW16Chunk];
+    __shared__ float w[kW16Chunk * 16];       // [u][o] for the current chunk, 16 observations
+    __shared__ float redV[NW][16];
+    __shared__ int redJ[NW][16];
+
+    if (blockIdx.x >= off[count])
+        return;   // a grid larger than the plan's workgroups does nothing
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, m = P.m, z = P.z, r = P.r;
+    const unsigned local = blockIdx.x - off[b];
+    const unsigned rb = P.rb;
+    const unsigned ia = local % (rb * m);
+    const unsigned tile = local / (rb * m);
+    const unsigned a = ia % m;
+    const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
+
+    const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
+    const int uEndP = (uEnd + STEP - 1) / STEP * STEP;
+    const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
+    const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
+    const unsigned jw = tile * kMTile + wave * NB * 16;
+    const unsigned jl = jw + lane % 16;
+    const size_t rS = P.rS;
+
+    for (unsigned o0 = 0; o0 < z; o0 += 16) {
+        const int zc = min(16, (int)(z - o0));
+        floatx4_t acc[NB];
+#pragma unroll
+        for (int y = 0; y < NB; y++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                acc[y][v] = 0.0f;
+        for (int c0 = 0; c0 < uEnd; c0 += kW16Chunk) {
+            const int cPad = min(kW16Chunk, uEndP - c0);     // a multiple of STEP
+            __syncthreads();
+            for (int u = t; u < cPad; u += NT)
+                spU[u] = (c0 + u < uEnd) ? pSp[c0 + u] : 0;
+            __syncthreads();
+            for (int x = t; x < cPad * 16; x += NT) {
+                const int u = x / 16, oo = x % 16;
+                float v = 0.0f;
+                if (c0 + u < uEnd && oo < zc)
+                    v = pw[c0 + u] * Oa[(size_t)spU[u] * z + o0 + oo];
+                w[x] = v;
+            }
+            __syncthreads();
+            float fb[PS][NB], nb[PS][NB];
+            auto loadB = [&](int u0, float (&dst)[PS][NB]) {
+#pragma unroll
+                for (int q = 0; q < PS; q++) {
+                    const int uu = u0 + 4 * q + (int)(lane / 16);
+                    const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+#pragma unroll
+                    for (int y = 0; y < NB; y++)
+                        dst[q][y] = row[y * 16];
+                }
+            };
+            auto mfmaTrip = [&](int u0, const float (&bq)[PS][NB]) {
+                float fa[PS];
+#pragma unroll
+                for (int q = 0; q < PS; q++)
+                    fa[q] = w[(u0 + 4 * q + (int)(lane / 16)) * 16 + lane % 16];
+#pragma unroll
+                for (int q = 0; q < PS; q++)
+#pragma unroll
+                    for (int y = 0; y < NB; y++)
+                        acc[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[q], bq[q][y], acc[y],
+                                                                      0, 0, 0);
+            };
+            loadB(0, fb);
+            for (int u0 = 0; u0 < cPad; u0 += STEP) {
+                loadB(u0 + STEP / 2, nb);
+                __builtin_amdgcn_sched_barrier(0);
+                mfmaTrip(u0, fb);
+                __builtin_amdgcn_sched_barrier(0);
+                loadB(min(u0 + STEP, cPad - STEP / 2), fb);
+                __builtin_amdgcn_sched_barrier(0);
+                mfmaTrip(u0 + STEP / 2, nb);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // per row (observation): first maximiser over the wave's alphas, then across the waves
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float bv = kLow;
+            int bj = INT_MAX;
+#pragma unroll
+            for (int y = 0; y < NB; y++) {
+                const unsigned j = jw + y * 16 + lane % 16;
+                if (j < r && acc[y][v] > bv) {
+                    bv = acc[y][v];
+                    bj = (int)j;
+                }
+            }
+            for (int off2 = 8; off2 >= 1; off2 >>= 1) {
+                const float ov = __shfl_xor(bv, off2, 16);
+                const int oj = __shfl_xor(bj, off2, 16);
+                if (better(ov, oj, bv, bj)) {
+                    bv = ov;
+                    bj = oj;
+                }
+            }
+            if (lane % 16 == 0) {
+                const int o = 4 * (int)(lane / 16) + v;
+                redV[wave][o] = bv;
+                redJ[wave][o] = bj;
+            }
+        }
+        __syncthreads();
+        if (t < (unsigned)zc) {
+            float v = redV[0][t];
+            int j = redJ[0][t];
+            for (int wv = 1; wv < NW; wv++)
+                if (better(redV[wv][t], redJ[wv][t], v, j)) {
+                    v = redV[wv][t];
+                    j = redJ[wv][t];
+                }
+            const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
+            P.partVal[idx] = v;
+            P.partJ[idx] = j;
+        }
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------------------- 3. action
 
 // EXACT: each action's alpha is formed on the belief's support in the reference's rounding order
@@ -634,9 +865,10 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
 {
     __shared__ int js[kZC * 8];
     __shared__ float bestO[kZC * 8];
-    __shared__ float alphaK[kKMax];
-    __shared__ float bS[kKMax];
     __shared__ int kEndS;
+    // alphaK and bS: one float per support entry each (exact mode; fast mode uses alphaK[0]),
+    // dynamic LDS of 2 * max(kKMax, the longest exact support) floats (plan action_lds)
+    extern __shared__ float actLds[];
     if (blockIdx.x >= off[count])
         return;   // a grid larger than the plan's workgroups does nothing
     const unsigned b = find_inst(off, count, blockIdx.x);
@@ -647,12 +879,15 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
     const float gamma = P.gamma;
     const float *__restrict__ G = src_gamma(P, iter, flip);
     const unsigned i = blockIdx.x - off[b], t = threadIdx.x;
+    const unsigned kCap = EXACT ? max(rz, (unsigned)kKMax) : 1u;
+    float *alphaK = actLds;
+    float *bS = actLds + kCap;
     if (t == 0)
         kEndS = (int)rz;
     __syncthreads();
     for (unsigned k = t; k < rz; k += kPB) {
         if (EXACT)
-            bS[k] = P.B[(size_t)i * rz + k];   // exact mode runs only with rz <= kKMax
+            bS[k] = P.B[(size_t)i * rz + k];   // rz <= kActMax in exact mode
         if (P.Z[(size_t)i * rz + k] < 0)
             atomicMin(&kEndS, (int)k);
     }
@@ -1123,17 +1358,26 @@ static bool dense_mode(int mode) { return mode == 2 || mode == 3; }
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Scoring variants: jpt = 1, 2, 4 alphas per lane on the vector ALUs (tiles of 256 * jpt alphas),
-// jpt = 8, 16, 32 the matrix-core kernel of fast mode (tiles of kMTile alphas).
+// jpt = 8, 16, 32 the matrix-core kernel of fast mode, jpt = 64 its 16-observation-row form
+// (tiles of kMTile alphas).
 static unsigned tile_width(unsigned jpt) { return jpt >= 8 ? (unsigned)kMTile : kPB * jpt; }
 static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
 // Row stride of the transposed Gamma: r rounded up to the matrix-core tile.
 static unsigned gt_stride(unsigned r) { return (r + kMTile - 1) / kMTile * kMTile; }
 
-// Supports the LDS-staged kernels take: exact mode and the vector-ALU scoring variants.
+// Supports exact mode takes: its scorer stages the support in chunks of at most kUMax pairs
+// (so ns <= kUMax), and its action kernel keeps one alpha value per support entry in LDS.
 static bool pbvi_shape_ok(const nova::POMDP *p)
 {
-    return p->rz <= (unsigned)kKMax && (size_t)p->rz * p->ns <= (size_t)kUMax;
+    return p->rz <= (unsigned)kActMax && p->ns <= (unsigned)kUMax;
+}
+
+// Dynamic LDS of pbvi_action for an instance: alphaK and bS over its support (exact mode), or
+// the running action value (fast mode).
+static size_t action_lds(const PbviInst &I, int mode)
+{
+    return mode == 1 ? 2 * sizeof(float) * std::max<size_t>(I.rz, kKMax) : 2 * sizeof(float);
 }
 
 // Whether the fast-mode pair lists (r*m*rz*ns entries of 8 bytes) are built: always, unless they
@@ -1284,9 +1528,9 @@ static bool fused_fits(const PbviInst &I, int mode)
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
 struct DenseWs;
 
-// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32); PbviPlan::totSc and the offset
+// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32, 64); PbviPlan::totSc and the offset
 // table hold one entry per variant.
-constexpr int kVariants = 6;
+constexpr int kVariants = 7;
 
 struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
@@ -1296,6 +1540,7 @@ struct PbviPlan {
     unsigned totTr = 0, totSc[kVariants] = {}, totBe = 0, totFu = 0;
     size_t fuLds = 0;                // dynamic LDS of pbvi_fused: the largest fused instance's
     size_t ldsBytes = 0;
+    size_t actLds = 2 * sizeof(float) * kKMax;   // dynamic LDS of pbvi_action
     unsigned maxHorizon = 0;
     std::vector<PbviInst> host;
 };
@@ -1305,7 +1550,8 @@ static_assert(sizeof(PbviPlan::totSc) == kVariants * sizeof(unsigned),
 
 static int jpt_slot(unsigned jpt)
 {
-    return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : (jpt == 16 ? 4 : 5))));
+    return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : (jpt == 16 ? 4 :
+           (jpt == 32 ? 5 : 6)))));
 }
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
@@ -1323,6 +1569,7 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
         if (jpt == 8) hipLaunchKernelGGL((pbvi_score_mfma<4, 4, 4>), g, dim3(256), 0, st, dInst, sc, c, iter);
         if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 4>), g, dim3(512), 0, st, dInst, sc, c, iter);
         if (jpt == 32) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 8>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 64) hipLaunchKernelGGL((pbvi_score_mfma16<2>), g, dim3(512), 0, st, dInst, sc, c, iter);
     }
 }
 
@@ -1360,6 +1607,8 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
             pl.fuLds = std::max(pl.fuLds, sizeof(float) * (size_t)fused_stage_words(I));
         if (I.useLds && four)
             pl.ldsBytes = std::max(pl.ldsBytes, sizeof(float) * (size_t)I.n);
+        if (four)
+            pl.actLds = std::max(pl.actLds, action_lds(I, 1));
         if (I.horizon != kNoHorizon)
             pl.maxHorizon = std::max(pl.maxHorizon, I.horizon);
     }
@@ -1368,8 +1617,13 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
         pl.totSc[v] = scs[v][c];
     pl.totBe = be[c];
     pl.totFu = fu[c];
-    hipError_t ea = hipSuccess;
-    if (pl.totFu > 0)
+    hipError_t ea = hipFuncSetAttribute((const void *)pbvi_action<true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)pl.actLds);
+    if (ea == hipSuccess)
+        ea = hipFuncSetAttribute((const void *)pbvi_action<false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.actLds);
+    if (ea == hipSuccess && pl.totFu > 0)
         ea = hipFuncSetAttribute((const void *)pbvi_fused,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 4 * kFStage);
     if (ea != hipSuccess)
@@ -1505,7 +1759,7 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     if (pl.totBe == 0)
         return;
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32};
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64};
     for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
@@ -1513,9 +1767,9 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
                      iter, st);
     }
     if (mode == 1)
-        hipLaunchKernelGGL(pbvi_action<true>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
+        hipLaunchKernelGGL(pbvi_action<true>, dim3(pl.totBe), dim3(kPB), pl.actLds, st, pl.dInst, be, c, iter, flip);
     else
-        hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, iter, flip);
+        hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), pl.actLds, st, pl.dInst, be, c, iter, flip);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
     hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kMB), pl.ldsBytes, st, pl.dInst, be, c, iter,
                        flip);
@@ -1580,11 +1834,13 @@ static unsigned bucket(unsigned x)
 static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
 {
     // the vector-ALU variants stage the support in LDS; longer supports take the matrix cores
-    const bool valuOk = I.rz <= (unsigned)kKMax && I.rz * I.ns <= (unsigned)kUMax;
+    const bool valuOk = mode == 1 ||
+                        (I.rz <= (unsigned)kKMax && I.rz * I.ns <= (unsigned)kUMax);
     if (const char *f = std::getenv("NOVA_AMD_PBVI_JPT")) {
         const int v = std::atoi(f);
         if (((v == 1 || v == 2 || v == 4) && valuOk) ||
-                ((v == 8 || v == 16 || v == 32) && mode != 1))
+                ((v == 8 || v == 16 || v == 32) && mode != 1) ||
+                (v == 64 && mode != 1 && I.z <= 16))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
@@ -1599,9 +1855,11 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u}) {
+    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) {
         if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
+        if (jpt == 64 && I.z > 16)
+            continue;                          // the 16-row kernel: at most 16 observations
         if (jpt < 8 && !valuOk)
             continue;
         PbviPlan pl;
@@ -2014,7 +2272,7 @@ int pomdp_pbvi_initialize_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi)
     }
     if (!dense_mode(g_pbvi_mode) && !sparse_shape_ok(pomdp, g_pbvi_mode)) {
         NOVA_ERR(fn, g_pbvi_mode == 1
-                 ? "Belief supports too long for exact mode (rz <= 512 and rz*ns <= 512); "
+                 ? "Belief supports too long for exact mode (rz <= 16,384 and ns <= 512); "
                    "fast mode takes any support length."
                  : "r*m*rz*ns above 2^30: beliefs with full support use the dense modes 2/3.");
         return NOVA_ERROR_INVALID_DATA;
@@ -2315,9 +2573,9 @@ int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, float *stageM
     launch_score(pl.host[0].jpt, mode, dim3(pl.totSc[v]), pl.dInst, sc, c, 0u, st);
     hipEventRecord(ev[2], st);
     if (mode == 1)
-        hipLaunchKernelGGL(pbvi_action<true>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
+        hipLaunchKernelGGL(pbvi_action<true>, dim3(pl.totBe), dim3(kPB), pl.actLds, st, pl.dInst, be, c, 0u, flip);
     else
-        hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), 0, st, pl.dInst, be, c, 0u, flip);
+        hipLaunchKernelGGL(pbvi_action<false>, dim3(pl.totBe), dim3(kPB), pl.actLds, st, pl.dInst, be, c, 0u, flip);
     hipEventRecord(ev[3], st);
     auto matFn = (mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
     hipLaunchKernelGGL(matFn, dim3(pl.totBe), dim3(kMB), pl.ldsBytes, st, pl.dInst, be, c, 0u, flip);
@@ -2661,9 +2919,9 @@ int pomdp_perseus_execute_gpu(const POMDP *pomdp, POMDPPerseusCPU *per, POMDPAlp
             const unsigned *sc = pl.dOff + (1 + v) * (c + 1), *be = pl.dOff + (kVariants + 1) * (c + 1);
             launch_score(J.jpt, g_pbvi_mode, dim3(pl.totSc[v]), pl.dInst, sc, c, 0u, st);
             if (g_pbvi_mode == 1)
-                hipLaunchKernelGGL(pbvi_action<true>, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
+                hipLaunchKernelGGL(pbvi_action<true>, dim3(1), dim3(kPB), pl.actLds, st, pl.dInst, be, c, 0u, 0u);
             else
-                hipLaunchKernelGGL(pbvi_action<false>, dim3(1), dim3(kPB), 0, st, pl.dInst, be, c, 0u, 0u);
+                hipLaunchKernelGGL(pbvi_action<false>, dim3(1), dim3(kPB), pl.actLds, st, pl.dInst, be, c, 0u, 0u);
             auto matFn = (g_pbvi_mode == 1) ? pbvi_materialize<true> : pbvi_materialize<false>;
             hipLaunchKernelGGL(matFn, dim3(1), dim3(kMB), pl.ldsBytes, st, pl.dInst, be, c, 0u, 0u);
         }

@@ -95,7 +95,8 @@ class MDPValueIterationGPU(_abi.NovaMDPVIGPU):
         return ms.value
 
     def sweep_kernel(self):
-        """0: one thread per state, 1: wave-tiled sweep, 2: successor-sorted tiles."""
+        """0: one thread per state, 1: wave-tiled sweep, 2: successor-sorted tiles,
+        3: successor chunks (V chunks in LDS, products through HBM)."""
         k = ct.c_uint(0)
         if self._lib.mdp_vi_sweep_kernel_gpu(self.mdpPtr, self, ct.byref(k)) != 0:
             raise Exception("mdp_vi_sweep_kernel_gpu failed")
@@ -105,3 +106,32 @@ class MDPValueIterationGPU(_abi.NovaMDPVIGPU):
         """(V, pi) read back from the device without building a policy."""
         pol = self.get_policy()
         return pol.V_array(), pol.pi_array()
+
+
+def batch_execute(solvers, policies=True):
+    """Runs mdp_vi_batch_execute_gpu over MDPValueIterationGPU solvers (their MDPs resident on
+    the device): every instance solved as its plain solve would be, all sweeps in one graph.
+    Returns (list of (V, pi) or None, device ms)."""
+    lib = _abi.load()
+    count = len(solvers)
+    MP = ct.POINTER(_abi.NovaMDP) * count
+    VP = ct.POINTER(_abi.NovaMDPVIGPU) * count
+    marr = MP(*[ct.cast(s.mdpPtr, ct.POINTER(_abi.NovaMDP)) for s in solvers])
+    varr = VP(*[ct.cast(ct.pointer(s), ct.POINTER(_abi.NovaMDPVIGPU)) for s in solvers])
+    pols = (_abi.NovaMDPValueFunction * count)() if policies else None
+    ms = ct.c_float(0.0)
+    rc = lib.mdp_vi_batch_execute_gpu(ct.cast(marr, ct.c_void_p), ct.cast(varr, ct.c_void_p),
+                                      count, ct.cast(pols, ct.c_void_p) if pols else None,
+                                      ct.byref(ms))
+    if rc != 0:
+        raise Exception("mdp_vi_batch_execute_gpu failed with code %d" % rc)
+    out = None
+    if pols is not None:
+        out = []
+        for k in range(count):
+            p = pols[k]
+            V = np.ctypeslib.as_array(p.V, (p.n,)).copy()
+            pi = np.ctypeslib.as_array(p.pi, (p.n,)).copy()
+            lib.mdp_value_function_uninitialize(ct.byref(p))
+            out.append((V, pi))
+    return out, ms.value

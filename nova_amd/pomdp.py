@@ -176,11 +176,16 @@ class POMDP(_abi.NovaPOMDP):
             raise Exception("Belief expansion '%s' failed with code %d." % (method, rc))
         self._refresh_beliefs_gpu()
 
-    def sigma_approximate(self, numDesiredNonZeroValues):
+    def sigma_approximate(self, numDesiredNonZeroValues, gpu=False):
         """Keeps each belief's numDesiredNonZeroValues largest entries, renormalised
-        (pomdp_sigma_cpu); returns sigma, the smallest kept mass. A resident POMDP's device
-        copy of Z/B is refreshed."""
+        (pomdp_sigma_cpu, or pomdp_sigma_gpu on the device copy with gpu=True); returns sigma,
+        the smallest kept mass. Both the host and the device copies of Z/B end up updated."""
         sig = ct.c_float()
+        if gpu:
+            rc = self._lib.pomdp_sigma_gpu(self, int(numDesiredNonZeroValues), ct.byref(sig))
+            if rc != 0:
+                raise Exception("pomdp_sigma_gpu failed with code %d" % rc)
+            return sig.value
         rc = self._lib.pomdp_sigma_cpu(self, int(numDesiredNonZeroValues), ct.byref(sig))
         if rc != 0:
             raise Exception("pomdp_sigma_cpu failed with code %d" % rc)

@@ -116,3 +116,19 @@ def batch_instances(count=256, seed=4, n_range=(500, 8000), r_range=(256, 2048),
         out.append(pomdp_arrays(n=n, m=m, z=z, ns=ns, r=r, rz=min(rz, n), gamma=gamma,
                                 horizon=horizon, seed=(seed << 20) + k))
     return out
+
+
+def mdp_batch_instances(count=256, seed=5, n_range=(500, 8000), m_choices=range(4, 17),
+                        ns_choices=(4, 8), gamma=0.95, horizon=50):
+    """The uneven MDP batch (the batched-solve workload's MDP counterpart): per instance n
+    log-uniform, m and ns uniform over their choices, T Dirichlet(1), R uniform[-1, 1]; each
+    instance drawn by its own generator seeded (seed, index)."""
+    meta = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        n = int(round(np.exp(meta.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        m = int(meta.choice(list(m_choices)))
+        ns = int(meta.choice(list(ns_choices)))
+        out.append(mdp_arrays(n=n, m=m, ns=min(ns, n), gamma=gamma, horizon=horizon,
+                              seed=(seed << 20) + k))
+    return out

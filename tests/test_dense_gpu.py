@@ -1,10 +1,12 @@
 """Dense-belief PBVI scoring on the matrix cores (modes 2 = bf16 operands, 3 = fp32 operands,
 both fp32 accumulation) against the float oracle from the same Gamma (one backup): with fp32
-operands the chosen alphas must match except at near-ties; bf16 operands are reported as a
-disagreement rate and a value error, and must keep the beliefs' values within 1e-2 relative."""
+operands the chosen alphas must match except at near-ties (conftest.tie_rel with K = n) and
+the beliefs' values agree within 1e-4 relative; bf16 operands are reported as a disagreement
+rate and a value error, and must keep the beliefs' values within 1e-2 relative."""
 import numpy as np
 import pytest
 
+from conftest import check_choices
 from nova_amd import synth
 from nova_amd.pomdp import POMDP
 from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, set_mode
@@ -49,9 +51,12 @@ def test_dense_step_vs_oracle(oracle, mode, label):
     print("dense %s: alpha-choice disagreement %.4f, action disagreement %.4f, value error %.3g"
           % (label, rate, np.mean(act != ref["pi"].astype(np.int32)), verr))
     if mode == 3:
-        assert not np.any(bad & (ref["gap"] > 1e-4 * scale)), "fp32 choices differ beyond near-ties"
+        # the score sums K = n products per (belief, action, observation) in the GEMM
+        check_choices(js, jo, ref["gap"], scale, P["S"].shape[0], "dense fp32 alpha choices")
         assert verr < 1e-4
     else:
+        # bf16 operands carry about 3 significant digits: reported, and held to 1e-2; the
+        # measured value error at this shape is about 3e-4
         assert verr < 1e-2
 
 

@@ -2,28 +2,28 @@
 after a random expansion of 2^k - 1 beliefs), loaded by the .pomdp loader, expanded on the GPU
 with a fixed seed, and solved on the GPU against the oracle on the same expanded belief set.
 
-exact mode: Gamma and pi equal the oracle's float twin bit for bit (domains whose supports fit
-            exact mode: rz <= 512 and rz*ns <= 512).
+exact mode: Gamma and pi equal the oracle's float twin bit for bit, on every domain (the long
+            supports of hallway2 and tag staged in chunks).
 fast mode : every domain, including those with long supports (hallway2: rz*ns up to 92*88,
             tag: rz up to 870), which only the matrix-core scorer takes. One iteration from
-            the GPU's Gamma_t chooses the oracle's alphas except at near-ties (gap below
-            1e-5 * max(1, |action value|)) and the new alphas match; every step of the solve
-            path is a backup of its previous Gamma up to near-ties.
+            the GPU's Gamma_t chooses the oracle's alphas except at near-ties (the rule of
+            conftest.tie_rel with K = rz * ns) and the new alphas match; every step of the
+            solve path is a backup of its previous Gamma, the belief values within 1e-4
+            relative of the oracle's; whole runs stay within 1e-4 relative of FP64 wherever
+            nova's own float arithmetic does.
 """
 import os
 
 import numpy as np
 import pytest
 
-from conftest import bits
+from conftest import bits, check_choices
 from nova_amd.pomdp import POMDP
 from nova_amd.pomdp_pbvi import EXACT, FAST, POMDPPBVIGPU, set_mode
 
 pytestmark = pytest.mark.gpu
 
 DOMAINS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "domains")
-FAST_TIE_REL = 1e-5
-
 # name, expansion steps k (2^k - 1 beliefs added), horizon of the comparison
 SMALL = [("tiger_95", 4, 50), ("4x3_95", 5, 50), ("paint_95", 4, 50), ("shuttle_95", 4, 50),
          ("tiger_grid", 6, 50), ("aloha_10", 6, 50), ("drive_san_francisco", 8, 50)]
@@ -94,21 +94,23 @@ def test_exact_bitwise_on_domain(oracle, name, k, horizon):
           % (name, P["Z"].shape[0], P["Z"].shape[1], horizon), flush=True)
 
 
-@pytest.mark.parametrize("name,k,horizon", LONG, ids=[d[0] for d in LONG])
-def test_exact_mode_refuses_long_supports(name, k, horizon):
+@pytest.mark.parametrize("name,k,horizon", [("hallway2", 6, 12), ("tag", 6, 4)],
+                         ids=["hallway2", "tag"])
+def test_exact_mode_long_supports_bitwise(oracle, name, k, horizon):
+    """Exact mode on the domains whose supports pass the LDS (hallway2: rz * ns up to 92 * 88;
+    tag: rz up to 870): the scorer stages each support in chunks and the action kernel keeps
+    every entry's alpha value, in nova's order, so Gamma and pi equal the oracle's bit for
+    bit."""
     P = expanded(name, k, horizon)
     assert P["Z"].shape[1] * P["S"].shape[2] > 512
-    set_mode(EXACT)
-    try:
-        p = POMDP.from_dict(P)
-        p.initialize_gpu()
-        with pytest.raises(Exception):
-            POMDPPBVIGPU(p, GammaInitial=P["Gamma0"])
-    finally:
-        set_mode(FAST)
+    G, pi = gpu_solve(P, EXACT)
+    Go, pio = oracle.pbvi_run(P)
+    assert np.array_equal(bits(G), bits(Go)) and np.array_equal(pi, pio)
+    print("%s: exact mode over %d iterations (rz=%d, ns=%d) bit-identical to the oracle"
+          % (name, horizon, P["Z"].shape[1], P["S"].shape[2]), flush=True)
 
 
-@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32"])
+@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64"])
 @pytest.mark.parametrize("name,k,horizon", LONG + SMALL[4:6],
                          ids=[d[0] for d in LONG + SMALL[4:6]])
 def test_fast_step_on_domain(oracle, monkeypatch, name, k, horizon, variant):
@@ -133,9 +135,9 @@ def test_fast_step_on_domain(oracle, monkeypatch, name, k, horizon, variant):
         G1 = s.get_policy().Gamma_array()
         del s
         jo, gap = ref["jstar"], ref["gap"]
-        tie = gap <= FAST_TIE_REL * np.maximum(1.0, np.abs(ref["aval"]))[:, :, None]
         bad = js != jo
-        assert not np.any(bad & ~tie), "alpha choice differs outside near-ties (t=%d)" % t
+        check_choices(js, jo, gap, np.abs(ref["aval"])[:, :, None],
+                      P["Z"].shape[1] * P["S"].shape[2], "%s t=%d alpha choices" % (name, t))
         mismatch += int(bad.sum())
         total += bad.size
         ok = ~bad.any(axis=(1, 2)) & (act == ref["pi"].astype(np.int32))
@@ -151,10 +153,11 @@ def test_fast_step_on_domain(oracle, monkeypatch, name, k, horizon, variant):
 def test_fast_run_on_domain(oracle, name, k, horizon):
     """Steps of the solve path checked against the oracle's backup of the GPU's own previous
     Gamma (solves of t and t + 1 iterations): every belief's new value b_i . alpha_i equals
-    the oracle's best action value within (z + 1) * 1e-5 * max(1, |value|): a near-tied
-    choice moves each observation's term by at most its gap. The drift of the whole run from FP64 is reported beside nova's own float arithmetic's:
-    rounding flips exactly tied alphas (tiger_grid has many) and the point-based runs part
-    ways afterwards, so the drift has no fixed bound."""
+    the oracle's best action value within 1e-4 * max(1, |value|). The drift of the whole run
+    from FP64 is asserted within 1e-4 relative where nova's own float arithmetic stays within
+    1e-5; elsewhere it is reported beside nova's: rounding flips exactly tied alphas
+    (tiger_grid has many) and the point-based runs part ways afterwards, so nova's own drift
+    has no fixed bound there."""
     P = expanded(name, k, horizon)
     z = P["O"].shape[2]
     Z, B = P["Z"], P["B"]
@@ -167,7 +170,7 @@ def test_fast_run_on_domain(oracle, name, k, horizon):
         for i in range(Z.shape[0]):
             kk = np.argmax(Z[i] < 0) if (Z[i] < 0).any() else Z.shape[1]
             v = float(G1[i, Z[i, :kk]].astype(np.float64) @ B[i, :kk].astype(np.float64))
-            assert abs(v - best[i]) <= (z + 1) * 1e-5 * scale[i], \
+            assert abs(v - best[i]) <= 1e-4 * scale[i], \
                 "belief %d, step %d: value %.9g, oracle %.9g" % (i, t, v, best[i])
         print("%s: step %d of the solve path agrees with the oracle's backup" % (name, t),
               flush=True)
@@ -176,9 +179,11 @@ def test_fast_run_on_domain(oracle, name, k, horizon):
     G32, _ = oracle.pbvi_run(P)
     v, v64, v32 = values(P, G), values(P, G64), values(P, G32)
     sc = max(1.0, np.max(np.abs(v64)))
+    err, err_nova = np.max(np.abs(v - v64)) / sc, np.max(np.abs(v32 - v64)) / sc
     print("%s: whole run of %d iterations, max_b |V(b) - V_fp64(b)| / ||V|| = %.3g (nova's "
-          "float arithmetic: %.3g)" % (name, horizon, np.max(np.abs(v - v64)) / sc,
-                                       np.max(np.abs(v32 - v64)) / sc), flush=True)
+          "float arithmetic: %.3g)" % (name, horizon, err, err_nova), flush=True)
+    if err_nova <= 1e-5:
+        assert err <= 1e-4
 
 
 RAW = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "raw")

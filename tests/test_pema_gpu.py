@@ -1,6 +1,7 @@
 """pomdp_expand_pema_gpu against the library's host PEMA (itself bit for bit nova's build in
-nova's alpha choice, and pinned by an FP64 restatement in both choices: test_pema_sigma.py):
-the appended beliefs must be equal bit for bit, in both alpha choices, over repeated calls."""
+nova's alpha choice, and pinned by an FP64 restatement in both choices: test_pema_sigma.py)
+and, in nova's alpha choice, directly against nova's own build (oracle/_ref): the appended
+beliefs must be equal bit for bit, in both alpha choices, over repeated calls."""
 import ctypes as ct
 import os
 
@@ -79,3 +80,19 @@ def test_pema_gpu_tiger_and_tiger_grid(oracle):
             Zg, Bg = run(P, G, pi, 4 if name == "tiger_95" else 2, mode, gpu=True)
             assert np.array_equal(Zh, Zg) and np.array_equal(bits(Bh), bits(Bg)), (name, mode)
         print("%s: GPU PEMA equals the host's in both alpha choices" % name, flush=True)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_pema_gpu_bitwise_vs_nova_build(reflib, case):
+    """nova's alpha choice (mode 0): the GPU PEMA appends the beliefs nova's own build
+    (oracle/_ref, pomdp_expand_pema_cpu, src/pomdp/utilities/pomdp_expand_cpu.cpp:269-466)
+    appends, bit for bit, over two calls."""
+    c = CASES[case]
+    P = synth.pomdp_arrays(gamma=0.9, horizon=5, **c)
+    rng = np.random.default_rng(c["seed"] + 10)
+    G = rng.uniform(-5, 5, (7, c["n"])).astype(np.float32)
+    pi = rng.integers(0, c["m"], 7).astype(np.uint32)
+    Zg, Bg = run(P, G, pi, 2, 0, gpu=True)
+    Zr, Br = reflib.expand_pema(P, G, pi, steps=2)
+    assert np.array_equal(Zg, Zr) and np.array_equal(bits(Bg), bits(Br))
+    print("PEMA case %d: GPU equals nova's build over two calls" % case, flush=True)

@@ -213,3 +213,27 @@ def test_belief_update_batch_long_lists_and_global_belief(oracle):
         rc, bp = oracle.belief_update(P, B[i], int(a[i]), int(o[i]))
         assert status[i] == rc and np.array_equal(bits(Bp[i]), bits(bp)), i
     print("belief update batch: n=30000, skewed lists, bit-identical", flush=True)
+
+
+@pytest.mark.parametrize("n,m,z,ns,k", [(65536, 2, 3, 4, 6), (65537, 2, 3, 4, 6),
+                                        (8, 4097, 2, 2, 40)],
+                         ids=["n65536_16bit_lists", "n65537_32bit_lists", "m4097_unordered"])
+def test_belief_update_batch_index_width_and_order_bounds(oracle, n, m, z, ns, k):
+    """The boundaries of the batched belief update's internal forms, bit for bit against the
+    oracle's float twin: predecessor lists with 16-bit indices (n <= 65,536) and 32-bit ones
+    (n = 65,537), and triples run in call order instead of grouped by action (m > 4,096)."""
+    P = synth.pomdp_arrays(n=n, m=m, z=z, ns=ns, r=2, rz=2, seed=n + m)
+    rng = np.random.default_rng(n + m)
+    B = rng.exponential(size=(k, n)).astype(np.float32)
+    B = (B / B.sum(axis=1, keepdims=True)).astype(np.float32)
+    a = rng.integers(0, m, k).astype(np.uint32)
+    o = rng.integers(0, z, k).astype(np.uint32)
+    a[0] = m - 1                                  # the last action and state indices in use
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    Bp, status = p.belief_update_batch(B, a, o)
+    Bp, status = Bp.cpu().numpy(), status.cpu().numpy()
+    for i in range(k):
+        rc, bp = oracle.belief_update(P, B[i], int(a[i]), int(o[i]))
+        assert status[i] == rc and np.array_equal(bits(Bp[i]), bits(bp)), i
+    print("belief update batch: n=%d m=%d bit-identical" % (n, m), flush=True)

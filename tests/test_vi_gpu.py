@@ -176,18 +176,56 @@ def test_sorted_tiles_bitwise_vs_oracle(oracle, monkeypatch, n, m, ns, horizon, 
     assert np.array_equal(bits(pol.V_array()), bits(Vo)) and np.array_equal(pol.pi_array(), pio)
 
 
-def test_full_size_sorted_equals_tiled(monkeypatch):
-    """At n = 1e6, m = 8, ns = 16 the default is the sorted layout; four sweeps of it equal
-    four sweeps of the wave-tiled kernel bit for bit."""
+@pytest.mark.parametrize("n,m,ns,horizon,seed,chunk,piece,parts", [
+    (1000, 8, 16, 40, 41, 4096, 8192, 1),       # one successor chunk
+    (9000, 3, 12, 25, 42, 4096, 8192, 1),       # three chunks, a partial last tile and chunk
+    (20000, 8, 16, 12, 43, 4096, 16384, 1),     # five chunks, several pieces per chunk
+    (20000, 8, 16, 12, 43, 4096, 16384, 4),     # the same in four row parts
+    (5000, 1, 5, 10, 44, 4096, 8192, 3),        # m = 1, ns = 5 (rows not a multiple of 4)
+    (300, 16, 200, 6, 45, 4096, 8192, 1),       # m * ns = 3,200: five states per tile
+    (70000, 2, 8, 5, 46, 32768, 131072, 1),     # the default chunk and piece sizes
+])
+def test_chunked_layout_bitwise_vs_oracle(oracle, monkeypatch, n, m, ns, horizon, seed, chunk,
+                                          piece, parts):
+    """The successor-chunked layout (V chunks in LDS, products through HBM), forced at small
+    sizes with small chunks, whole or in row parts: same V and pi bit for bit, with ragged
+    successor lists and an empty row."""
+    monkeypatch.setenv("NOVA_AMD_VI_KERNEL", "chunked")
+    monkeypatch.setenv("NOVA_AMD_VI_CHUNK", str(chunk))
+    monkeypatch.setenv("NOVA_AMD_VI_PIECE", str(piece))
+    monkeypatch.setenv("NOVA_AMD_VI_PARTS", str(parts))
+    print("chunked layout n=%d m=%d ns=%d chunk=%d piece=%d parts=%d"
+          % (n, m, ns, chunk, piece, parts), flush=True)
+    w = synth.mdp_arrays(n=n, m=m, ns=ns, seed=seed, gamma=0.95)
+    S = w["S"].copy()
+    rng = np.random.default_rng(seed)
+    cut = rng.random((n, m)) < 0.3
+    pos = rng.integers(0, ns, size=(n, m))
+    for s, a in zip(*np.nonzero(cut)):
+        S[s, a, pos[s, a]] = -1
+    S[0, 0, :] = -1                                  # an empty row
+    v0 = rng.uniform(-5, 5, n).astype(np.float32)
+    mdp = MDP.from_arrays(S, w["T"], w["R"], gamma=w["gamma"], horizon=horizon)
+    mdp.initialize_gpu()
+    vi = MDPValueIterationGPU(mdp, numThreads=512, Vinitial=v0)
+    assert vi.sweep_kernel() == 3
+    pol = vi.solve()
+    Vo, pio = oracle.vi_run(S, w["T"], w["R"], w["gamma"], horizon, V0=v0)
+    assert np.array_equal(bits(pol.V_array()), bits(Vo)) and np.array_equal(pol.pi_array(), pio)
+
+
+def test_full_size_layouts_equal_tiled(monkeypatch):
+    """At n = 1e6, m = 8, ns = 16 the default is the successor-chunked layout; four sweeps of
+    it, of the successor-sorted layout and of the wave-tiled kernel are equal bit for bit."""
     w = synth.workload("vi_n1e6")
     mdp = MDP.from_arrays(w["S"], w["T"], w["R"], gamma=w["gamma"], horizon=1)
     mdp.initialize_gpu()
     rng = np.random.default_rng(5)
     v0 = rng.uniform(-50, 0, mdp.n).astype(np.float32)
     out = {}
-    for kernel in ("default", "tiled"):
-        if kernel == "tiled":
-            monkeypatch.setenv("NOVA_AMD_VI_KERNEL", "tiled")
+    for kernel in ("default", "sorted", "tiled"):
+        if kernel != "default":
+            monkeypatch.setenv("NOVA_AMD_VI_KERNEL", kernel)
         vi = MDPValueIterationGPU(mdp, Vinitial=v0)
         out[kernel] = vi.sweep_kernel()
         for _ in range(4):
@@ -195,6 +233,8 @@ def test_full_size_sorted_equals_tiled(monkeypatch):
         V, pi = vi.device_values()
         out[kernel + "_V"], out[kernel + "_pi"] = V, pi
         del vi
-    assert out["default"] == 2 and out["tiled"] == 1
-    assert np.array_equal(bits(out["default_V"]), bits(out["tiled_V"]))
-    assert np.array_equal(out["default_pi"], out["tiled_pi"])
+        print("n=1e6: %s layout (kernel %d) done" % (kernel, out[kernel]), flush=True)
+    assert out["default"] == 3 and out["sorted"] == 2 and out["tiled"] == 1
+    for k in ("sorted", "tiled"):
+        assert np.array_equal(bits(out["default_V"]), bits(out[k + "_V"]))
+        assert np.array_equal(out["default_pi"], out[k + "_pi"])

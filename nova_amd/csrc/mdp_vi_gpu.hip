@@ -467,6 +467,8 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_products(unsigned n,
     const uint4 pc = pieces[pieceBase + blockIdx.x];   // chunk, first entry, count (of 8s)
     const unsigned c = pc.x, t = threadIdx.x;
     const unsigned base = c * chunkStates;
+    if (base >= n)
+        return;               // a piece of another layout's chunking: touch nothing
     const unsigned cntV = min(chunkStates, n - base);
     // the chunk of V into LDS: 16-byte loads, eight per lane requested before any is stored
     // (chunk starts are multiples of 4,096 states, so V + base is 16-byte aligned)
@@ -695,9 +697,16 @@ unsigned pow2_at_least(unsigned x)
 
 } // namespace
 
+// Every sweep layout built gets a serial number of its own. The cached sweep graph is keyed by
+// it (not by the layout's address, which a later layout of other settings may reuse) and is
+// dropped with the layout it was captured on.
+static unsigned long long g_layoutSerial = 0;
+static void vi_graph_forget_layout(unsigned long long serial);
+
 // The successor-sorted layout of one solver's model (built by mdp_vi_initialize_gpu for large
 // models, freed with the solver; keyed by the solver's V buffer and the model's arrays).
 struct ViSorted {
+    unsigned long long serial = 0;
     const float *owner = nullptr;
     const int *dS = nullptr;
     const float *dT = nullptr;
@@ -730,6 +739,7 @@ static void sorted_drop(const float *owner, const char *fn)
     for (size_t x = 0; x < g_sorted.size();) {
         if (g_sorted[x].owner == owner) {
             hipDeviceSynchronize();
+            vi_graph_forget_layout(g_sorted[x].serial);
             device_free(g_sorted[x].mem, fn, "the sorted sweep layout");
             g_sorted.erase(g_sorted.begin() + (long)x);
         } else {
@@ -844,12 +854,14 @@ static int sorted_build(const nova::MDP *mdp, const float *owner, hipStream_t st
     device_free(tmp, fn, "sort buffers");
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_sweep_sorted,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.ldsBytes);
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(float) * 2 * (size_t)kSortedEntries));
     if (e != hipSuccess || rc != NOVA_SUCCESS) {
         device_free(L.mem, fn, "the sorted sweep layout");
         return rc != NOVA_SUCCESS ? rc : hip_fail(e, fn, "Failed to build the sorted layout.",
                                                   NOVA_ERROR_KERNEL_EXECUTION);
     }
+    L.serial = ++g_layoutSerial;
     g_sorted.push_back(L);
     return NOVA_SUCCESS;
 }
@@ -857,6 +869,7 @@ static int sorted_build(const nova::MDP *mdp, const float *owner, hipStream_t st
 // The successor-chunked layout of one solver's model (built by mdp_vi_initialize_gpu for large
 // models, freed with the solver; keyed like the sorted layout).
 struct ViChunked {
+    unsigned long long serial = 0;
     const float *owner = nullptr;
     const int *dS = nullptr;
     const float *dT = nullptr;
@@ -891,6 +904,7 @@ static void chunked_drop(const float *owner, const char *fn)
     for (size_t x = 0; x < g_chunked.size();) {
         if (g_chunked[x].owner == owner) {
             hipDeviceSynchronize();
+            vi_graph_forget_layout(g_chunked[x].serial);
             device_free(g_chunked[x].mem, fn, "the chunked sweep layout");
             g_chunked.erase(g_chunked.begin() + (long)x);
         } else {
@@ -1059,18 +1073,23 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
     device_free(tmp, fn, "layout build buffers");
     L.ldsA = sizeof(float) * (size_t)cs;
     L.ldsB = sizeof(float) * ((size_t)L.statesPerTile * m + L.tileEntries);
+    // the largest dynamic LDS any layout launches with (chunks of 32,768 states; row sums and
+    // products of 16,384 slots each), so that the graphs of layouts alive at once all launch
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_chunk_products,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.ldsA);
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(float) * 32768));
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_chunk_rows,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.ldsB);
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(float) * 2 * (size_t)kSortedEntries));
     if (e != hipSuccess || rc != NOVA_SUCCESS) {
         if (L.mem)
             device_free(L.mem, fn, "the chunked sweep layout");
         return rc != NOVA_SUCCESS ? rc : hip_fail(e, fn, "Failed to build the chunked layout.",
                                                   NOVA_ERROR_KERNEL_EXECUTION);
     }
+    L.serial = ++g_layoutSerial;
     g_chunked.push_back(L);
     return NOVA_SUCCESS;
 }
@@ -1162,15 +1181,23 @@ struct ViGraph {
 };
 static ViGraph g_graph;
 
+static void vi_graph_forget_layout(unsigned long long serial)
+{
+    if (g_graph.exec != nullptr && g_graph.key[7] == (const void *)(uintptr_t)serial) {
+        hipGraphExecDestroy(g_graph.exec);
+        g_graph.exec = nullptr;
+    }
+}
+
 static int vi_run_sweeps(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned sweeps,
                          hipStream_t st, hipEvent_t t0, hipEvent_t t1, const char *fn)
 {
     const ViPlan plan = vi_plan(mdp, vi);
     const unsigned parity = vi->currentHorizon % 2;
+    const unsigned long long layout = plan.chunked ? plan.chunked->serial
+                                                   : (plan.sorted ? plan.sorted->serial : 0ull);
     const void *key[8] = {mdp->d_S, mdp->d_T, mdp->d_R, vi->d_V, vi->d_VPrime, vi->d_pi,
-                          (const void *)plan.tiled,
-                          plan.chunked ? (const void *)plan.chunked->mem
-                                       : (plan.sorted ? (const void *)plan.sorted->mem : nullptr)};
+                          (const void *)plan.tiled, (const void *)(uintptr_t)layout};
     bool hit = g_graph.exec != nullptr && g_graph.sweeps == sweeps && g_graph.parity == parity &&
                g_graph.n == mdp->n && g_graph.m == mdp->m && g_graph.ns == mdp->ns &&
                g_graph.gamma == mdp->gamma && std::memcmp(g_graph.key, key, sizeof(key)) == 0;

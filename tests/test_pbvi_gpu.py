@@ -6,7 +6,9 @@ fast mode : FMA accumulation. From the same Gamma, the chosen alphas must match 
             except at near-ties (conftest.tie_rel: a best-minus-runner-up gap below
             max(1e-5, 4 sqrt(K) u) * max(1, |action value|), K = rz * ns); the new alphas of
             beliefs whose choices all match agree within 1e-5 relative; whole-run values
-            b.alpha at the beliefs agree with the FP64 oracle within 1e-4 relative.
+            b.alpha at the beliefs agree with the FP64 oracle within 1e-4 relative where
+            nova's own float run does within 1e-5; elsewhere every checked step of the run is
+            a backup of the run's own previous Gamma within 1e-4 relative.
 """
 import numpy as np
 import pytest
@@ -198,12 +200,25 @@ def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag
     err_nova = np.max(np.abs(v32 - v64)) / scale
     print("%s: fast mode (scoring %s), max_b |V(b) - V_fp64(b)| / ||V|| = %.3g (nova's float "
           "arithmetic: %.3g)" % (name, variant, err, err_nova))
-    # nova's own float arithmetic drifts 0.58% from FP64 at pbvi_cpu_n64 after 100
-    # iterations (rounding flips near-tied choices that later steer point-based values apart).
-    # The fast path either stays within 1e-4 relative of FP64 (FMA / matrix-core sums) or, for
-    # instances the one-launch kernel takes, computes in nova's own rounding order and then
-    # equals nova's float run bit for bit
-    assert err <= 1e-4 or np.array_equal(bits(G), bits(G32)), (err, err_nova)
+    if err_nova <= 1e-5:
+        assert err <= 1e-4, (err, err_nova)
+        return
+    # Where nova's own float arithmetic drifts from FP64 (0.58% at pbvi_cpu_n64 after 100
+    # iterations: rounding flips near-tied choices, which later steer point-based values
+    # apart), any other rounding order drifts too, and no fixed bound holds for whole runs.
+    # What holds is each step: iteration t + 1 of the GPU's run is a backup of its own Gamma_t,
+    # every belief's value within 1e-4 relative of the oracle's best action value.
+    Z, B = P["Z"], P["B"]
+    for t in sorted({0, horizon // 4, horizon // 2, horizon - 1}):
+        Gt = gpu_solve(P, FAST, horizon=t)[0] if t > 0 else P["Gamma0"]
+        G1 = gpu_solve(P, FAST, horizon=t + 1)[0]
+        ref = oracle.pbvi_backup(P, Gt)
+        best = ref["aval"].astype(np.float64).max(axis=1)
+        v1 = sparse_values(P, G1)
+        step = np.max(np.abs(v1 - best) / np.maximum(1.0, np.abs(best)))
+        print("%s: step %d of the fast run is a backup of its Gamma_t within %.3g relative"
+              % (name, t, step), flush=True)
+        assert step <= 1e-4
 
 
 def test_full_size_sample_vs_oracle(oracle):

@@ -238,3 +238,35 @@ def test_full_size_layouts_equal_tiled(monkeypatch):
     for k in ("sorted", "tiled"):
         assert np.array_equal(bits(out["default_V"]), bits(out[k + "_V"]))
         assert np.array_equal(out["default_pi"], out[k + "_pi"])
+
+
+def test_layouts_rebuilt_with_other_settings(oracle, monkeypatch):
+    """Solvers of one model built one after another with other layout settings (sorted, chunks
+    of 32,768 and of 4,096 states, several row parts), each freed before the next is built, as
+    a tuning loop does: each run equals the oracle bit for bit. A freed layout's memory can be
+    reused by the next one at the same address, so the cached sweep graph must be keyed by the
+    layout itself (a graph captured on another layout's chunking would read past V)."""
+    w = synth.mdp_arrays(n=70000, m=2, ns=8, seed=47, gamma=0.95)
+    mdp = MDP.from_arrays(w["S"], w["T"], w["R"], gamma=w["gamma"], horizon=6)
+    mdp.initialize_gpu()
+    v0 = np.random.default_rng(47).uniform(-5, 5, mdp.n).astype(np.float32)
+    Vo, pio = oracle.vi_run(w["S"], w["T"], w["R"], w["gamma"], 6, V0=v0)
+    import torch
+    d0 = torch.from_numpy(v0).cuda()
+    for setting in (dict(NOVA_AMD_VI_KERNEL="sorted"),
+                    dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="32768"),
+                    dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="4096"),
+                    dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="4096",
+                         NOVA_AMD_VI_PARTS="3"),
+                    dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="32768")):
+        for k in ("NOVA_AMD_VI_KERNEL", "NOVA_AMD_VI_CHUNK", "NOVA_AMD_VI_PARTS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in setting.items():
+            monkeypatch.setenv(k, v)
+        print("layout setting %s" % setting, flush=True)
+        vi = MDPValueIterationGPU(mdp, Vinitial=v0)
+        for _ in range(2):
+            vi.run(6, d_V0=d0.data_ptr())
+            V, pi = vi.device_values()
+            assert np.array_equal(bits(V), bits(Vo)) and np.array_equal(pi, pio), setting
+        del vi

@@ -183,6 +183,19 @@ extern "C" int nova_amd_alpha_vectors_value_and_action_gpu(unsigned int n, unsig
         const float *d_Gamma, const unsigned int *d_pi, unsigned int k, const float *d_B,
         int precision, float *d_V, unsigned int *d_action, int *d_alpha);
 
+// No counterpart in the reference. The exact mode (precision 0) of
+// nova_amd_alpha_vectors_value_and_action_gpu for callers that cannot allocate device memory
+// (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc): it calls no
+// hipMalloc, hipFree or allocating library routine. Its per-(belief, 64-alpha tile) partials
+// live in a static 16 MiB device array that each GPU the library runs on holds until the
+// process ends, so ceil(r / 64) * k <= 2,097,152 (larger calls return NOVA_ERROR_INVALID_DATA).
+// Same arguments and results as the exact mode of that entry point, bit for bit. Calls on one
+// GPU share the array; calls that overlap on one GPU race (the library serialises the calls of
+// one process). Synchronous.
+extern "C" int nova_amd_alpha_vectors_value_and_action_noalloc_gpu(unsigned int n,
+        unsigned int r, const float *d_Gamma, const unsigned int *d_pi, unsigned int k,
+        const float *d_B, float *d_V, unsigned int *d_action, int *d_alpha);
+
 // pomdp_belief_update_cpu (src/pomdp/utilities/pomdp_model_cpu.cpp:89-137) for k (belief,
 // action, observation) triples at once, bit for bit: d_B (k x n beliefs), d_action and
 // d_observation (k each) on the device; d_Bp (k x n) receives the updated beliefs and

@@ -207,6 +207,9 @@ EXTRA_SIGNATURES = {
     "nova_amd_alpha_vectors_value_and_action_gpu": [U, U, ct.c_void_p, ct.c_void_p, U,
                                                     ct.c_void_p, ct.c_int, ct.c_void_p,
                                                     ct.c_void_p, ct.c_void_p],
+    "nova_amd_alpha_vectors_value_and_action_noalloc_gpu": [U, U, ct.c_void_p, ct.c_void_p, U,
+                                                            ct.c_void_p, ct.c_void_p,
+                                                            ct.c_void_p, ct.c_void_p],
     "nova_amd_ssp_backup_gpu": [_MDP, U, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p,
                                 ct.c_void_p],
     "nova_amd_ssp_sweeps_gpu": [_MDP, ct.c_int, ct.c_void_p, U, ct.c_float, ct.c_void_p,

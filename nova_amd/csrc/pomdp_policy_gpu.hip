@@ -688,3 +688,82 @@ extern "C" int nova_amd_belief_update_batch_gpu(const nova::POMDP *pomdp, unsign
     return belief_update_rows(pomdp, k, d_B, nullptr, d_action, d_observation, d_Bp, d_status,
                               fn);
 }
+
+// ---- the exact value/action without allocation
+//
+// The per-(belief, alpha tile) partials of the exact path live in `scratch`, a static device
+// array of kNoAllocScratchBytes that every GPU the library runs on holds until the process
+// ends. Host code never passes the array's name to a kernel (on the host it is only a stand-in
+// for the symbol): its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratchDev. Every partial the merge reads is written by the
+// tile kernel of the same call first, so nothing in it has to start at zero. The contents
+// persist between calls; calls on one GPU share the array, and overlapping ones would race
+// (the library lock serialises the calls of one process).
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+void *g_scratchDev[kScratchDevices];
+hipError_t g_scratchErr[kScratchDevices];
+std::once_flag g_scratchOnce[kScratchDevices];
+
+// Device address of `scratch` on the current device (nullptr and an error message on failure).
+void *scratch_address(const char *fn)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kScratchDevices) {
+        hip_fail(e != hipSuccess ? e : hipErrorInvalidDevice, fn,
+                 "No usable current device for the scratch array.", NOVA_ERROR_INVALID_DATA);
+        return nullptr;
+    }
+    std::call_once(g_scratchOnce[dev], [dev]() {
+        void *p = nullptr;
+        g_scratchErr[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratchDev[dev] = g_scratchErr[dev] == hipSuccess ? p : nullptr;
+    });
+    if (g_scratchDev[dev] == nullptr)
+        hip_fail(g_scratchErr[dev] != hipSuccess ? g_scratchErr[dev] : hipErrorInvalidSymbol, fn,
+                 "Could not look up the device address of the scratch array.",
+                 NOVA_ERROR_KERNEL_EXECUTION);
+    return g_scratchDev[dev];
+}
+} // namespace
+
+extern "C" int nova_amd_alpha_vectors_value_and_action_noalloc_gpu(unsigned int n,
+        unsigned int r, const float *d_Gamma, const unsigned int *d_pi, unsigned int k,
+        const float *d_B, float *d_V, unsigned int *d_action, int *d_alpha)
+{
+    NOVA_AMD_API_LOCK();
+    const char *fn = "nova_amd_alpha_vectors_value_and_action_noalloc_gpu";
+    if (n == 0 || r == 0 || k == 0 || d_Gamma == nullptr || d_pi == nullptr ||
+            d_B == nullptr || d_V == nullptr || d_action == nullptr ||
+            (r + kVT - 1) / kVT > 65535u) {
+        NOVA_ERR(fn, "Invalid arguments (n, r, k > 0, r <= 4,194,240; device arrays).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    const size_t colTiles = (r + kVT - 1) / kVT;
+    const size_t bPart = up256(colTiles * k * sizeof(float));
+    if (2 * bPart > kNoAllocScratchBytes) {
+        NOVA_ERR(fn, "ceil(r / 64) * k exceeds the 16 MiB scratch array (at most 2,097,152 "
+                     "(belief, 64-alpha tile) partials per call).");
+        return NOVA_ERROR_INVALID_DATA;
+    }
+    char *c = static_cast<char *>(scratch_address(fn));
+    if (c == nullptr)
+        return NOVA_ERROR_KERNEL_EXECUTION;
+    hipStream_t st = stream();
+    float *partV = reinterpret_cast<float *>(c);
+    int *partJ = reinterpret_cast<int *>(c + bPart);
+    hipLaunchKernelGGL(alpha_value_exact, dim3((k + kVT - 1) / kVT, (unsigned)colTiles),
+                       dim3(kVThreads), 0, st, k, n, r, d_B, d_Gamma, partV, partJ);
+    hipLaunchKernelGGL(alpha_value_merge, dim3((k + 255) / 256), dim3(256), 0, st, k,
+                       (unsigned)colTiles, partV, partJ, d_pi, d_V, d_action, d_alpha);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return hip_fail(e, fn, "Kernel execution failed.", NOVA_ERROR_KERNEL_EXECUTION);
+    return NOVA_SUCCESS;
+}

@@ -284,7 +284,8 @@ def bench_pbvi(args):
                         "precision": "fp32",
                         "kernel": {8: "pbvi_score_mfma<4 waves>", 16: "pbvi_score_mfma<8 waves>",
                                    32: "pbvi_score_mfma<8 waves, 8-step prefetch>",
-                                   64: "pbvi_score_mfma16 (16 observation rows)"}.get(
+                                   64: "pbvi_score_mfma16 (16 observation rows)",
+                                   128: "pbvi_score_mfma_all (every alpha tile per workgroup)"}.get(
                                        variant, "pbvi_score<fast,%d>" % variant),
                         "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": achieved / 157.3,

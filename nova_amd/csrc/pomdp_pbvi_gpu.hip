@@ -82,6 +82,8 @@ struct PbviInst {
     int *recJ;             // optional per-iteration record: recJ[(iter*r + i)*z + o] = j*
     int *recA;             // and recA[iter*r + i] = chosen action (-1: none)
     int fused;             // each iteration is one pbvi_fused launch (small instances)
+    int actionMajor;       // matrix-core scorers: workgroups enumerate (action, belief) with the
+                           // action slower, so that concurrent ones read one action's O rows
 };
 
 namespace {
@@ -582,7 +584,9 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     const float gamma = P.gamma;
     const unsigned local = blockIdx.x - off[b];
     const unsigned rb = P.rb;
-    const unsigned ia = local % (rb * m);        // belief * m + action
+    const unsigned q = local % (rb * m);
+    // ia = belief * m + action; the enumeration order of (belief, action) is the instance's
+    const unsigned ia = P.actionMajor ? (q % rb) * m + q / rb : q;
     const unsigned tile = local / (rb * m);
     const unsigned i = ia / m, a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
@@ -707,6 +711,149 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     }
 }
 
+// The 32-row matrix-core scoring with the whole pair list of (i, a) staged once and every alpha
+// tile of the instance swept by the same workgroup: W is built once per (belief, action) rather
+// than once per tile, and the workgroup writes the partials of every tile. For pair lists of at
+// most kWAll pairs (the tuner offers it then; longer lists use the kernel above).
+constexpr int kWAll = 512;
+
+template <int PS>
+__global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter)
+{
+    constexpr int NW = 8, NB = 2, NT = NW * 64;
+    static_assert(NW * NB * 32 == kMTile, "a workgroup sweeps whole alpha tiles");
+    constexpr int STEP = 4 * PS;
+    static_assert(kWAll % STEP == 0, "whole loop steps");
+    __shared__ int spU[kWAll];
+    __shared__ float w[kWAll * 32];
+    __shared__ float redV[NW][32];
+    __shared__ int redJ[NW][32];
+
+    if (blockIdx.x >= off[count])
+        return;
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, m = P.m, z = P.z, r = P.r;
+    const unsigned local = blockIdx.x - off[b];
+    const unsigned rb = P.rb;
+    const unsigned q = local % (rb * m);
+    const unsigned ia = P.actionMajor ? (q % rb) * m + q / rb : q;   // belief * m + action
+    const unsigned a = ia % m;
+    const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
+
+    const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
+    const int uPad = (uEnd + STEP - 1) / STEP * STEP;             // <= kWAll
+    const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
+    const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
+    const size_t rS = P.rS;
+    for (int u = t; u < uPad; u += NT)
+        spU[u] = (u < uEnd) ? pSp[u] : 0;
+
+    for (unsigned o0 = 0; o0 < z; o0 += 32) {
+        const int zc = min(32, (int)(z - o0));
+        __syncthreads();
+        for (int x = t; x < uPad * 32; x += NT) {
+            const int u = x / 32, oo = x % 32;
+            float v = 0.0f;
+            if (u < uEnd && oo < zc)
+                v = pw[u] * Oa[(size_t)spU[u] * z + o0 + oo];
+            w[x] = v;
+        }
+        __syncthreads();
+        for (unsigned tile = 0; tile < P.tiles; tile++) {
+            const unsigned jw = tile * kMTile + wave * NB * 32;
+            const unsigned jl = jw + lane % 32;
+            floatx16_t acc[NB];
+#pragma unroll
+            for (int y = 0; y < NB; y++)
+#pragma unroll
+                for (int v = 0; v < 16; v++)
+                    acc[y][v] = 0.0f;
+            if (uPad > 0) {
+                float fb[PS][NB], nb[PS][NB];
+                auto loadB = [&](int u0, float (&dst)[PS][NB]) {
+#pragma unroll
+                    for (int qq = 0; qq < PS; qq++) {
+                        const int uu = u0 + 2 * qq + (int)(lane / 32);
+                        const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+#pragma unroll
+                        for (int y = 0; y < NB; y++)
+                            dst[qq][y] = row[y * 32];
+                    }
+                };
+                auto mfmaTrip = [&](int u0, const float (&bq)[PS][NB]) {
+                    float fa[PS];
+#pragma unroll
+                    for (int qq = 0; qq < PS; qq++)
+                        fa[qq] = w[(u0 + 2 * qq + (int)(lane / 32)) * 32 + lane % 32];
+#pragma unroll
+                    for (int qq = 0; qq < PS; qq++)
+#pragma unroll
+                        for (int y = 0; y < NB; y++)
+                            acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[qq], bq[qq][y],
+                                                                          acc[y], 0, 0, 0);
+                };
+                loadB(0, fb);
+                for (int u0 = 0; u0 < uPad; u0 += STEP) {
+                    loadB(u0 + STEP / 2, nb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mfmaTrip(u0, fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    loadB(min(u0 + STEP, uPad - STEP / 2), fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mfmaTrip(u0 + STEP / 2, nb);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                float bv = kLow;
+                int bj = INT_MAX;
+#pragma unroll
+                for (int y = 0; y < NB; y++) {
+                    const unsigned j = jw + y * 32 + lane % 32;
+                    if (j < r && acc[y][v] > bv) {
+                        bv = acc[y][v];
+                        bj = (int)j;
+                    }
+                }
+                for (int off2 = 16; off2 >= 1; off2 >>= 1) {
+                    const float ov = __shfl_xor(bv, off2, 32);
+                    const int oj = __shfl_xor(bj, off2, 32);
+                    if (better(ov, oj, bv, bj)) {
+                        bv = ov;
+                        bj = oj;
+                    }
+                }
+                if (lane % 32 == 0) {
+                    const int o = (v % 4) + 4 * (int)(lane / 32) + 8 * (v / 4);
+                    redV[wave][o] = bv;
+                    redJ[wave][o] = bj;
+                }
+            }
+            __syncthreads();
+            if (t < (unsigned)zc) {
+                float v = redV[0][t];
+                int j = redJ[0][t];
+                for (int wv = 1; wv < NW; wv++)
+                    if (better(redV[wv][t], redJ[wv][t], v, j)) {
+                        v = redV[wv][t];
+                        j = redJ[wv][t];
+                    }
+                const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
+                P.partVal[idx] = v;
+                P.partJ[idx] = j;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // The same scoring for instances with at most 16 observations, on v_mfma_f32_16x16x4_f32:
 // rows are 16 observations (the 32-row kernel above would multiply half its rows by zero),
 // columns 16 alphas per block, the reduction over the pairs four at a time. Each of the 8 waves
@@ -738,7 +885,8 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
     const unsigned n = P.n, m = P.m, z = P.z, r = P.r;
     const unsigned local = blockIdx.x - off[b];
     const unsigned rb = P.rb;
-    const unsigned ia = local % (rb * m);
+    const unsigned q = local % (rb * m);
+    const unsigned ia = P.actionMajor ? (q % rb) * m + q / rb : q;   // belief * m + action
     const unsigned tile = local / (rb * m);
     const unsigned a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
@@ -1358,8 +1506,8 @@ static bool dense_mode(int mode) { return mode == 2 || mode == 3; }
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Scoring variants: jpt = 1, 2, 4 alphas per lane on the vector ALUs (tiles of 256 * jpt alphas),
-// jpt = 8, 16, 32 the matrix-core kernel of fast mode, jpt = 64 its 16-observation-row form
-// (tiles of kMTile alphas).
+// jpt = 8, 16, 32 the matrix-core kernel of fast mode, jpt = 64 its 16-observation-row form,
+// jpt = 128 its form that sweeps every alpha tile in one workgroup (tiles of kMTile alphas).
 static unsigned tile_width(unsigned jpt) { return jpt >= 8 ? (unsigned)kMTile : kPB * jpt; }
 static unsigned tiles_for(unsigned r, unsigned jpt) { return (r + tile_width(jpt) - 1) / tile_width(jpt); }
 
@@ -1490,6 +1638,9 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
     I.useLds = pbvi_lds_rows(p->n) ? 1 : 0;
     I.S = p->d_S; I.T = p->d_T; I.O = p->d_O; I.R = p->d_R; I.Z = p->d_Z; I.B = p->d_B;
     I.G0 = G0; I.G1 = G1; I.pi = pi;
+    I.actionMajor = 1;
+    if (const char *f = std::getenv("NOVA_AMD_PBVI_ORDER"))
+        I.actionMajor = std::atoi(f) != 0 ? 1 : 0;
     inst_ws(p, ws, I);
 }
 
@@ -1528,9 +1679,9 @@ static bool fused_fits(const PbviInst &I, int mode)
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
 struct DenseWs;
 
-// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32, 64); PbviPlan::totSc and the offset
-// table hold one entry per variant.
-constexpr int kVariants = 7;
+// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32, 64, 128); PbviPlan::totSc and the
+// offset table hold one entry per variant.
+constexpr int kVariants = 8;
 
 struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
@@ -1551,7 +1702,7 @@ static_assert(sizeof(PbviPlan::totSc) == kVariants * sizeof(unsigned),
 static int jpt_slot(unsigned jpt)
 {
     return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : (jpt == 16 ? 4 :
-           (jpt == 32 ? 5 : 6)))));
+           (jpt == 32 ? 5 : (jpt == 64 ? 6 : 7))))));
 }
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
@@ -1570,6 +1721,7 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
         if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 4>), g, dim3(512), 0, st, dInst, sc, c, iter);
         if (jpt == 32) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 8>), g, dim3(512), 0, st, dInst, sc, c, iter);
         if (jpt == 64) hipLaunchKernelGGL((pbvi_score_mfma16<2>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 128) hipLaunchKernelGGL((pbvi_score_mfma_all<4>), g, dim3(512), 0, st, dInst, sc, c, iter);
     }
 }
 
@@ -1600,7 +1752,8 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
         tr[b + 1] = tr[b] + (four ? ((I.n + 63) / 64) * (I.rS / 64) : 0u);
         for (int v = 0; v < kVariants; v++)
             scs[v][b + 1] = scs[v][b] +
-                            ((four && jpt_slot(I.jpt) == v) ? I.rb * I.m * I.tiles : 0u);
+                            ((four && jpt_slot(I.jpt) == v)
+                             ? I.rb * I.m * (I.jpt == 128 ? 1u : I.tiles) : 0u);
         be[b + 1] = be[b] + (four ? I.rb : 0u);
         fu[b + 1] = fu[b] + (four ? 0u : I.rb);
         if (!four)
@@ -1759,7 +1912,7 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     if (pl.totBe == 0)
         return;
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64};
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64, 128};
     for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
@@ -1840,7 +1993,8 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
         const int v = std::atoi(f);
         if (((v == 1 || v == 2 || v == 4) && valuOk) ||
                 ((v == 8 || v == 16 || v == 32) && mode != 1) ||
-                (v == 64 && mode != 1 && I.z <= 16))
+                (v == 64 && mode != 1 && I.z <= 16) ||
+                (v == 128 && mode != 1 && I.kmax <= (unsigned)kWAll))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
@@ -1855,11 +2009,13 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) {
+    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u, 128u}) {
         if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
         if (jpt == 64 && I.z > 16)
             continue;                          // the 16-row kernel: at most 16 observations
+        if (jpt == 128 && I.kmax > (unsigned)kWAll)
+            continue;                          // the all-tiles kernel stages every pair once
         if (jpt < 8 && !valuOk)
             continue;
         PbviPlan pl;

@@ -554,6 +554,13 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_rows(unsigned n, unsig
     float *Q = lds;                                  // rowsPerTile row sums
     float *Pl = lds + rowsPerTile;                   // the tile's products by slot
     const unsigned tile = tileBase + blockIdx.x, t = threadIdx.x;
+    const unsigned s0 = tile * statesPerTile;
+    const unsigned states = min(statesPerTile, n - s0);
+    const unsigned rows = states * m;
+    // the first row's R and valid length, requested before the gathers
+    const size_t g0 = (size_t)s0 * m + t;
+    const float r0 = t < rows ? R[g0] : 0.0f;
+    const unsigned c0 = t < rows ? cnt[g0] : 0u;
     for (unsigned c = t; c <= chunks; c += kChunkThreads)
         seg[c] = tileSeg[(size_t)tile * (chunks + 1) + c];
     __syncthreads();
@@ -582,13 +589,10 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_rows(unsigned n, unsig
                 Pl[sl[u]] = p[u];
     }
     __syncthreads();
-    const unsigned s0 = tile * statesPerTile;
-    const unsigned states = min(statesPerTile, n - s0);
-    const unsigned rows = states * m;
     for (unsigned row = t; row < rows; row += kChunkThreads) {
         const size_t g = (size_t)s0 * m + row;
-        float q = R[g];
-        const unsigned c = cnt[g];
+        float q = row == t ? r0 : R[g];
+        const unsigned c = row == t ? c0 : cnt[g];
         for (unsigned i = 0; i < c; i++)
             q = __fadd_rn(q, Pl[i * rowsPerTile + row]);
         Q[row] = q;
@@ -935,7 +939,10 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
     L.n = n;
     L.m = m;
     L.ns = ns;
-    L.statesPerTile = kSortedEntries / (m * ns);
+    // tiles of 16,384 slots (NOVA_AMD_VI_TILE: 4,096 to 16,384), whole states each
+    const unsigned tileSlots = std::min(kSortedEntries, std::max(std::max(4096u, m * ns),
+            env_unsigned("NOVA_AMD_VI_TILE", kSortedEntries)));
+    L.statesPerTile = tileSlots / (m * ns);
     L.tileEntries = L.statesPerTile * m * ns;
     L.tiles = (n + L.statesPerTile - 1) / L.statesPerTile;
     // chunk of successor states whose V fills a pass-A workgroup's LDS (a power of two from

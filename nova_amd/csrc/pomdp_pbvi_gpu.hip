@@ -655,16 +655,18 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
             // earlier. The loads carry no conditions (the last prefetch re-reads pairs already
             // staged), so the compiler can count the outstanding ones exactly.
             // sched_barrier keeps the scheduler from sinking the prefetch below the MFMAs.
-            loadB(0, fb);
-            for (int u0 = 0; u0 < cPad; u0 += STEP) {
-                loadB(u0 + STEP / 2, nb);
-                __builtin_amdgcn_sched_barrier(0);
-                mfmaTrip(u0, fb);
-                __builtin_amdgcn_sched_barrier(0);
-                loadB(min(u0 + STEP, cPad - STEP / 2), fb);
-                __builtin_amdgcn_sched_barrier(0);
-                mfmaTrip(u0 + STEP / 2, nb);
-                __builtin_amdgcn_sched_barrier(0);
+            if (jw < r) {       // a wave whose alphas all lie past r has nothing to score
+                loadB(0, fb);
+                for (int u0 = 0; u0 < cPad; u0 += STEP) {
+                    loadB(u0 + STEP / 2, nb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mfmaTrip(u0, fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    loadB(min(u0 + STEP, cPad - STEP / 2), fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mfmaTrip(u0 + STEP / 2, nb);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
         }
         // per row (observation): first maximiser over the wave's alphas, then across the waves
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
 #pragma unroll
                 for (int v = 0; v < 16; v++)
                     acc[y][v] = 0.0f;
-            if (uPad > 0) {
+            if (uPad > 0 && jw < r) {   // waves whose alphas all lie past r skip
                 float fb[PS][NB], nb[PS][NB];
                 auto loadB = [&](int u0, float (&dst)[PS][NB]) {
 #pragma unroll
@@ -946,16 +948,18 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
                         acc[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[q], bq[q][y], acc[y],
                                                                       0, 0, 0);
             };
-            loadB(0, fb);
-            for (int u0 = 0; u0 < cPad; u0 += STEP) {
-                loadB(u0 + STEP / 2, nb);
-                __builtin_amdgcn_sched_barrier(0);
-                mfmaTrip(u0, fb);
-                __builtin_amdgcn_sched_barrier(0);
-                loadB(min(u0 + STEP, cPad - STEP / 2), fb);
-                __builtin_amdgcn_sched_barrier(0);
-                mfmaTrip(u0 + STEP / 2, nb);
-                __builtin_amdgcn_sched_barrier(0);
+            if (jw < r) {       // a wave whose alphas all lie past r has nothing to score
+                loadB(0, fb);
+                for (int u0 = 0; u0 < cPad; u0 += STEP) {
+                    loadB(u0 + STEP / 2, nb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mfmaTrip(u0, fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    loadB(min(u0 + STEP, cPad - STEP / 2), fb);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mfmaTrip(u0 + STEP / 2, nb);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
         }
         // per row (observation): first maximiser over the wave's alphas, then across the waves

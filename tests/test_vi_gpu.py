@@ -176,26 +176,31 @@ def test_sorted_tiles_bitwise_vs_oracle(oracle, monkeypatch, n, m, ns, horizon, 
     assert np.array_equal(bits(pol.V_array()), bits(Vo)) and np.array_equal(pol.pi_array(), pio)
 
 
-@pytest.mark.parametrize("n,m,ns,horizon,seed,chunk,piece,parts", [
-    (1000, 8, 16, 40, 41, 4096, 8192, 1),       # one successor chunk
-    (9000, 3, 12, 25, 42, 4096, 8192, 1),       # three chunks, a partial last tile and chunk
-    (20000, 8, 16, 12, 43, 4096, 16384, 1),     # five chunks, several pieces per chunk
-    (20000, 8, 16, 12, 43, 4096, 16384, 4),     # the same in four row parts
-    (5000, 1, 5, 10, 44, 4096, 8192, 3),        # m = 1, ns = 5 (rows not a multiple of 4)
-    (300, 16, 200, 6, 45, 4096, 8192, 1),       # m * ns = 3,200: five states per tile
-    (70000, 2, 8, 5, 46, 32768, 131072, 1),     # the default chunk and piece sizes
+@pytest.mark.parametrize("n,m,ns,horizon,seed,chunk,piece,parts,tile", [
+    (1000, 8, 16, 40, 41, 4096, 8192, 1, 16384),     # one successor chunk
+    (9000, 3, 12, 25, 42, 4096, 8192, 1, 16384),     # three chunks, a partial last tile and chunk
+    (20000, 8, 16, 12, 43, 4096, 16384, 1, 16384),   # five chunks, several pieces per chunk
+    (20000, 8, 16, 12, 43, 4096, 16384, 4, 16384),   # the same in four row parts
+    (5000, 1, 5, 10, 44, 4096, 8192, 3, 16384),      # m = 1, ns = 5 (rows not a multiple of 4)
+    (300, 16, 200, 6, 45, 4096, 8192, 1, 16384),     # m * ns = 3,200: five states per tile
+    (70000, 2, 8, 5, 46, 32768, 131072, 1, 16384),   # the default chunk and piece sizes
+    (20000, 8, 16, 12, 43, 4096, 16384, 1, 8192),    # tiles of 8,192 slots
+    (9000, 3, 12, 25, 42, 4096, 8192, 3, 12288),     # tiles of 12,288 slots, three parts
+    (5000, 4, 8, 10, 48, 4096, 8192, 1, 4096),       # tiles of 4,096 slots
+    (300, 16, 200, 6, 45, 4096, 8192, 1, 4096),      # m * ns = 3,200: one state per tile
 ])
 def test_chunked_layout_bitwise_vs_oracle(oracle, monkeypatch, n, m, ns, horizon, seed, chunk,
-                                          piece, parts):
+                                          piece, parts, tile):
     """The successor-chunked layout (V chunks in LDS, products through HBM), forced at small
-    sizes with small chunks, whole or in row parts: same V and pi bit for bit, with ragged
-    successor lists and an empty row."""
+    sizes with small chunks, whole or in row parts, with tiles of several slot counts: same V
+    and pi bit for bit, with ragged successor lists and an empty row."""
     monkeypatch.setenv("NOVA_AMD_VI_KERNEL", "chunked")
     monkeypatch.setenv("NOVA_AMD_VI_CHUNK", str(chunk))
     monkeypatch.setenv("NOVA_AMD_VI_PIECE", str(piece))
     monkeypatch.setenv("NOVA_AMD_VI_PARTS", str(parts))
-    print("chunked layout n=%d m=%d ns=%d chunk=%d piece=%d parts=%d"
-          % (n, m, ns, chunk, piece, parts), flush=True)
+    monkeypatch.setenv("NOVA_AMD_VI_TILE", str(tile))
+    print("chunked layout n=%d m=%d ns=%d chunk=%d piece=%d parts=%d tile=%d"
+          % (n, m, ns, chunk, piece, parts, tile), flush=True)
     w = synth.mdp_arrays(n=n, m=m, ns=ns, seed=seed, gamma=0.95)
     S = w["S"].copy()
     rng = np.random.default_rng(seed)
@@ -258,8 +263,11 @@ def test_layouts_rebuilt_with_other_settings(oracle, monkeypatch):
                     dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="4096"),
                     dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="4096",
                          NOVA_AMD_VI_PARTS="3"),
+                    dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="4096",
+                         NOVA_AMD_VI_TILE="8192"),
                     dict(NOVA_AMD_VI_KERNEL="chunked", NOVA_AMD_VI_CHUNK="32768")):
-        for k in ("NOVA_AMD_VI_KERNEL", "NOVA_AMD_VI_CHUNK", "NOVA_AMD_VI_PARTS"):
+        for k in ("NOVA_AMD_VI_KERNEL", "NOVA_AMD_VI_CHUNK", "NOVA_AMD_VI_PARTS",
+                  "NOVA_AMD_VI_TILE"):
             monkeypatch.delenv(k, raising=False)
         for k, v in setting.items():
             monkeypatch.setenv(k, v)

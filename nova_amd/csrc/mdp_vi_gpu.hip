@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kSortedThreads) void vi_sweep_sorted(unsigned n, un
 // sums are those of the sweeps above, so V' and pi are the same bit for bit.
 // Per valid entry: 2 + 4 bytes read and 4 written by pass A, 4 + 2 read by pass B.
 constexpr unsigned kChunkStatesDefault = 32768;    // 128 KB of V per pass-A workgroup
-constexpr unsigned kPieceEntriesDefault = 131072;  // entries per pass-A workgroup
+constexpr unsigned kPieceEntriesDefault = 65536;   // entries per pass-A workgroup
 constexpr unsigned kChunkAlign = 64;               // entries: chunk starts and piece lengths
 constexpr int kChunkThreads = 1024;
 constexpr int kChunkVec = 8;                       // entries per lane per load round (pass A)
@@ -535,13 +535,18 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_products(unsigned n,
 }
 
 // Pass B: one workgroup per state tile. The tile's entries are numbered flat across its
-// chunk segments (a prefix table in LDS); each lane requests kRowLoads entries (product and
-// slot) at once and places the products into LDS by slot. Then every row is summed in order
-// from R and the max over actions taken.
+// chunk segments (a prefix table in LDS, and a table from each block of 64 flat positions to
+// its first segment); each lane requests kRowLoads entries (product and slot) at once and
+// places the products into LDS by slot. Then every row is summed in order from R and the max
+// over actions taken.
 constexpr unsigned kMaxChunks = 1024;   // chunks a layout may have (the prefix table in LDS)
-constexpr int kRowLoads = 8;
+constexpr int kBufferWord3 = 0x00020000;   // raw buffer descriptor flags (32-bit dword data)
+constexpr size_t kMaxStored = (size_t)1 << 30;   // stored entries: byte offsets fit 32 bits
+constexpr int kRowLoads = 16;           // a tile of 16,384 slots in one round of loads
 
-__global__ __launch_bounds__(kChunkThreads) void vi_chunk_rows(unsigned n, unsigned m,
+// (at most 64 registers: two workgroups of 1,024 lanes per CU)
+__global__ __launch_bounds__(kChunkThreads) __attribute__((amdgpu_waves_per_eu(8)))
+void vi_chunk_rows(unsigned n, unsigned m,
         unsigned statesPerTile, unsigned chunks, unsigned tileBase,
         const uint2 *__restrict__ tileSeg, const float *__restrict__ P,
         const unsigned short *__restrict__ slot, const unsigned char *__restrict__ cnt,
@@ -549,39 +554,74 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_rows(unsigned n, unsig
         int writePi)
 {
     extern __shared__ float lds[];
-    __shared__ uint2 seg[kMaxChunks + 1];
+    __shared__ uint4 blockMap[kSortedEntries / 64];
     const unsigned rowsPerTile = statesPerTile * m;
-    float *Q = lds;                                  // rowsPerTile row sums
-    float *Pl = lds + rowsPerTile;                   // the tile's products by slot
+    uint2 *seg = reinterpret_cast<uint2 *>(lds);     // chunks + 1 (begin, prefix) pairs
+    float *Q = lds + 2 * (chunks + 1);               // rowsPerTile row sums
+    float *Pl = Q + rowsPerTile;                     // the tile's products by slot
     const unsigned tile = tileBase + blockIdx.x, t = threadIdx.x;
     const unsigned s0 = tile * statesPerTile;
     const unsigned states = min(statesPerTile, n - s0);
     const unsigned rows = states * m;
-    // the first row's R and valid length, requested before the gathers
-    const size_t g0 = (size_t)s0 * m + t;
-    const float r0 = t < rows ? R[g0] : 0.0f;
-    const unsigned c0 = t < rows ? cnt[g0] : 0u;
     for (unsigned c = t; c <= chunks; c += kChunkThreads)
         seg[c] = tileSeg[(size_t)tile * (chunks + 1) + c];
     __syncthreads();
+    // block b of 64 flat positions: the segment c holding position 64 b (ceil(prefix / 64) is
+    // the first block starting inside a segment) and the one after it, as (stored position
+    // minus flat position, end) pairs. A position of the block finds its stored position with
+    // one table read when the block meets at most those two segments (segments are hundreds
+    // of entries long), else by a binary search over the prefix table.
+    for (unsigned c = t; c < chunks; c += kChunkThreads) {
+        const unsigned b1 = (seg[c + 1].y + 63) / 64;
+        uint4 e;
+        e.x = seg[c].x - seg[c].y;
+        e.y = seg[c + 1].y;
+        e.z = c + 1 < chunks ? seg[c + 1].x - seg[c + 1].y : 0u;
+        e.w = c + 1 < chunks ? seg[c + 2].y : seg[c + 1].y;
+        for (unsigned b = (seg[c].y + 63) / 64; b < b1; b++)
+            blockMap[b] = e;
+    }
+    __syncthreads();
     const unsigned total = seg[chunks].y;
+    const __amdgpu_buffer_rsrc_t rP = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(P), 0, -1, kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<unsigned short *>(slot), 0, -1, kBufferWord3);
     for (unsigned f0 = t; f0 < total; f0 += kRowLoads * kChunkThreads) {
         float p[kRowLoads];
         unsigned short sl[kRowLoads];
+        unsigned at[kRowLoads];
+        unsigned far = 0;                               // positions past the block's two
 #pragma unroll
         for (int u = 0; u < kRowLoads; u++) {
             const unsigned f = min(f0 + u * kChunkThreads, total - 1);
-            unsigned lo = 0, hi = chunks;             // seg[lo].y <= f < seg[hi].y
-            while (hi - lo > 1) {
-                const unsigned mid = (lo + hi) / 2;
-                if (seg[mid].y <= f)
-                    lo = mid;
-                else
-                    hi = mid;
+            const uint4 e = blockMap[f / 64];
+            at[u] = f + (f < e.y ? e.x : e.z);
+            far |= (f >= e.w ? 1u : 0u) << u;
+        }
+        if (far != 0) {                                 // a block meeting three segments
+#pragma unroll
+            for (int u = 0; u < kRowLoads; u++) {
+                if ((far >> u) & 1u) {
+                    const unsigned f = min(f0 + u * kChunkThreads, total - 1);
+                    unsigned lo = 0, hi = chunks;       // seg[lo].y <= f < seg[hi].y
+                    while (hi - lo > 1) {
+                        const unsigned mid = (lo + hi) / 2;
+                        if (seg[mid].y <= f)
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    at[u] = seg[lo].x + (f - seg[lo].y);
+                }
             }
-            const size_t at = (size_t)seg[lo].x + (f - seg[lo].y);
-            p[u] = __builtin_nontemporal_load(P + at);
-            sl[u] = __builtin_nontemporal_load(slot + at);
+        }
+        // buffer loads take the 32-bit byte offset in one register (the layout keeps fewer
+        // than 2^30 stored entries), streaming past the caches
+#pragma unroll
+        for (int u = 0; u < kRowLoads; u++) {
+            p[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rP, at[u] * 4u, 0, 2));
+            sl[u] = __builtin_amdgcn_raw_buffer_load_b16(rS, at[u] * 2u, 0, 2);
         }
 #pragma unroll
         for (int u = 0; u < kRowLoads; u++)
@@ -591,8 +631,8 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_rows(unsigned n, unsig
     __syncthreads();
     for (unsigned row = t; row < rows; row += kChunkThreads) {
         const size_t g = (size_t)s0 * m + row;
-        float q = row == t ? r0 : R[g];
-        const unsigned c = row == t ? c0 : cnt[g];
+        float q = R[g];
+        const unsigned c = cnt[g];
         for (unsigned i = 0; i < c; i++)
             q = __fadd_rn(q, Pl[i * rowsPerTile + row]);
         Q[row] = q;
@@ -766,7 +806,9 @@ static int layout_wanted(const nova::MDP *mdp)
     if (!common)
         return 0;
     const bool sortedOk = mdp->ns >= 8;
-    const bool chunkedOk = (size_t)mdp->n <= (size_t)kMaxChunks * 32768;
+    // (the chunk groups' alignment padding stays far below the margin of 2^26 entries)
+    const bool chunkedOk = (size_t)mdp->n <= (size_t)kMaxChunks * 32768 &&
+                           slots + ((size_t)1 << 26) < kMaxStored;
     if (const char *f = std::getenv("NOVA_AMD_VI_KERNEL")) {
         if (std::strcmp(f, "tiled") == 0)
             return 0;
@@ -1028,7 +1070,7 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
                 L.partPieces[g / L.chunks + 1] = (unsigned)pieces.size();
         }
         chunkBase[groups] = (unsigned)stored;
-        if (stored >= ((size_t)1 << 32) || pieces.empty())
+        if (stored >= kMaxStored || pieces.empty())
             rc = NOVA_ERROR_INVALID_DATA;
     }
     L.pieceCount = (unsigned)pieces.size();
@@ -1079,9 +1121,11 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
         device_free(cub, fn, "sort storage");
     device_free(tmp, fn, "layout build buffers");
     L.ldsA = sizeof(float) * (size_t)cs;
-    L.ldsB = sizeof(float) * ((size_t)L.statesPerTile * m + L.tileEntries);
-    // the largest dynamic LDS any layout launches with (chunks of 32,768 states; row sums and
-    // products of 16,384 slots each), so that the graphs of layouts alive at once all launch
+    L.ldsB = sizeof(uint2) * ((size_t)L.chunks + 1) +
+             sizeof(float) * ((size_t)L.statesPerTile * m + L.tileEntries);
+    // the largest dynamic LDS any layout launches with (chunks of 32,768 states; the segment
+    // table of kMaxChunks chunks, row sums and products of 16,384 slots each), so that the
+    // graphs of layouts alive at once all launch
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_chunk_products,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1089,7 +1133,8 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_chunk_rows,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(sizeof(float) * 2 * (size_t)kSortedEntries));
+                                (int)(sizeof(uint2) * (kMaxChunks + 1) +
+                                      sizeof(float) * 2 * (size_t)kSortedEntries));
     if (e != hipSuccess || rc != NOVA_SUCCESS) {
         if (L.mem)
             device_free(L.mem, fn, "the chunked sweep layout");

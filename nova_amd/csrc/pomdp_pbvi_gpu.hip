@@ -551,6 +551,39 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
 }
 
 
+// Stages the matrix-core A operand of a chunk of cPad pairs into LDS: w[u][oo] =
+// pw[u] * O[a, sp_u, o0 + oo] for the uLim valid pairs and the zc valid observations of the
+// block of ROWS, zero elsewhere (up to the next whole batch of the workgroup, within the CAP
+// words of w). Every lane requests kWBatch elements' factors at once, with indices clamped into
+// range and the invalid elements zeroed after the loads, so that staging costs a few memory
+// round trips rather than one per element.
+constexpr int kWBatch = 4;
+
+template <int ROWS, int NT, int CAP>
+__device__ __forceinline__ void stage_w(float *__restrict__ w, const int *__restrict__ spU,
+        int cPad, int uLim, int zc, unsigned o0, unsigned z, const NOVA_GLOBAL float *pw,
+        const NOVA_GLOBAL float *Oa, unsigned t)
+{
+    static_assert(CAP % (kWBatch * NT) == 0, "whole batches fit the staging array");
+    const int total = cPad * ROWS;           // <= CAP
+#pragma unroll 1
+    for (int x0 = (int)t; x0 - (int)t < total; x0 += kWBatch * NT) {
+        float pv[kWBatch], ov[kWBatch];
+#pragma unroll
+        for (int k = 0; k < kWBatch; k++) {
+            const int x = x0 + k * NT;
+            const int u = min(x / ROWS, uLim - 1), oo = min(x % ROWS, zc - 1);
+            pv[k] = pw[u];
+            ov[k] = Oa[(size_t)spU[u] * z + o0 + oo];
+        }
+#pragma unroll
+        for (int k = 0; k < kWBatch; k++) {
+            const int x = x0 + k * NT;
+            w[x] = (x / ROWS < uLim && x % ROWS < zc) ? pv[k] * ov[k] : 0.0f;
+        }
+    }
+}
+
 // Fast-mode scoring on the matrix cores. For belief i, action a and a tile of kMTile alphas it
 // forms C[o][j] = sum_u W[u][o] * GT[sp_u][j], W = gamma * b_k * O[a,sp_u,o] * T[s_k,a,l] over
 // the belief's valid (k, l) pairs u, as v_mfma_f32_32x32x2_f32 products: rows are 32
@@ -619,13 +652,7 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
             for (int u = t; u < cPad; u += NT)
                 spU[u] = (c0 + u < uEnd) ? pSp[c0 + u] : 0;
             __syncthreads();
-            for (int x = t; x < cPad * 32; x += NT) {
-                const int u = x / 32, oo = x % 32;
-                float v = 0.0f;
-                if (c0 + u < uEnd && oo < zc)
-                    v = pw[c0 + u] * Oa[(size_t)spU[u] * z + o0 + oo];
-                w[x] = v;
-            }
+            stage_w<32, NT, kWChunk * 32>(w, spU, cPad, uEnd - c0, zc, o0, z, pw + c0, Oa, t);
             __syncthreads();
             // PS pair-steps (2 PS pairs) per trip; the B operands of the next trip are loaded
             // before this trip's MFMAs, so their latency hides behind the matrix work
@@ -759,13 +786,7 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
     for (unsigned o0 = 0; o0 < z; o0 += 32) {
         const int zc = min(32, (int)(z - o0));
         __syncthreads();
-        for (int x = t; x < uPad * 32; x += NT) {
-            const int u = x / 32, oo = x % 32;
-            float v = 0.0f;
-            if (u < uEnd && oo < zc)
-                v = pw[u] * Oa[(size_t)spU[u] * z + o0 + oo];
-            w[x] = v;
-        }
+        stage_w<32, NT, kWAll * 32>(w, spU, uPad, uEnd, zc, o0, z, pw, Oa, t);
         __syncthreads();
         for (unsigned tile = 0; tile < P.tiles; tile++) {
             const unsigned jw = tile * kMTile + wave * NB * 32;
@@ -917,13 +938,7 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
             for (int u = t; u < cPad; u += NT)
                 spU[u] = (c0 + u < uEnd) ? pSp[c0 + u] : 0;
             __syncthreads();
-            for (int x = t; x < cPad * 16; x += NT) {
-                const int u = x / 16, oo = x % 16;
-                float v = 0.0f;
-                if (c0 + u < uEnd && oo < zc)
-                    v = pw[c0 + u] * Oa[(size_t)spU[u] * z + o0 + oo];
-                w[x] = v;
-            }
+            stage_w<16, NT, kW16Chunk * 16>(w, spU, cPad, uEnd - c0, zc, o0, z, pw + c0, Oa, t);
             __syncthreads();
             float fb[PS][NB], nb[PS][NB];
             auto loadB = [&](int u0, float (&dst)[PS][NB]) {

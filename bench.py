@@ -285,7 +285,11 @@ def bench_pbvi(args):
                         "kernel": {8: "pbvi_score_mfma<4 waves>", 16: "pbvi_score_mfma<8 waves>",
                                    32: "pbvi_score_mfma<8 waves, 8-step prefetch>",
                                    64: "pbvi_score_mfma16 (16 observation rows)",
-                                   128: "pbvi_score_mfma_all (every alpha tile per workgroup)"}.get(
+                                   128: "pbvi_score_mfma_all (every alpha tile per workgroup)",
+                                   256: "pbvi_score_mfma<4 waves, ring of 4 operand sets>",
+                                   512: "pbvi_score_mfma<8 waves, ring of 4 operand sets>",
+                                   1024: "pbvi_score_mfma16 (ring of 4 operand sets)",
+                                   2048: "pbvi_score_mfma_all (ring of 4 operand sets)"}.get(
                                        variant, "pbvi_score<fast,%d>" % variant),
                         "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": achieved / 157.3,

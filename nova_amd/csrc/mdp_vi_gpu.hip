@@ -540,7 +540,6 @@ __global__ __launch_bounds__(kChunkThreads) void vi_chunk_products(unsigned n,
 // places the products into LDS by slot. Then every row is summed in order from R and the max
 // over actions taken.
 constexpr unsigned kMaxChunks = 1024;   // chunks a layout may have (the prefix table in LDS)
-constexpr int kBufferWord3 = 0x00020000;   // raw buffer descriptor flags (32-bit dword data)
 constexpr size_t kMaxStored = (size_t)1 << 30;   // stored entries: byte offsets fit 32 bits
 constexpr int kRowLoads = 16;           // a tile of 16,384 slots in one round of loads
 
@@ -557,14 +556,37 @@ void vi_chunk_rows(unsigned n, unsigned m,
     __shared__ uint4 blockMap[kSortedEntries / 64];
     const unsigned rowsPerTile = statesPerTile * m;
     uint2 *seg = reinterpret_cast<uint2 *>(lds);     // chunks + 1 (begin, prefix) pairs
-    float *Q = lds + 2 * (chunks + 1);               // rowsPerTile row sums
+    unsigned char *cntL = reinterpret_cast<unsigned char *>(lds + 2 * (chunks + 1));
+    float *Q = lds + 2 * (chunks + 1) + (rowsPerTile + 3) / 4;   // rowsPerTile row sums
     float *Pl = Q + rowsPerTile;                     // the tile's products by slot
     const unsigned tile = tileBase + blockIdx.x, t = threadIdx.x;
     const unsigned s0 = tile * statesPerTile;
     const unsigned states = min(statesPerTile, n - s0);
     const unsigned rows = states * m;
-    for (unsigned c = t; c <= chunks; c += kChunkThreads)
-        seg[c] = tileSeg[(size_t)tile * (chunks + 1) + c];
+    // the segment table, and R and the valid lengths of the rows (Q starts as R), requested
+    // together so that their latencies overlap
+    const size_t segBase = (size_t)tile * (chunks + 1), g0 = (size_t)s0 * m;
+    uint2 sg0 = make_uint2(0u, 0u);
+    float r0 = 0.0f;
+    unsigned c0 = 0;
+    if (t <= chunks)
+        sg0 = tileSeg[segBase + t];
+    if (t < rows) {
+        r0 = R[g0 + t];
+        c0 = cnt[g0 + t];
+    }
+    if (t <= chunks)
+        seg[t] = sg0;
+    if (t < rows) {
+        Q[t] = r0;
+        cntL[t] = (unsigned char)c0;
+    }
+    for (unsigned c = t + kChunkThreads; c <= chunks; c += kChunkThreads)
+        seg[c] = tileSeg[segBase + c];
+    for (unsigned row = t + kChunkThreads; row < rows; row += kChunkThreads) {
+        Q[row] = R[g0 + row];
+        cntL[row] = cnt[g0 + row];
+    }
     __syncthreads();
     // block b of 64 flat positions: the segment c holding position 64 b (ceil(prefix / 64) is
     // the first block starting inside a segment) and the one after it, as (stored position
@@ -584,9 +606,9 @@ void vi_chunk_rows(unsigned n, unsigned m,
     __syncthreads();
     const unsigned total = seg[chunks].y;
     const __amdgpu_buffer_rsrc_t rP = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<float *>(P), 0, -1, kBufferWord3);
+            const_cast<float *>(P), 0, -1, kRawBufferWord3);
     const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<unsigned short *>(slot), 0, -1, kBufferWord3);
+            const_cast<unsigned short *>(slot), 0, -1, kRawBufferWord3);
     for (unsigned f0 = t; f0 < total; f0 += kRowLoads * kChunkThreads) {
         float p[kRowLoads];
         unsigned short sl[kRowLoads];
@@ -630,9 +652,8 @@ void vi_chunk_rows(unsigned n, unsigned m,
     }
     __syncthreads();
     for (unsigned row = t; row < rows; row += kChunkThreads) {
-        const size_t g = (size_t)s0 * m + row;
-        float q = R[g];
-        const unsigned c = cnt[g];
+        float q = Q[row];
+        const unsigned c = cntL[row];
         for (unsigned i = 0; i < c; i++)
             q = __fadd_rn(q, Pl[i * rowsPerTile + row]);
         Q[row] = q;
@@ -1122,10 +1143,11 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
     device_free(tmp, fn, "layout build buffers");
     L.ldsA = sizeof(float) * (size_t)cs;
     L.ldsB = sizeof(uint2) * ((size_t)L.chunks + 1) +
+             ((size_t)L.statesPerTile * m + 3) / 4 * 4 +
              sizeof(float) * ((size_t)L.statesPerTile * m + L.tileEntries);
     // the largest dynamic LDS any layout launches with (chunks of 32,768 states; the segment
-    // table of kMaxChunks chunks, row sums and products of 16,384 slots each), so that the
-    // graphs of layouts alive at once all launch
+    // table of kMaxChunks chunks, row lengths, row sums and products of 16,384 slots each), so
+    // that the graphs of layouts alive at once all launch
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_chunk_products,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1133,7 +1155,7 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
     if (e == hipSuccess && rc == NOVA_SUCCESS)
         e = hipFuncSetAttribute((const void *)vi_chunk_rows,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(sizeof(uint2) * (kMaxChunks + 1) +
+                                (int)(sizeof(uint2) * (kMaxChunks + 1) + kSortedEntries +
                                       sizeof(float) * 2 * (size_t)kSortedEntries));
     if (e != hipSuccess || rc != NOVA_SUCCESS) {
         if (L.mem)

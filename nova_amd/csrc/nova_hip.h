@@ -10,6 +10,10 @@
 
 namespace nova_amd {
 
+// Word 3 of a raw buffer resource (32-bit dword data, no format conversion) for
+// __builtin_amdgcn_make_buffer_rsrc: buffer loads take a 32-bit byte offset in one register.
+constexpr int kRawBufferWord3 = 0x00020000;
+
 // The library's stream on the current device: created on first use with the default
 // priority and as a blocking stream, so it orders after work on the legacy null stream (the
 // stream PyTorch and plain HIP callers use by default). Every ABI call is synchronous at

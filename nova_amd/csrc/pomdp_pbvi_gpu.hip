@@ -594,14 +594,17 @@ __device__ __forceinline__ void stage_w(float *__restrict__ w, const int *__rest
 constexpr int kMTile = 512;          // alphas per workgroup
 constexpr int kWChunk = 256;         // support pairs per staged W chunk
 
-template <int NW, int NB, int PS>
+template <int NW, int NB, int PS, int NSET, bool BUF>
 __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     static_assert(NW * NB * 32 == kMTile, "a workgroup covers one alpha tile");
-    static_assert(kWChunk % (4 * PS) == 0, "whole loop steps per chunk");
+    static_assert(NSET >= 2, "at least one trip's operands in flight");
     constexpr int NT = NW * 64;
-    constexpr int STEP = 4 * PS;              // pairs per loop step (two trips of PS pair-steps)
+    constexpr int HALF = 2 * PS;              // pairs per trip (PS pair-steps of two pairs)
+    constexpr int RING = NSET * HALF;         // pairs per turn of the ring of NSET register sets
+    static_assert(kWChunk % RING == 0, "whole turns of the ring per chunk");
+    static_assert(NSET * PS * NB <= 63, "a turn of the ring fits the vector-memory counter");
     __shared__ int spU[kWChunk];              // successor of each pair of the current chunk
     __shared__ float w[kWChunk * 32];         // [u][o] for the current chunk and 32 observations
     __shared__ float redV[NW][32];
@@ -625,9 +628,9 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
     // the precomputed pair list of (i, a), any length, staged chunk by chunk (padded with state 0
-    // to whole loop steps of pairs, whose factors are 0)
+    // to whole turns of the ring, whose factors are 0)
     const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
-    const int uEnd16 = (uEnd + STEP - 1) / STEP * STEP;
+    const int uEndR = (uEnd + RING - 1) / RING * RING;
     const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
     const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
@@ -637,6 +640,8 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     // only feed C columns the epilogue ignores
     const unsigned jl = jw + lane % 32;
     const size_t rS = P.rS;
+    const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>((const float *)P.GT), 0, -1, kRawBufferWord3);
 
     for (unsigned o0 = 0; o0 < z; o0 += 32) {
         const int zc = min(32, (int)(z - o0));
@@ -647,24 +652,33 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
             for (int v = 0; v < 16; v++)
                 acc[y][v] = 0.0f;
         for (int c0 = 0; c0 < uEnd; c0 += kWChunk) {
-            const int cPad = min(kWChunk, uEnd16 - c0);     // a multiple of STEP
+            const int cPad = min(kWChunk, uEndR - c0);      // a multiple of RING
             __syncthreads();
             for (int u = t; u < cPad; u += NT)
                 spU[u] = (c0 + u < uEnd) ? pSp[c0 + u] : 0;
             __syncthreads();
             stage_w<32, NT, kWChunk * 32>(w, spU, cPad, uEnd - c0, zc, o0, z, pw + c0, Oa, t);
             __syncthreads();
-            // PS pair-steps (2 PS pairs) per trip; the B operands of the next trip are loaded
-            // before this trip's MFMAs, so their latency hides behind the matrix work
-            float fb[PS][NB], nb[PS][NB];
+            // PS pair-steps (2 PS pairs) per trip; the B operands of a trip are requested
+            // NSET - 1 trips ahead of its MFMAs, so their latency hides behind the matrix work
+            float bq[NSET][PS][NB];
             auto loadB = [&](int u0, float (&dst)[PS][NB]) {
 #pragma unroll
                 for (int q = 0; q < PS; q++) {
                     const int uu = u0 + 2 * q + (int)(lane / 32);
-                    const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+                    if constexpr (BUF) {
+                        // one 32-bit byte offset per row (GT is below 4 GB for these variants)
+                        const unsigned o = ((unsigned)spU[uu] * (unsigned)rS + jl) * 4u;
 #pragma unroll
-                    for (int y = 0; y < NB; y++)
-                        dst[q][y] = row[y * 32];
+                        for (int y = 0; y < NB; y++)
+                            dst[q][y] = __uint_as_float(
+                                    __builtin_amdgcn_raw_buffer_load_b32(rG, o, y * 128, 0));
+                    } else {
+                        const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+#pragma unroll
+                        for (int y = 0; y < NB; y++)
+                            dst[q][y] = row[y * 32];
+                    }
                 }
             };
             auto mfmaTrip = [&](int u0, const float (&b)[PS][NB]) {
@@ -678,22 +692,29 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
                     for (int y = 0; y < NB; y++)
                         acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[q], b[q][y], acc[y], 0, 0, 0);
             };
-            // ping-pong between two register sets: trip t's operands were requested one trip
-            // earlier. The loads carry no conditions (the last prefetch re-reads pairs already
-            // staged), so the compiler can count the outstanding ones exactly.
-            // sched_barrier keeps the scheduler from sinking the prefetch below the MFMAs.
+            // a ring of NSET register sets: trip t's operands sit in set t mod NSET, requested
+            // right after the MFMAs of trip t - NSET consumed that set, so NSET - 1 trips of
+            // loads are in flight behind each trip's matrix work (cPad is a whole number of
+            // turns). The loads carry no conditions and the last turn issues none, so the
+            // compiler counts the outstanding ones exactly and nothing is left in flight after
+            // the chunk. sched_barrier keeps the scheduler from moving the loads across the
+            // MFMAs.
             if (jw < r) {       // a wave whose alphas all lie past r has nothing to score
-                loadB(0, fb);
-                for (int u0 = 0; u0 < cPad; u0 += STEP) {
-                    loadB(u0 + STEP / 2, nb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    mfmaTrip(u0, fb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    loadB(min(u0 + STEP, cPad - STEP / 2), fb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    mfmaTrip(u0 + STEP / 2, nb);
-                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int s0 = 0; s0 < NSET; s0++)
+                    loadB(s0 * HALF, bq[s0]);
+                for (int u0 = 0; u0 < cPad - RING; u0 += RING) {
+#pragma unroll
+                    for (int j = 0; j < NSET; j++) {
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfmaTrip(u0 + j * HALF, bq[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        loadB(u0 + (j + NSET) * HALF, bq[j]);
+                    }
                 }
+#pragma unroll
+                for (int j = 0; j < NSET; j++)      // the last turn requests nothing more
+                    mfmaTrip(cPad - RING + j * HALF, bq[j]);
             }
         }
         // per row (observation): first maximiser over the wave's alphas, then across the waves
@@ -746,14 +767,17 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
 // most kWAll pairs (the tuner offers it then; longer lists use the kernel above).
 constexpr int kWAll = 512;
 
-template <int PS>
+template <int PS, int NSET, bool BUF>
 __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     constexpr int NW = 8, NB = 2, NT = NW * 64;
     static_assert(NW * NB * 32 == kMTile, "a workgroup sweeps whole alpha tiles");
-    constexpr int STEP = 4 * PS;
-    static_assert(kWAll % STEP == 0, "whole loop steps");
+    static_assert(NSET >= 2, "at least one trip's operands in flight");
+    constexpr int HALF = 2 * PS;              // pairs per trip
+    constexpr int RING = NSET * HALF;         // pairs per turn of the ring of NSET register sets
+    static_assert(kWAll % RING == 0, "whole turns of the ring");
+    static_assert(NSET * PS * NB <= 63, "a turn of the ring fits the vector-memory counter");
     __shared__ int spU[kWAll];
     __shared__ float w[kWAll * 32];
     __shared__ float redV[NW][32];
@@ -774,12 +798,14 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
     const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
-    const int uPad = (uEnd + STEP - 1) / STEP * STEP;             // <= kWAll
+    const int uPad = (uEnd + RING - 1) / RING * RING;             // <= kWAll
     const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
     const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
     const size_t rS = P.rS;
+    const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>((const float *)P.GT), 0, -1, kRawBufferWord3);
     for (int u = t; u < uPad; u += NT)
         spU[u] = (u < uEnd) ? pSp[u] : 0;
 
@@ -798,18 +824,27 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
                 for (int v = 0; v < 16; v++)
                     acc[y][v] = 0.0f;
             if (uPad > 0 && jw < r) {   // waves whose alphas all lie past r skip
-                float fb[PS][NB], nb[PS][NB];
+                float bq[NSET][PS][NB];
                 auto loadB = [&](int u0, float (&dst)[PS][NB]) {
 #pragma unroll
                     for (int qq = 0; qq < PS; qq++) {
                         const int uu = u0 + 2 * qq + (int)(lane / 32);
-                        const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+                        if constexpr (BUF) {
+                            // one 32-bit byte offset per row (GT is below 4 GB here)
+                            const unsigned o = ((unsigned)spU[uu] * (unsigned)rS + jl) * 4u;
 #pragma unroll
-                        for (int y = 0; y < NB; y++)
-                            dst[qq][y] = row[y * 32];
+                            for (int y = 0; y < NB; y++)
+                                dst[qq][y] = __uint_as_float(
+                                        __builtin_amdgcn_raw_buffer_load_b32(rG, o, y * 128, 0));
+                        } else {
+                            const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+#pragma unroll
+                            for (int y = 0; y < NB; y++)
+                                dst[qq][y] = row[y * 32];
+                        }
                     }
                 };
-                auto mfmaTrip = [&](int u0, const float (&bq)[PS][NB]) {
+                auto mfmaTrip = [&](int u0, const float (&bb)[PS][NB]) {
                     float fa[PS];
 #pragma unroll
                     for (int qq = 0; qq < PS; qq++)
@@ -818,20 +853,25 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
                     for (int qq = 0; qq < PS; qq++)
 #pragma unroll
                         for (int y = 0; y < NB; y++)
-                            acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[qq], bq[qq][y],
+                            acc[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[qq], bb[qq][y],
                                                                           acc[y], 0, 0, 0);
                 };
-                loadB(0, fb);
-                for (int u0 = 0; u0 < uPad; u0 += STEP) {
-                    loadB(u0 + STEP / 2, nb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    mfmaTrip(u0, fb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    loadB(min(u0 + STEP, uPad - STEP / 2), fb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    mfmaTrip(u0 + STEP / 2, nb);
-                    __builtin_amdgcn_sched_barrier(0);
+                // the ring of register sets of the per-tile kernel above
+#pragma unroll
+                for (int s0 = 0; s0 < NSET; s0++)
+                    loadB(s0 * HALF, bq[s0]);
+                for (int u0 = 0; u0 < uPad - RING; u0 += RING) {
+#pragma unroll
+                    for (int j = 0; j < NSET; j++) {
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfmaTrip(u0 + j * HALF, bq[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        loadB(u0 + (j + NSET) * HALF, bq[j]);
+                    }
                 }
+#pragma unroll
+                for (int j = 0; j < NSET; j++)      // the last turn requests nothing more
+                    mfmaTrip(uPad - RING + j * HALF, bq[j]);
             }
 #pragma unroll
             for (int v = 0; v < 16; v++) {
@@ -886,14 +926,17 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
 constexpr int kW16Chunk = 256;       // support pairs per staged W chunk
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
-template <int PS>
+template <int PS, int NSET, bool BUF>
 __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     constexpr int NW = 8, NB = 4, NT = NW * 64;
     static_assert(NW * NB * 16 == kMTile, "a workgroup covers one alpha tile");
-    constexpr int STEP = 8 * PS;              // pairs per loop step (two trips of PS MFMAs)
-    static_assert(kW16Chunk % STEP == 0, "whole loop steps per chunk");
+    static_assert(NSET >= 2, "at least one trip's operands in flight");
+    constexpr int HALF = 4 * PS;              // pairs per trip (PS MFMAs of four pairs)
+    constexpr int RING = NSET * HALF;         // pairs per turn of the ring of NSET register sets
+    static_assert(kW16Chunk % RING == 0, "whole turns of the ring per chunk");
+    static_assert(NSET * PS * NB <= 63, "a turn of the ring fits the vector-memory counter");
     __shared__ int spU[kW16Chunk];
     __shared__ float w[kW16Chunk * 16];       // [u][o] for the current chunk, 16 observations
     __shared__ float redV[NW][16];
@@ -915,7 +958,7 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
     const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
-    const int uEndP = (uEnd + STEP - 1) / STEP * STEP;
+    const int uEndP = (uEnd + RING - 1) / RING * RING;
     const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
     const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
     const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
@@ -923,6 +966,8 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
     const unsigned jw = tile * kMTile + wave * NB * 16;
     const unsigned jl = jw + lane % 16;
     const size_t rS = P.rS;
+    const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>((const float *)P.GT), 0, -1, kRawBufferWord3);
 
     for (unsigned o0 = 0; o0 < z; o0 += 16) {
         const int zc = min(16, (int)(z - o0));
@@ -933,22 +978,31 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
             for (int v = 0; v < 4; v++)
                 acc[y][v] = 0.0f;
         for (int c0 = 0; c0 < uEnd; c0 += kW16Chunk) {
-            const int cPad = min(kW16Chunk, uEndP - c0);     // a multiple of STEP
+            const int cPad = min(kW16Chunk, uEndP - c0);     // a multiple of RING
             __syncthreads();
             for (int u = t; u < cPad; u += NT)
                 spU[u] = (c0 + u < uEnd) ? pSp[c0 + u] : 0;
             __syncthreads();
             stage_w<16, NT, kW16Chunk * 16>(w, spU, cPad, uEnd - c0, zc, o0, z, pw + c0, Oa, t);
             __syncthreads();
-            float fb[PS][NB], nb[PS][NB];
+            float bq[NSET][PS][NB];
             auto loadB = [&](int u0, float (&dst)[PS][NB]) {
 #pragma unroll
                 for (int q = 0; q < PS; q++) {
                     const int uu = u0 + 4 * q + (int)(lane / 16);
-                    const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+                    if constexpr (BUF) {
+                        // one 32-bit byte offset per row (GT is below 4 GB for these variants)
+                        const unsigned o = ((unsigned)spU[uu] * (unsigned)rS + jl) * 4u;
 #pragma unroll
-                    for (int y = 0; y < NB; y++)
-                        dst[q][y] = row[y * 16];
+                        for (int y = 0; y < NB; y++)
+                            dst[q][y] = __uint_as_float(
+                                    __builtin_amdgcn_raw_buffer_load_b32(rG, o, y * 64, 0));
+                    } else {
+                        const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+#pragma unroll
+                        for (int y = 0; y < NB; y++)
+                            dst[q][y] = row[y * 16];
+                    }
                 }
             };
             auto mfmaTrip = [&](int u0, const float (&bq)[PS][NB]) {
@@ -963,18 +1017,23 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
                         acc[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[q], bq[q][y], acc[y],
                                                                       0, 0, 0);
             };
+            // the ring of register sets of the 32-row kernel above
             if (jw < r) {       // a wave whose alphas all lie past r has nothing to score
-                loadB(0, fb);
-                for (int u0 = 0; u0 < cPad; u0 += STEP) {
-                    loadB(u0 + STEP / 2, nb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    mfmaTrip(u0, fb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    loadB(min(u0 + STEP, cPad - STEP / 2), fb);
-                    __builtin_amdgcn_sched_barrier(0);
-                    mfmaTrip(u0 + STEP / 2, nb);
-                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int s0 = 0; s0 < NSET; s0++)
+                    loadB(s0 * HALF, bq[s0]);
+                for (int u0 = 0; u0 < cPad - RING; u0 += RING) {
+#pragma unroll
+                    for (int j = 0; j < NSET; j++) {
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfmaTrip(u0 + j * HALF, bq[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        loadB(u0 + (j + NSET) * HALF, bq[j]);
+                    }
                 }
+#pragma unroll
+                for (int j = 0; j < NSET; j++)      // the last turn requests nothing more
+                    mfmaTrip(cPad - RING + j * HALF, bq[j]);
             }
         }
         // per row (observation): first maximiser over the wave's alphas, then across the waves
@@ -1700,7 +1759,7 @@ struct DenseWs;
 
 // Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32, 64, 128); PbviPlan::totSc and the
 // offset table hold one entry per variant.
-constexpr int kVariants = 8;
+constexpr int kVariants = 12;
 
 struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
@@ -1721,7 +1780,8 @@ static_assert(sizeof(PbviPlan::totSc) == kVariants * sizeof(unsigned),
 static int jpt_slot(unsigned jpt)
 {
     return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : (jpt == 16 ? 4 :
-           (jpt == 32 ? 5 : (jpt == 64 ? 6 : 7))))));
+           (jpt == 32 ? 5 : (jpt == 64 ? 6 : (jpt == 128 ? 7 : (jpt == 256 ? 8 :
+           (jpt == 512 ? 9 : (jpt == 1024 ? 10 : 11))))))))));
 }
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
@@ -1736,11 +1796,15 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
         if (jpt == 1) hipLaunchKernelGGL((pbvi_score<false, 1>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
         if (jpt == 2) hipLaunchKernelGGL((pbvi_score<false, 2>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
         if (jpt == 4) hipLaunchKernelGGL((pbvi_score<false, 4>), g, dim3(kPB), 0, st, dInst, sc, c, iter);
-        if (jpt == 8) hipLaunchKernelGGL((pbvi_score_mfma<4, 4, 4>), g, dim3(256), 0, st, dInst, sc, c, iter);
-        if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 4>), g, dim3(512), 0, st, dInst, sc, c, iter);
-        if (jpt == 32) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 8>), g, dim3(512), 0, st, dInst, sc, c, iter);
-        if (jpt == 64) hipLaunchKernelGGL((pbvi_score_mfma16<2>), g, dim3(512), 0, st, dInst, sc, c, iter);
-        if (jpt == 128) hipLaunchKernelGGL((pbvi_score_mfma_all<4>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 8) hipLaunchKernelGGL((pbvi_score_mfma<4, 4, 4, 2, false>), g, dim3(256), 0, st, dInst, sc, c, iter);
+        if (jpt == 16) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 4, 2, false>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 32) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 8, 2, false>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 256) hipLaunchKernelGGL((pbvi_score_mfma<4, 4, 2, 4, true>), g, dim3(256), 0, st, dInst, sc, c, iter);
+        if (jpt == 512) hipLaunchKernelGGL((pbvi_score_mfma<8, 2, 4, 4, true>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 64) hipLaunchKernelGGL((pbvi_score_mfma16<2, 2, false>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 1024) hipLaunchKernelGGL((pbvi_score_mfma16<2, 4, true>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 128) hipLaunchKernelGGL((pbvi_score_mfma_all<4, 2, false>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 2048) hipLaunchKernelGGL((pbvi_score_mfma_all<4, 4, true>), g, dim3(512), 0, st, dInst, sc, c, iter);
     }
 }
 
@@ -1772,7 +1836,8 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
         for (int v = 0; v < kVariants; v++)
             scs[v][b + 1] = scs[v][b] +
                             ((four && jpt_slot(I.jpt) == v)
-                             ? I.rb * I.m * (I.jpt == 128 ? 1u : I.tiles) : 0u);
+                             ? I.rb * I.m * ((I.jpt == 128 || I.jpt == 2048) ? 1u : I.tiles)
+                             : 0u);
         be[b + 1] = be[b] + (four ? I.rb : 0u);
         fu[b + 1] = fu[b] + (four ? 0u : I.rb);
         if (!four)
@@ -1931,7 +1996,7 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     if (pl.totBe == 0)
         return;
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64, 128};
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048};
     for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
@@ -1980,9 +2045,13 @@ static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mod
 // The scoring kernel's variants (1, 2 or 4 alphas per lane) trade occupancy against reuse of
 // the staged O*T factors. For each size class (r, belief-support pairs rz*ns and z, each
 // bucketed to a power of two) the first instance seen is timed with every variant on one
-// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT=1|2|4|8|16|32
-// forces a variant (8, 16, 32: the matrix-core kernel with 4 waves, 8 waves, or 8 waves and a
-// two-fold prefetch; fast mode only);
+// scoring launch and the fastest is remembered for the process. NOVA_AMD_PBVI_JPT forces a
+// variant: 1, 2, 4 (vector ALUs), and in fast mode only the matrix-core kernels 8, 16, 32 (4
+// waves; 8 waves; 8 waves with twice the pairs per trip), 256 and 512 (the 4- and 8-wave kernels
+// with operands requested three trips ahead, 32 loads in flight per wave), 64 (16 observation
+// rows, z <= 16), 1024 (the 16-row kernel with operands requested three trips ahead), 128
+// (every alpha tile in one workgroup, pair lists of at most 512 pairs) and 2048 (that one with
+// operands requested three trips ahead);
 // NOVA_AMD_AUTOTUNE=0 disables timing (default variant 2).
 
 struct TuneKey {
@@ -2003,6 +2072,12 @@ static unsigned bucket(unsigned x)
     return p;
 }
 
+// Gamma^T of the instance spans less than 4 GB (byte offsets of 32 bits reach all of it).
+static bool gt_below_4g(const PbviInst &I)
+{
+    return (size_t)I.n * I.rS * sizeof(float) < ((size_t)1 << 32);
+}
+
 static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
 {
     // the vector-ALU variants stage the support in LDS; longer supports take the matrix cores
@@ -2012,6 +2087,9 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
         const int v = std::atoi(f);
         if (((v == 1 || v == 2 || v == 4) && valuOk) ||
                 ((v == 8 || v == 16 || v == 32) && mode != 1) ||
+                ((v == 256 || v == 512) && mode != 1 && gt_below_4g(I)) ||
+                (v == 1024 && mode != 1 && I.z <= 16 && gt_below_4g(I)) ||
+                (v == 2048 && mode != 1 && I.kmax <= (unsigned)kWAll && gt_below_4g(I)) ||
                 (v == 64 && mode != 1 && I.z <= 16) ||
                 (v == 128 && mode != 1 && I.kmax <= (unsigned)kWAll))
             return (unsigned)v;
@@ -2028,13 +2106,15 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u, 128u}) {
+    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u, 128u, 256u, 512u, 1024u, 2048u}) {
         if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
-        if (jpt == 64 && I.z > 16)
-            continue;                          // the 16-row kernel: at most 16 observations
-        if (jpt == 128 && I.kmax > (unsigned)kWAll)
+        if ((jpt == 64 || jpt == 1024) && I.z > 16)
+            continue;                          // the 16-row kernels: at most 16 observations
+        if ((jpt == 128 || jpt == 2048) && I.kmax > (unsigned)kWAll)
             continue;                          // the all-tiles kernel stages every pair once
+        if ((jpt == 256 || jpt == 512 || jpt == 1024 || jpt == 2048) && !gt_below_4g(I))
+            continue;                          // their buffer loads take 32-bit byte offsets
         if (jpt < 8 && !valuOk)
             continue;
         PbviPlan pl;

@@ -101,7 +101,8 @@ class POMDPPBVIGPU(_abi.NovaPOMDPPBVIGPU):
         """1, 2, 4: alphas per lane on the vector ALUs; 8, 16, 32: fp32 matrix cores (4 waves,
         8 waves, 8 waves with a deeper prefetch); 64: fp32 matrix cores with 16 observation
         rows (z <= 16); 128: fp32 matrix cores, every alpha tile in one workgroup (rz * ns <=
-        512); 0: dense GEMM."""
+        512); 256, 512: the 4- and 8-wave matrix-core kernels with operands requested three
+        trips ahead; 0: dense GEMM."""
         v = ct.c_uint(0)
         if self._lib.pomdp_pbvi_scoring_variant_gpu(self, ct.byref(v)) != 0:
             raise Exception("pomdp_pbvi_scoring_variant_gpu failed")

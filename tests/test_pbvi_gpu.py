@@ -138,7 +138,7 @@ def sparse_values(P, G):
     return V
 
 
-@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8", "16", "32", "64", "128"])
+@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8", "16", "32", "64", "128", "256", "512", "1024", "2048"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES, ids=[s[0] for s in SHAPES])
 def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, variant):
     """From the same Gamma (the oracle's after t iterations), one fast GPU iteration chooses
@@ -146,7 +146,9 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
     one (1 and 4 alphas per lane on the vector ALUs; 8, 16, 32 = the matrix-core kernel with 4
     waves, 8 waves, 8 waves and a deeper prefetch; 64 = its 16-observation-row form, taken
     only when z <= 16; 128 = its form that sweeps every alpha tile in one workgroup, taken when
-    rz * ns <= 512; otherwise the tuned variant runs). A forced variant also turns off the
+    rz * ns <= 512; 256 and 512 = the 4- and 8-wave kernels with a ring of four operand sets
+    and buffer loads, 1024 and 2048 = the same for the 16-row and the all-tiles kernels;
+    otherwise the tuned variant runs). A forced variant also turns off the
     one-launch path of small instances, which would otherwise run instead of it."""
     if variant != "tuned":
         monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
@@ -183,7 +185,7 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
           % (name, variant, mismatch, total))
 
 
-@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64", "128"])
+@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64", "128", "256", "512", "1024", "2048"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES[:2], ids=[s[0] for s in SHAPES[:2]])
 def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag, variant):
     if variant != "tuned":

@@ -357,7 +357,8 @@ __global__ __launch_bounds__(kSortedThreads) void vi_sweep_sorted(unsigned n, un
 // sums are those of the sweeps above, so V' and pi are the same bit for bit.
 // Per valid entry: 2 + 4 bytes read and 4 written by pass A, 4 + 2 read by pass B.
 constexpr unsigned kChunkStatesDefault = 32768;    // 128 KB of V per pass-A workgroup
-constexpr unsigned kPieceEntriesDefault = 65536;   // entries per pass-A workgroup
+constexpr unsigned kPieceEntriesDefault = 32768;   // entries per pass-A workgroup
+constexpr unsigned kChunkTileDefault = 8192;       // slots per pass-B tile
 constexpr unsigned kChunkAlign = 64;               // entries: chunk starts and piece lengths
 constexpr int kChunkThreads = 1024;
 constexpr int kChunkVec = 8;                       // entries per lane per load round (pass A)
@@ -543,8 +544,9 @@ constexpr unsigned kMaxChunks = 1024;   // chunks a layout may have (the prefix 
 constexpr size_t kMaxStored = (size_t)1 << 30;   // stored entries: byte offsets fit 32 bits
 constexpr int kRowLoads = 16;           // a tile of 16,384 slots in one round of loads
 
-// (at most 64 registers: two workgroups of 1,024 lanes per CU)
-__global__ __launch_bounds__(kChunkThreads) __attribute__((amdgpu_waves_per_eu(8)))
+// (at most 64 registers: 32 waves per CU, two workgroups of 1,024 lanes or four of 512)
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8)))
 void vi_chunk_rows(unsigned n, unsigned m,
         unsigned statesPerTile, unsigned chunks, unsigned tileBase,
         const uint2 *__restrict__ tileSeg, const float *__restrict__ P,
@@ -581,9 +583,9 @@ void vi_chunk_rows(unsigned n, unsigned m,
         Q[t] = r0;
         cntL[t] = (unsigned char)c0;
     }
-    for (unsigned c = t + kChunkThreads; c <= chunks; c += kChunkThreads)
+    for (unsigned c = t + NT; c <= chunks; c += NT)
         seg[c] = tileSeg[segBase + c];
-    for (unsigned row = t + kChunkThreads; row < rows; row += kChunkThreads) {
+    for (unsigned row = t + NT; row < rows; row += NT) {
         Q[row] = R[g0 + row];
         cntL[row] = cnt[g0 + row];
     }
@@ -593,7 +595,7 @@ void vi_chunk_rows(unsigned n, unsigned m,
     // minus flat position, end) pairs. A position of the block finds its stored position with
     // one table read when the block meets at most those two segments (segments are hundreds
     // of entries long), else by a binary search over the prefix table.
-    for (unsigned c = t; c < chunks; c += kChunkThreads) {
+    for (unsigned c = t; c < chunks; c += NT) {
         const unsigned b1 = (seg[c + 1].y + 63) / 64;
         uint4 e;
         e.x = seg[c].x - seg[c].y;
@@ -609,14 +611,14 @@ void vi_chunk_rows(unsigned n, unsigned m,
             const_cast<float *>(P), 0, -1, kRawBufferWord3);
     const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<unsigned short *>(slot), 0, -1, kRawBufferWord3);
-    for (unsigned f0 = t; f0 < total; f0 += kRowLoads * kChunkThreads) {
+    for (unsigned f0 = t; f0 < total; f0 += kRowLoads * NT) {
         float p[kRowLoads];
         unsigned short sl[kRowLoads];
         unsigned at[kRowLoads];
         unsigned far = 0;                               // positions past the block's two
 #pragma unroll
         for (int u = 0; u < kRowLoads; u++) {
-            const unsigned f = min(f0 + u * kChunkThreads, total - 1);
+            const unsigned f = min(f0 + u * NT, total - 1);
             const uint4 e = blockMap[f / 64];
             at[u] = f + (f < e.y ? e.x : e.z);
             far |= (f >= e.w ? 1u : 0u) << u;
@@ -625,7 +627,7 @@ void vi_chunk_rows(unsigned n, unsigned m,
 #pragma unroll
             for (int u = 0; u < kRowLoads; u++) {
                 if ((far >> u) & 1u) {
-                    const unsigned f = min(f0 + u * kChunkThreads, total - 1);
+                    const unsigned f = min(f0 + u * NT, total - 1);
                     unsigned lo = 0, hi = chunks;       // seg[lo].y <= f < seg[hi].y
                     while (hi - lo > 1) {
                         const unsigned mid = (lo + hi) / 2;
@@ -647,11 +649,11 @@ void vi_chunk_rows(unsigned n, unsigned m,
         }
 #pragma unroll
         for (int u = 0; u < kRowLoads; u++)
-            if (f0 + u * kChunkThreads < total)
+            if (f0 + u * NT < total)
                 Pl[sl[u]] = p[u];
     }
     __syncthreads();
-    for (unsigned row = t; row < rows; row += kChunkThreads) {
+    for (unsigned row = t; row < rows; row += NT) {
         float q = Q[row];
         const unsigned c = cntL[row];
         for (unsigned i = 0; i < c; i++)
@@ -659,7 +661,7 @@ void vi_chunk_rows(unsigned n, unsigned m,
         Q[row] = q;
     }
     __syncthreads();
-    for (unsigned x = t; x < states; x += kChunkThreads) {
+    for (unsigned x = t; x < states; x += NT) {
         const float *qs = Q + (size_t)x * m;
         float best = qs[0];
         unsigned bestA = 0;
@@ -1002,9 +1004,10 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
     L.n = n;
     L.m = m;
     L.ns = ns;
-    // tiles of 16,384 slots (NOVA_AMD_VI_TILE: 4,096 to 16,384), whole states each
+    // tiles of 8,192 slots (NOVA_AMD_VI_TILE: 4,096 to 16,384), whole states each: pass B then
+    // runs four 512-lane workgroups per CU rather than two of 1,024 lanes
     const unsigned tileSlots = std::min(kSortedEntries, std::max(std::max(4096u, m * ns),
-            env_unsigned("NOVA_AMD_VI_TILE", kSortedEntries)));
+            env_unsigned("NOVA_AMD_VI_TILE", kChunkTileDefault)));
     L.statesPerTile = tileSlots / (m * ns);
     L.tileEntries = L.statesPerTile * m * ns;
     L.tiles = (n + L.statesPerTile - 1) / L.statesPerTile;
@@ -1153,7 +1156,12 @@ static int chunked_build(const nova::MDP *mdp, const float *owner, hipStream_t s
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(float) * 32768));
     if (e == hipSuccess && rc == NOVA_SUCCESS)
-        e = hipFuncSetAttribute((const void *)vi_chunk_rows,
+        e = hipFuncSetAttribute((const void *)vi_chunk_rows<1024>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(uint2) * (kMaxChunks + 1) + kSortedEntries +
+                                      sizeof(float) * 2 * (size_t)kSortedEntries));
+    if (e == hipSuccess && rc == NOVA_SUCCESS)
+        e = hipFuncSetAttribute((const void *)vi_chunk_rows<512>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(sizeof(uint2) * (kMaxChunks + 1) + kSortedEntries +
                                       sizeof(float) * 2 * (size_t)kSortedEntries));
@@ -1221,9 +1229,14 @@ static void vi_launch(const ViPlan &p, const nova::MDP *mdp, const float *V, flo
                 hipLaunchKernelGGL(vi_chunk_products, dim3(p1 - p0), dim3(kChunkThreads),
                                    L.ldsA, st, mdp->n, L.chunkStates, mdp->gamma, p0, L.pieces,
                                    L.succ, L.T, V, L.P);
-            hipLaunchKernelGGL(vi_chunk_rows, dim3(t1 - t0), dim3(kChunkThreads), L.ldsB, st,
-                               mdp->n, mdp->m, L.statesPerTile, L.chunks, t0, L.tileSeg, L.P,
-                               L.slot, L.cnt, mdp->d_R, Vp, pi, (int)writePi);
+            if (L.tileEntries <= kSortedEntries / 2)   // 512 lanes of 16 slots each
+                hipLaunchKernelGGL(vi_chunk_rows<512>, dim3(t1 - t0), dim3(512), L.ldsB, st,
+                                   mdp->n, mdp->m, L.statesPerTile, L.chunks, t0, L.tileSeg,
+                                   L.P, L.slot, L.cnt, mdp->d_R, Vp, pi, (int)writePi);
+            else
+                hipLaunchKernelGGL(vi_chunk_rows<1024>, dim3(t1 - t0), dim3(1024), L.ldsB, st,
+                                   mdp->n, mdp->m, L.statesPerTile, L.chunks, t0, L.tileSeg,
+                                   L.P, L.slot, L.cnt, mdp->d_R, Vp, pi, (int)writePi);
         }
         return;
     }

@@ -561,7 +561,7 @@ DOMAINS = [("tiger_95", 4), ("4x3_95", 5), ("shuttle_95", 4), ("paint_95", 4),
 def bench_autograd(args):
     """The differentiable paths (no counterpart in the reference) on the workloads of their
     tests. Differentiable VI on vi_diff_n2e5 (n = 200,000, m = 8, ns = 16, gamma = 0.99): the
-    residual-stopped solve to 1e-6 (FP32 sweeps while the residual halves, then FP64), then the
+    residual-stopped solve to 1e-6 (FP32 sweeps while the residual falls, then FP64), then the
     adjoint (I - gamma P_pi)^T lambda = rho with dL/dR and dL/dT, each timed after a warm-up;
     CPU: the oracle's FP64 sweep on one pinned core over 10 sweeps of the same instance. PBVI
     argmax replay on pbvi_diff_n5000 (n = 5,000, r = 1,024, 30 iterations): the recorded

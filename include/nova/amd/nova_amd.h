@@ -41,7 +41,8 @@ extern "C" int mdp_vi_batch_execute_gpu(const MDP *const *mdps, MDPVIGPU *const 
 
 // Solves V* = max_a (R + gamma T.V*) on the device to an FP64 residual max|V' - V| <= tol.
 // Starts from vi->d_V (float; vi initialized or its d_* fields set by the caller) with float
-// sweeps while their residual still halves every 8 sweeps, then continues in FP64 until the
+// sweeps (on the sweep layout of the model's size, built for the call if the state has none)
+// while their residual still falls by 0.1% per 8 sweeps, then continues in FP64 until the
 // residual is <= tol or has stopped falling (the FP64 rounding floor). d_V64 is
 // caller device memory of 2n doubles; on return d_V64[0..n) holds V* and vi->d_pi the greedy
 // policy of the last sweep. flags bit 0: skip the float phase and start from d_V64[0..n)

@@ -1,9 +1,9 @@
 // nova-mi355x: value iteration to a residual threshold and its adjoint (no reference
 // counterpart: nova's VI runs a fixed horizon, mdp_vi_cpu.cpp:120-131, and has no gradients).
 //
-// Forward (mdp_vi_solve_gpu): float sweeps with the bit-exact kernel while the float residual
-// max|V' - V| can still fall, then FP64 sweeps (V in double, T and R promoted) until the FP64
-// residual is <= tol. FP32 alone cannot resolve residuals below ulp(|V|) (7.6e-6 at |V| = 78).
+// Forward (mdp_vi_solve_gpu): float sweeps with the bit-exact kernels (on the sweep layout of
+// the model's size) while the float residual max|V' - V| still falls, then FP64 sweeps (V in
+// double, T and R promoted) until the FP64 residual is <= tol. FP32 alone cannot resolve residuals below ulp(|V|) (7.6e-6 at |V| = 78).
 // The greedy policy of the last sweep is left in d_pi.
 //
 // Backward (mdp_vi_adjoint_gpu): at the converged greedy policy pi, L(V*) has
@@ -26,6 +26,9 @@ namespace nova_amd {
 
 int vi_sweeps_float(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned sweeps, hipStream_t st,
                     const char *fn);   // mdp_vi_gpu.hip
+int vi_layout_acquire(const nova::MDP *mdp, nova::MDPVIGPU *vi, hipStream_t st, const char *fn,
+                      bool *built);    // mdp_vi_gpu.hip
+void vi_layout_release(nova::MDPVIGPU *vi, const char *fn);   // mdp_vi_gpu.hip
 
 namespace {
 
@@ -273,9 +276,14 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
     const unsigned check = 8;
     const unsigned rgrid = std::min(1024u, (n + kB - 1) / kB);
     if (!(flags & 1u)) {
-        // float phase: continue while the float residual falls by at least 2x per check
+        // float phase, on the sweep layout of the model's size (built for this call when the
+        // state has none): continue while the float residual still falls (by 0.1% per check of
+        // 8 sweeps; a contraction of gamma = 0.99 alone gives 7.7%), that is until it reaches
+        // the rounding floor of float V
+        bool built = false;
+        rc = vi_layout_acquire(mdp, vi, st, fn, &built);
         double prev = 1e300;
-        while (done + check <= maxSweeps) {
+        while (rc == NOVA_SUCCESS && done + check <= maxSweeps) {
             rc = vi_sweeps_float(mdp, vi, check, st, fn);
             if (rc != NOVA_SUCCESS)
                 break;
@@ -284,10 +292,12 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
             hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, vi->d_V,
                                vi->d_VPrime, dmax);
             const double r = read_max(dmax, st);
-            if (r <= tol || r > 0.5 * prev)
+            if (r <= tol || r > 0.999 * prev)
                 break;
             prev = r;
         }
+        if (built)
+            vi_layout_release(vi, fn);
         const float *cur = (vi->currentHorizon % 2 == 0) ? vi->d_V : vi->d_VPrime;
         hipLaunchKernelGGL(to_double, dim3((n + kB - 1) / kB), dim3(kB), 0, st, n, cur, d_V64);
     }

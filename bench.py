@@ -261,10 +261,11 @@ def bench_c1(args):
 
 
 def bench_pbvi(args):
-    """PBVI at n=20,000, m=16, z=32, ns=8, r=4,096, rz=64, 50 iterations (fast mode)."""
+    """PBVI at n=20,000, m=16, z=32, ns=8, r=4,096, rz=64, 50 iterations (fast mode), and one
+    exact-mode iteration."""
     from nova_amd import synth
     from nova_amd.pomdp import POMDP
-    from nova_amd.pomdp_pbvi import FAST, POMDPPBVIGPU, set_mode
+    from nova_amd.pomdp_pbvi import EXACT, FAST, POMDPPBVIGPU, set_mode
     set_mode(FAST)
     P = synth.workload("pbvi_n20000")
     r, iters = P["Z"].shape[0], P["horizon"]
@@ -308,6 +309,19 @@ def bench_pbvi(args):
                                    "note": "multiply-adds the direct scoring kernel issues "
                                            "(2 * valid pairs * z * r), over its own duration"}}}
     del s
+    # exact mode (nova's rounding order, bit for bit): one iteration from the same Gamma after a
+    # warm-up iteration (which also tunes the exact-mode variants)
+    set_mode(EXACT)
+    s = POMDPPBVIGPU(p, GammaInitial=P["Gamma0"])
+    s.profile()
+    ex = s.profile()
+    out["exact"] = {"stage_ms": dict(zip(["transpose", "score", "action", "materialize"], ex)),
+                    "ms_per_iteration": float(sum(ex)), "backups_per_s": r / (sum(ex) * 1e-3),
+                    "scoring_variant": s.scoring_variant(),
+                    "note": "one iteration's four stages (HIP events); every product and sum "
+                            "rounded as nova's CPU backup, so Gamma' and pi equal its bit for bit"}
+    del s
+    set_mode(FAST)
     p.uninitialize_gpu()
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -561,7 +561,12 @@ void vi_chunk_rows(unsigned n, unsigned m,
     unsigned char *cntL = reinterpret_cast<unsigned char *>(lds + 2 * (chunks + 1));
     float *Q = lds + 2 * (chunks + 1) + (rowsPerTile + 3) / 4;   // rowsPerTile row sums
     float *Pl = Q + rowsPerTile;                     // the tile's products by slot
-    const unsigned tile = tileBase + blockIdx.x, t = threadIdx.x;
+    // workgroups are dealt to the 8 XCDs in turn; XCD x takes a contiguous run of this launch's
+    // tiles, so neighbouring tiles, which share the cache lines at their segments' ends, are
+    // read through the same L2
+    const unsigned nb = gridDim.x, xcd = blockIdx.x % 8, q8 = nb / 8, rem8 = nb % 8;
+    const unsigned tile = tileBase + xcd * q8 + min(xcd, rem8) + blockIdx.x / 8;
+    const unsigned t = threadIdx.x;
     const unsigned s0 = tile * statesPerTile;
     const unsigned states = min(statesPerTile, n - s0);
     const unsigned rows = states * m;

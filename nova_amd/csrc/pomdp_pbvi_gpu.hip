@@ -1251,14 +1251,29 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
             }
             const float *srcv = useLds ? row : Gj;
             const float *ot = P.OT + ((size_t)(a * z + o) * n) * ns;
+            const int *sa = P.SA + (size_t)a * n * ns;    // the action-major copy of S
             for (unsigned s = t; s < n; s += kMB) {
-                const size_t rw = ((size_t)s * m + a) * ns;
+                const size_t rw = (size_t)s * ns;
                 float vt = 0.0f;
-                for (unsigned l = 0; l < ns; l++) {
-                    const int sp = P.S[rw + l];
-                    if (sp < 0)
-                        break;
-                    vt = __fadd_rn(vt, __fmul_rn(ot[(size_t)s * ns + l], srcv[sp]));
+                // successors and factors four at a time (requested together), summed in order
+                // up to the first terminator
+                bool stop = false;
+                for (unsigned l0 = 0; l0 < ns && !stop; l0 += 4) {
+                    int spv[4];
+                    float otv[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const unsigned l = min(l0 + k, ns - 1);
+                        spv[k] = l0 + k < ns ? sa[rw + l] : -1;
+                        otv[k] = ot[rw + l];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (!stop && spv[k] < 0)
+                            stop = true;
+                        if (!stop)
+                            vt = __fadd_rn(vt, __fmul_rn(otv[k], srcv[spv[k]]));
+                    }
                 }
                 out[s] = __fadd_rn(out[s], __fmul_rn(vt, gamma));
             }

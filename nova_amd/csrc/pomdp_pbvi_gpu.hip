@@ -458,49 +458,73 @@ __global__ __launch_bounds__(kPB) void pbvi_score(const PbviInst *__restrict__ i
         __syncthreads();
 
         float vtk[EXACT ? kZC : 1][JPT];
-
-        // exact: per support entry k, its pairs (rounded as the reference); fast: all pairs
-        const int kLoops = EXACT ? kEnd : (uEnd > 0 ? 1 : 0);
-        for (int k = 0; k < kLoops; k++) {
-            if (EXACT) {
 #pragma unroll
-                for (int oo = 0; oo < kZC; oo++)
+        for (int oo = 0; oo < (EXACT ? kZC : 1); oo++)
 #pragma unroll
-                    for (int q = 0; q < JPT; q++)
-                        vtk[oo % (EXACT ? kZC : 1)][q] = 0.0f;
-            }
-            const int u1 = EXACT ? L.startK[k + 1] : uEnd;
-            for (int u = EXACT ? L.startK[k] : 0; u < u1; u++) {
+            for (int q = 0; q < JPT; q++)
+                vtk[oo][q] = 0.0f;
+        // exact: closes support entry k (its pairs summed in the reference's order) into the
+        // scores and starts the next one. Entries close right after their last pair, and an
+        // entry without pairs right after the entry before it, so the scores see the same
+        // operations in the same order as an entry-by-entry loop
+        auto closeEntry = [&](int k) {
+            const float bk = L.bK[k];
+#pragma unroll
+            for (int oo = 0; oo < kZC; oo++)
+#pragma unroll
+                for (int q = 0; q < JPT; q++) {
+                    const float v = __fmul_rn(vtk[oo % (EXACT ? kZC : 1)][q], gamma);
+                    val[oo][q] = __fadd_rn(val[oo][q], __fmul_rn(v, bk));
+                    vtk[oo % (EXACT ? kZC : 1)][q] = 0.0f;
+                }
+        };
+        // the pairs of all entries in order; the Gamma^T values of pair u + kPre are requested
+        // while pair u is summed (a ring of kPre register sets)
+        constexpr int kPre = 4;
+        int k = 0;
+        if (EXACT)
+            while (k < kEnd && L.startK[k + 1] == 0)
+                closeEntry(k++);
+        if (uEnd > 0) {
+            float gr[kPre][JPT];
+            auto loadG = [&](int u, float (&dst)[JPT]) {
                 const size_t rowOff = (size_t)L.spU[u] * P.rS;
-                float g[JPT];
 #pragma unroll
                 for (int q = 0; q < JPT; q++)
-                    g[q] = live[q] ? GT[rowOff + jq[q]] : 0.0f;
-                const float *wu = w + u * kZC;
+                    dst[q] = live[q] ? GT[rowOff + jq[q]] : 0.0f;
+            };
 #pragma unroll
-                for (int oo = 0; oo < kZC; oo++) {
-                    const float f = wu[oo];
+            for (int d = 0; d < kPre; d++)
+                loadG(min(d, uEnd - 1), gr[d]);
+            for (int u0 = 0; u0 < uEnd; u0 += kPre) {
 #pragma unroll
-                    for (int q = 0; q < JPT; q++) {
+                for (int d = 0; d < kPre; d++) {
+                    const int u = u0 + d;
+                    if (u < uEnd) {
+                        const float *wu = w + u * kZC;
+#pragma unroll
+                        for (int oo = 0; oo < kZC; oo++) {
+                            const float f = wu[oo];
+#pragma unroll
+                            for (int q = 0; q < JPT; q++) {
+                                if (EXACT)
+                                    vtk[oo % (EXACT ? kZC : 1)][q] = __fadd_rn(
+                                            vtk[oo % (EXACT ? kZC : 1)][q], __fmul_rn(f, gr[d][q]));
+                                else
+                                    val[oo][q] = __builtin_fmaf(f, gr[d][q], val[oo][q]);
+                            }
+                        }
                         if (EXACT)
-                            vtk[oo % (EXACT ? kZC : 1)][q] =
-                                __fadd_rn(vtk[oo % (EXACT ? kZC : 1)][q], __fmul_rn(f, g[q]));
-                        else
-                            val[oo][q] = __builtin_fmaf(f, g[q], val[oo][q]);
+                            while (k < kEnd && L.startK[k + 1] == u + 1)
+                                closeEntry(k++);
                     }
+                    loadG(min(u + kPre, uEnd - 1), gr[d]);
                 }
             }
-            if (EXACT) {
-                const float bk = L.bK[k];
-#pragma unroll
-                for (int oo = 0; oo < kZC; oo++)
-#pragma unroll
-                    for (int q = 0; q < JPT; q++) {
-                        const float v = __fmul_rn(vtk[oo % (EXACT ? kZC : 1)][q], gamma);
-                        val[oo][q] = __fadd_rn(val[oo][q], __fmul_rn(v, bk));
-                    }
-            }
         }
+        if (EXACT)
+            while (k < kEnd)                  // entries without pairs after the last pair
+                closeEntry(k++);
         }   // support chunks
 
         // per lane: first maximiser over its alphas (visited in increasing j)

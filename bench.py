@@ -575,9 +575,11 @@ DOMAINS = [("tiger_95", 4), ("4x3_95", 5), ("shuttle_95", 4), ("paint_95", 4),
 def bench_autograd(args):
     """The differentiable paths (no counterpart in the reference) on the workloads of their
     tests. Differentiable VI on vi_diff_n2e5 (n = 200,000, m = 8, ns = 16, gamma = 0.99): the
-    residual-stopped solve to 1e-6 (FP32 sweeps while the residual falls, then FP64), then the
-    adjoint (I - gamma P_pi)^T lambda = rho with dL/dR and dL/dT, each timed after a warm-up;
-    CPU: the oracle's FP64 sweep on one pinned core over 10 sweeps of the same instance. PBVI
+    residual-stopped solve to 1e-6 (FP32 sweeps while the residual falls, then FP64, the
+    iterate moved along the slow error mode between checks), the same solve with plain sweeps
+    for comparison, then the adjoint (I - gamma P_pi)^T lambda = rho with dL/dR and dL/dT, each
+    timed after a warm-up; CPU: the oracle's FP64 sweep on one pinned core over 10 sweeps of
+    the same instance, and that rate over the GPU solve's sweep count. PBVI
     argmax replay on pbvi_diff_n5000 (n = 5,000, r = 1,024, 30 iterations): the recorded
     forward run and the backward replay of dL/dR."""
     import torch
@@ -599,12 +601,16 @@ def bench_autograd(args):
         return res, time.perf_counter() - t0
 
     (V64, pi, sweeps, res), ts = timed(lambda: solve(R, T, S, g))
+    (V64p, _, sweepsp, resp), tp = timed(lambda: solve(R, T, S, g, plain=True))
     rho = torch.full((n,), 1.0 / n, device=dev)
     (lam, dR, dT, it, r64), ta = timed(lambda: adjoint(R, T, S, g, pi, V64, rho))
     entry = {"workload": "vi_diff_n2e5 (n=200000, m=8, ns=16, gamma=0.99)",
              "solve_ms": ts * 1e3, "solve_sweeps": int(sweeps), "solve_residual": float(res),
              "solve_state_updates_per_s": sweeps * n / ts,
              "solve_algorithmic_GBs": vi_bytes_per_sweep(n, m, ns) * sweeps / ts / 1e9,
+             "plain_solve_ms": tp * 1e3, "plain_solve_sweeps": int(sweepsp),
+             "plain_solve_residual": float(resp),
+             "plain_vs_moved_max_abs_diff": float((V64p - V64).abs().max()),
              "adjoint_ms": ta * 1e3, "adjoint_iterations": int(it),
              "adjoint_fp64_residual": float(r64)}
     if not args.no_cpu:
@@ -623,6 +629,7 @@ def bench_autograd(args):
             os.sched_setaffinity(0, old)
         entry["cpu_baseline"] = baseline(10 * n / tc, "state-updates/s (FP64 sweeps)", "port",
                                          "10 FP64 sweeps of the same instance")
+        entry["cpu_ms_for_the_gpu_solve_sweeps"] = tc / 10 * sweeps * 1e3
     out["diff_vi"] = entry
     print("autograd VI leg done", file=sys.stderr, flush=True)
     P = synth.workload("pbvi_diff_n5000")

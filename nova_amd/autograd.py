@@ -26,9 +26,11 @@ def _mdp_view(R, T, S, gamma):
     return mdp
 
 
-def solve(R, T, S, gamma, tol=1e-6, max_sweeps=100000, V0=None, warm64=None):
+def solve(R, T, S, gamma, tol=1e-6, max_sweeps=100000, V0=None, warm64=None, plain=False):
     """(V64, pi, sweeps, residual): the FP64 fixed point on the device and its greedy policy.
-    warm64: optional (n,) float64 CUDA tensor to start the FP64 phase from."""
+    warm64: optional (n,) float64 CUDA tensor to start the FP64 phase from. plain: sweeps
+    only, without the shifts along the constant vector that remove the slow part of the
+    error (same fixed point, about 1/(1 - gamma) times more sweeps)."""
     lib = _abi.load()
     n = S.shape[0]
     dev = R.device
@@ -41,10 +43,10 @@ def solve(R, T, S, gamma, tol=1e-6, max_sweeps=100000, V0=None, warm64=None):
     vi.d_VPrime = ct.cast(Vp.data_ptr(), _abi.c_float_p)
     vi.d_pi = ct.cast(pi.data_ptr(), _abi.c_uint_p)
     V64 = torch.empty(2 * n, dtype=torch.float64, device=dev)
-    flags = 0
+    flags = 2 if plain else 0
     if warm64 is not None:
         V64[:n].copy_(warm64)
-        flags = 1
+        flags |= 1
     torch.cuda.synchronize()
     sweeps = ct.c_uint(0)
     res = ct.c_double(0)

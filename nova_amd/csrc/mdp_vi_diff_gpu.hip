@@ -1,10 +1,14 @@
 // nova-mi355x: value iteration to a residual threshold and its adjoint (no reference
 // counterpart: nova's VI runs a fixed horizon, mdp_vi_cpu.cpp:120-131, and has no gradients).
 //
-// Forward (mdp_vi_solve_gpu): float sweeps with the bit-exact kernels (on the sweep layout of
-// the model's size) while the float residual max|V' - V| still falls, then FP64 sweeps (V in
-// double, T and R promoted) until the FP64 residual is <= tol. FP32 alone cannot resolve residuals below ulp(|V|) (7.6e-6 at |V| = 78).
-// The greedy policy of the last sweep is left in d_pi.
+// Forward (mdp_vi_solve_gpu): float sweeps with the bit-exact kernels (on the state's sweep
+// layout if it has one) while the float residual max|V' - V| still falls, then FP64 sweeps (V
+// in double, T and R promoted) until the FP64 residual is <= tol. FP32 alone cannot resolve
+// residuals below ulp(|V|) (7.6e-6 at |V| = 78). Between blocks of 8 sweeps the iterate is
+// moved along the slow mode of the error when the last sweep's step shows it (SolveChecks):
+// at gamma = 0.99 plain sweeps shrink that mode by 1% each, about 1,400 sweeps to 1e-6, and
+// the moves leave the few dozen sweeps the faster modes need. The greedy policy of the last
+// sweep is left in d_pi.
 //
 // Backward (mdp_vi_adjoint_gpu): at the converged greedy policy pi, L(V*) has
 //   dL/dR[s,a]   = lambda(s) [a = pi(s)]
@@ -12,7 +16,9 @@
 // with (I - gamma P_pi)^T lambda = rho, rho = dL/dV*. lambda is found by the fixed-point
 // iteration lambda <- rho + gamma P_pi^T lambda in FP32 over a transposed successor list (the
 // pull form, built once per call with a stable radix sort, so sums run in a fixed order and
-// results are reproducible); the final residual is evaluated in FP64.
+// results are reproducible), with the slow mode's remaining steps added once the last two
+// steps isolate it; the final residual is evaluated in FP64.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -26,9 +32,6 @@ namespace nova_amd {
 
 int vi_sweeps_float(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned sweeps, hipStream_t st,
                     const char *fn);   // mdp_vi_gpu.hip
-int vi_layout_acquire(const nova::MDP *mdp, nova::MDPVIGPU *vi, hipStream_t st, const char *fn,
-                      bool *built);    // mdp_vi_gpu.hip
-void vi_layout_release(nova::MDPVIGPU *vi, const char *fn);   // mdp_vi_gpu.hip
 
 namespace {
 
@@ -98,6 +101,52 @@ __global__ __launch_bounds__(kB) void max_abs_diff(unsigned n, const VT *__restr
         atomicMax(out, (unsigned long long)__double_as_longlong(mx));
 }
 
+// Order-preserving key of a double for atomicMax (negative values below positive ones), and
+// its inverse.
+__device__ __forceinline__ unsigned long long order_key(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+double from_order_key(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    return v;
+}
+
+// The extremes of the signed sweep difference d = cur - prev: out[0] = key(max d),
+// out[1] = key(max -d), both zeroed by the caller.
+template <typename VT>
+__global__ __launch_bounds__(kB) void diff_extremes(unsigned n, const VT *__restrict__ cur,
+        const VT *__restrict__ prev, unsigned long long *__restrict__ out)
+{
+    double hi = -INFINITY, lo = -INFINITY;
+    for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB) {
+        const double d = (double)cur[s] - (double)prev[s];
+        hi = fmax(hi, d);
+        lo = fmax(lo, -d);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        hi = fmax(hi, __shfl_xor(hi, off));
+        lo = fmax(lo, __shfl_xor(lo, off));
+    }
+    if (threadIdx.x % 64 == 0 && hi > -INFINITY) {
+        atomicMax(&out[0], order_key(hi));
+        atomicMax(&out[1], order_key(lo));
+    }
+}
+
+template <typename VT>
+__global__ void add_const(unsigned n, VT *__restrict__ v, double c)
+{
+    const unsigned s = blockIdx.x * kB + threadIdx.x;
+    if (s < n)
+        v[s] = (VT)((double)v[s] + c);
+}
+
 __global__ void to_double(unsigned n, const float *__restrict__ a, double *__restrict__ b)
 {
     const unsigned s = blockIdx.x * kB + threadIdx.x;
@@ -162,6 +211,87 @@ __global__ void adj_step(unsigned n, float gamma, const unsigned *__restrict__ s
     out[sp] = rho[sp] + gamma * acc;
 }
 
+// Sums of the last two adjoint steps, d1 = lam_k - lam_{k-1} and d2 = lam_{k+1} - lam_k, in a
+// fixed order (per-workgroup partials in part[2 * block + {0, 1}], then adj_sum_final), so
+// that the extrapolation below, and lambda with it, is reproducible.
+constexpr int kSumBlocks = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *lds)
+{
+    for (int off = 32; off >= 1; off >>= 1)
+        v += __shfl_xor(v, off);
+    const unsigned w = threadIdx.x / 64;
+    __syncthreads();
+    if (threadIdx.x % 64 == 0)
+        lds[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (unsigned i = 0; i < kB / 64; i++)
+        t += lds[i];
+    return t;
+}
+
+__global__ __launch_bounds__(kB) void adj_sum_partials(unsigned n,
+        const float *__restrict__ l0, const float *__restrict__ l1, const float *__restrict__ l2,
+        double *__restrict__ part)
+{
+    __shared__ double lds[kB / 64];
+    double s1 = 0.0, s2 = 0.0;
+    for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB) {
+        s1 += (double)l1[s] - (double)l0[s];
+        s2 += (double)l2[s] - (double)l1[s];
+    }
+    s1 = block_sum(s1, lds);
+    s2 = block_sum(s2, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1;
+        part[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+__global__ __launch_bounds__(kB) void adj_sum_final(unsigned blocks,
+        const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double lds[kB / 64];
+    double s1 = 0.0, s2 = 0.0;
+    for (unsigned b = threadIdx.x; b < blocks; b += kB) {
+        s1 += part[2 * b];
+        s2 += part[2 * b + 1];
+    }
+    s1 = block_sum(s1, lds);
+    s2 = block_sum(s2, lds);
+    if (threadIdx.x == 0) {
+        out[0] = s1;
+        out[1] = s2;
+    }
+}
+
+// max |d2 - q d1| over the states (the part of the last step outside the ratio-q mode)
+__global__ __launch_bounds__(kB) void adj_mode_dev(unsigned n, const float *__restrict__ l0,
+        const float *__restrict__ l1, const float *__restrict__ l2, double q,
+        unsigned long long *__restrict__ out)
+{
+    double mx = 0.0;
+    for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB) {
+        const double d1 = (double)l1[s] - (double)l0[s], d2 = (double)l2[s] - (double)l1[s];
+        mx = fmax(mx, fabs(d2 - q * d1));
+    }
+    for (int off = 32; off >= 1; off >>= 1)
+        mx = fmax(mx, __shfl_xor(mx, off));
+    if (threadIdx.x % 64 == 0)
+        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
+// out = l2 + (l2 - l1) f: the sum of the remaining steps of a mode that shrinks by q per step,
+// f = q / (1 - q)
+__global__ void adj_extrapolate(unsigned n, const float *__restrict__ l1,
+        const float *__restrict__ l2, double f, float *__restrict__ out)
+{
+    const unsigned s = blockIdx.x * kB + threadIdx.x;
+    if (s < n)
+        out[s] = (float)((double)l2[s] + ((double)l2[s] - (double)l1[s]) * f);
+}
+
 // FP64 residual of the adjoint equation, per state (max taken by max_abs_diff-like reduction)
 __global__ void adj_resid64(unsigned n, double gamma, const unsigned *__restrict__ start,
         const unsigned *__restrict__ src, const float *__restrict__ tv,
@@ -212,6 +342,158 @@ double read_max(unsigned long long *d, hipStream_t st)
     std::memcpy(&v, &h, sizeof(v));
     return v;
 }
+
+// Sweep-difference kernels of the solve's checks. d = cur - prv is the last sweep's step;
+// dprev (doubles) holds the step of the check before.
+template <typename VT>
+__global__ void diff_store(unsigned n, const VT *__restrict__ cur, const VT *__restrict__ prv,
+        double *__restrict__ dprev)
+{
+    const unsigned s = blockIdx.x * kB + threadIdx.x;
+    if (s < n)
+        dprev[s] = (double)cur[s] - (double)prv[s];
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kB) void ratio_sum_partials(unsigned n, const VT *__restrict__ cur,
+        const VT *__restrict__ prv, const double *__restrict__ dprev, double *__restrict__ part)
+{
+    __shared__ double lds[kB / 64];
+    double s1 = 0.0, s2 = 0.0;
+    for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB) {
+        s1 += dprev[s];
+        s2 += (double)cur[s] - (double)prv[s];
+    }
+    s1 = block_sum(s1, lds);
+    s2 = block_sum(s2, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1;
+        part[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kB) void ratio_dev(unsigned n, const VT *__restrict__ cur,
+        const VT *__restrict__ prv, const double *__restrict__ dprev, double q,
+        unsigned long long *__restrict__ out)
+{
+    double mx = 0.0;
+    for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB)
+        mx = fmax(mx, fabs(((double)cur[s] - (double)prv[s]) - q * dprev[s]));
+    for (int off = 32; off >= 1; off >>= 1)
+        mx = fmax(mx, __shfl_xor(mx, off));
+    if (threadIdx.x % 64 == 0)
+        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
+template <typename VT>
+__global__ void ratio_extrapolate(unsigned n, VT *__restrict__ cur, const VT *__restrict__ prv,
+        double f)
+{
+    const unsigned s = blockIdx.x * kB + threadIdx.x;
+    if (s < n)
+        cur[s] = (VT)((double)cur[s] + ((double)cur[s] - (double)prv[s]) * f);
+}
+
+// The checks of the residual-stopped solve and what they may do to the iterate between blocks
+// of sweeps. measure() gives the residual max|d| of the block's last sweep, d = V_k - V_{k-1}
+// = T V_{k-1} - V_{k-1}; accelerate() then moves V_k towards the fixed point when the step
+// shows which way:
+// * d of one sign and nearly uniform (its spread at most a quarter of its smallest magnitude):
+//   with every row of T summing to 1, T(V + c) = TV + gamma c, so V* lies within
+//   [V_k + g dmin, V_k + g dmax], g = gamma / (1 - gamma) (MacQueen's bounds), and V_k moves
+//   to the middle, which bounds its error by g (dmax - dmin) / 2, never more than V_k's own
+//   bound g max(|dmin|, |dmax|). The error then lay along the constant vector, the mode that
+//   sweeps shrink by gamma alone; the other modes had already died out.
+// * otherwise (rows summing below 1, as when successor lists are cut short), once the policy
+//   has settled the sweeps are linear and the slowest mode of gamma P_pi is left: d is then
+//   q8 = sum(d) / sum(dprev) times the step of the check before, 8 sweeps earlier, up to a
+//   tenth of max|d|, and the remaining steps of that mode sum to d q / (1 - q), q = q8^(1/8).
+// Either move is followed by at least one more check of plain sweeps, and the solve stops on
+// the residual alone, so a move can cost sweeps but never accuracy. The sums run in a fixed
+// order, so the solve is reproducible.
+struct SolveChecks {
+    unsigned n = 0;
+    double gamma = 0.0;
+    unsigned long long *dext = nullptr;   // [max d key, max -d key, scratch max]
+    double *dprev = nullptr;              // n doubles
+    double *part = nullptr;               // 2 kSumBlocks partials + 2 sums
+    bool havePrev = false;
+    double dmin = 0.0, dmax = 0.0;
+
+    int alloc(const char *fn)
+    {
+        int rc = device_alloc((void **)&dext, 3 * sizeof(unsigned long long), fn, "a residual");
+        if (rc == NOVA_SUCCESS)
+            rc = device_alloc((void **)&dprev, sizeof(double) * n, fn, "the solve's checks");
+        if (rc == NOVA_SUCCESS)
+            rc = device_alloc((void **)&part, sizeof(double) * (2 * kSumBlocks + 2), fn,
+                              "the solve's checks");
+        return rc;
+    }
+    void release(const char *fn)
+    {
+        void *bufs[] = {dext, dprev, part};
+        for (void *p : bufs)
+            if (p)
+                device_free(p, fn, "the solve's checks");
+    }
+    unsigned grid() const { return std::min(1024u, (n + kB - 1) / kB); }
+
+    template <typename VT>
+    double measure(const VT *cur, const VT *prv, hipStream_t st)
+    {
+        hipMemsetAsync(dext, 0, 2 * sizeof(unsigned long long), st);
+        hipLaunchKernelGGL(diff_extremes<VT>, dim3(grid()), dim3(kB), 0, st, n, cur, prv, dext);
+        unsigned long long h[2] = {0, 0};
+        hipMemcpyAsync(h, dext, sizeof(h), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        dmax = from_order_key(h[0]);
+        dmin = -from_order_key(h[1]);
+        return std::max(std::fabs(dmin), std::fabs(dmax));
+    }
+
+    template <typename VT>
+    void accelerate(VT *cur, const VT *prv, hipStream_t st)
+    {
+        const unsigned eg = (n + kB - 1) / kB;
+        const double res = std::max(std::fabs(dmin), std::fabs(dmax));
+        const bool oneSign = (dmin > 0.0 && dmax > 0.0) || (dmin < 0.0 && dmax < 0.0);
+        if (oneSign && gamma < 1.0 &&
+                dmax - dmin <= 0.25 * std::min(std::fabs(dmin), std::fabs(dmax))) {
+            hipLaunchKernelGGL(add_const<VT>, dim3(eg), dim3(kB), 0, st, n, cur,
+                               gamma / (1.0 - gamma) * 0.5 * (dmin + dmax));
+            havePrev = false;
+            return;
+        }
+        if (havePrev) {
+            const unsigned sb = std::min((unsigned)kSumBlocks, eg);
+            hipLaunchKernelGGL(ratio_sum_partials<VT>, dim3(sb), dim3(kB), 0, st, n,
+                               (const VT *)cur, prv, (const double *)dprev, part);
+            hipLaunchKernelGGL(adj_sum_final, dim3(1), dim3(kB), 0, st, sb, part,
+                               part + 2 * kSumBlocks);
+            double sums[2] = {0.0, 0.0};
+            hipMemcpyAsync(sums, part + 2 * kSumBlocks, sizeof(sums), hipMemcpyDeviceToHost, st);
+            hipStreamSynchronize(st);
+            const double q8 = (sums[0] != 0.0) ? sums[1] / sums[0] : 0.0;
+            if (q8 > 0.0 && q8 < 1.0) {
+                hipMemsetAsync(dext + 2, 0, sizeof(unsigned long long), st);
+                hipLaunchKernelGGL(ratio_dev<VT>, dim3(grid()), dim3(kB), 0, st, n,
+                                   (const VT *)cur, prv, (const double *)dprev, q8, dext + 2);
+                if (read_max(dext + 2, st) <= 0.1 * res) {
+                    const double q = std::pow(q8, 1.0 / 8.0);
+                    hipLaunchKernelGGL(ratio_extrapolate<VT>, dim3(eg), dim3(kB), 0, st, n, cur,
+                                       prv, q / (1.0 - q));
+                    havePrev = false;
+                    return;
+                }
+            }
+        }
+        hipLaunchKernelGGL(diff_store<VT>, dim3(eg), dim3(kB), 0, st, n, (const VT *)cur, prv,
+                           dprev);
+        havePrev = true;
+    }
+};
 
 } // namespace
 
@@ -267,54 +549,55 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
     }
     const unsigned n = mdp->n;
     hipStream_t st = stream();
-    unsigned long long *dmax = nullptr;
-    int rc = device_alloc((void **)&dmax, sizeof(unsigned long long), fn, "a residual");
-    if (rc != NOVA_SUCCESS)
+    SolveChecks chk;
+    chk.n = n;
+    chk.gamma = (double)mdp->gamma;
+    int rc = chk.alloc(fn);
+    if (rc != NOVA_SUCCESS) {
+        chk.release(fn);
         return rc;
+    }
     unsigned done = 0;
     double res = 1e300;
     const unsigned check = 8;
-    const unsigned rgrid = std::min(1024u, (n + kB - 1) / kB);
+    const unsigned eg = (n + kB - 1) / kB;
+    const bool shifts = !(flags & 2u);     // bit 1: plain sweeps, no moves between checks
     if (!(flags & 1u)) {
-        // float phase, on the sweep layout of the model's size (built for this call when the
-        // state has none): continue while the float residual still falls (by 0.1% per check of
-        // 8 sweeps; a contraction of gamma = 0.99 alone gives 7.7%), that is until it reaches
-        // the rounding floor of float V
-        bool built = false;
-        rc = vi_layout_acquire(mdp, vi, st, fn, &built);
+        // float phase, on the state's sweep layout if it has one (a few checks at most, too few
+        // to pay for building one): continue while the float residual still falls (by 0.1% per
+        // check of 8 sweeps), that is until it reaches the rounding floor of float V
         double prev = 1e300;
         while (rc == NOVA_SUCCESS && done + check <= maxSweeps) {
             rc = vi_sweeps_float(mdp, vi, check, st, fn);
             if (rc != NOVA_SUCCESS)
                 break;
             done += check;
-            hipMemsetAsync(dmax, 0, sizeof(*dmax), st);
-            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, vi->d_V,
-                               vi->d_VPrime, dmax);
-            const double r = read_max(dmax, st);
+            const bool even = vi->currentHorizon % 2 == 0;
+            float *cur = even ? vi->d_V : vi->d_VPrime;
+            const float *prv = even ? vi->d_VPrime : vi->d_V;
+            const double r = chk.measure<float>(cur, prv, st);
             if (r <= tol || r > 0.999 * prev)
                 break;
             prev = r;
+            if (shifts)
+                chk.accelerate<float>(cur, prv, st);
         }
-        if (built)
-            vi_layout_release(vi, fn);
         const float *cur = (vi->currentHorizon % 2 == 0) ? vi->d_V : vi->d_VPrime;
-        hipLaunchKernelGGL(to_double, dim3((n + kB - 1) / kB), dim3(kB), 0, st, n, cur, d_V64);
+        hipLaunchKernelGGL(to_double, dim3(eg), dim3(kB), 0, st, n, cur, d_V64);
     }
     // FP64 phase: d_V64 holds V; the second half is the ping-pong partner. Stops at the
     // tolerance, or when the residual has not fallen for 4 checks (FP64 rounding floor).
     double *A = d_V64, *Bv = d_V64 + n;
     double bestRes = 1e300;
     int stall = 0;
+    chk.havePrev = false;
     while (rc == NOVA_SUCCESS && done < maxSweeps) {
         unsigned k = 0;
         for (; k < check && done < maxSweeps; k++, done++) {
             sweep64(mdp, A, Bv, vi->d_pi, st);
             std::swap(A, Bv);
         }
-        hipMemsetAsync(dmax, 0, sizeof(*dmax), st);
-        hipLaunchKernelGGL(max_abs_diff<double>, dim3(rgrid), dim3(kB), 0, st, n, A, Bv, dmax);
-        res = read_max(dmax, st);
+        res = chk.measure<double>(A, Bv, st);
         if (res <= tol)
             break;
         if (res < 0.999 * bestRes) {
@@ -323,11 +606,13 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
         } else if (++stall >= 4) {
             break;
         }
+        if (shifts && done < maxSweeps)
+            chk.accelerate<double>(A, Bv, st);
     }
     if (rc == NOVA_SUCCESS && A != d_V64)
         hipMemcpyAsync(d_V64, A, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
     hipError_t e = hipStreamSynchronize(st);
-    device_free(dmax, fn, "a residual");
+    chk.release(fn);
     if (rc == NOVA_SUCCESS && e != hipSuccess)
         rc = hip_fail(e, fn, "Failed to run the sweeps.", NOVA_ERROR_KERNEL_EXECUTION);
     if (sweepsDone)
@@ -356,7 +641,8 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
     // transposed successor list of the policy's rows (stable sort: fixed order per bucket)
     unsigned *keys = nullptr, *keys2 = nullptr, *slot = nullptr, *slot2 = nullptr;
     unsigned *start = nullptr, *src = nullptr;
-    float *tv = nullptr, *lamB = nullptr;
+    float *tv = nullptr, *lamB = nullptr, *lamC = nullptr;
+    double *part = nullptr;
     unsigned long long *dmax = nullptr;
     void *temp = nullptr;
     size_t tempBytes = 0;
@@ -368,6 +654,10 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&src, E * 4, fn, "adjoint buffers");
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&tv, E * 4, fn, "adjoint buffers");
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&lamB, n * 4ull, fn, "adjoint buffers");
+    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&lamC, n * 4ull, fn, "adjoint buffers");
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc((void **)&part, sizeof(double) * (2 * kSumBlocks + 2), fn,
+                          "adjoint buffers");
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&dmax, 8, fn, "adjoint buffers");
     if (rc == NOVA_SUCCESS) {
         const unsigned gE = (unsigned)((E + kB - 1) / kB);
@@ -390,26 +680,59 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
         const unsigned g = (n + kB - 1) / kB;
         const unsigned rgrid = std::min(1024u, g);
         hipMemcpyAsync(d_lambda, d_rho, n * 4ull, hipMemcpyDeviceToDevice, st);
-        float *a = d_lambda, *b = lamB;
+        // cur = lambda_{k+1}, prv = lambda_k, old = lambda_{k-1}
+        float *cur = d_lambda, *prv = lamB, *old = lamC;
         const unsigned check = 16;
+        const unsigned sb = std::min((unsigned)kSumBlocks, g);
         while (it < maxIters) {
-            for (unsigned k = 0; k < check && it < maxIters; k++, it++) {
+            unsigned k = 0;
+            for (; k < check && it < maxIters; k++, it++) {
                 hipLaunchKernelGGL(adj_step, dim3(g), dim3(kB), 0, st, n, mdp->gamma, start, src,
-                                   tv, d_rho, a, b);
-                std::swap(a, b);
+                                   tv, d_rho, cur, old);
+                float *t3 = old;
+                old = prv;
+                prv = cur;
+                cur = t3;
             }
             hipMemsetAsync(dmax, 0, 8, st);
-            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, a, b, dmax);
+            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, cur, prv,
+                               dmax);
             const double d = read_max(dmax, st);
             hipMemsetAsync(dmax, 0, 8, st);
-            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, a,
+            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, cur,
                                (const float *)d_rho, dmax);
             const double scale = read_max(dmax, st);   // max|lambda - rho|, a scale of lambda
-            if (d <= (double)tol * std::max(1e-30, scale))
+            if (d <= (double)tol * std::max(1e-30, scale) || it >= maxIters || k < 2)
                 break;
+            // The error of the iteration lambda <- rho + gamma P^T lambda is a sum of modes that
+            // shrink by gamma times the eigenvalues of P^T per step; with P's rows summing to 1
+            // the slowest is the stationary distribution's, at gamma, and the others die out
+            // much faster. Once the last step d2 is q times the step before it up to a tenth
+            // (q from the sums of the two steps), what is left of the error is that one mode,
+            // and its remaining steps sum to d2 q / (1 - q): add them.
+            hipLaunchKernelGGL(adj_sum_partials, dim3(sb), dim3(kB), 0, st, n, old, prv, cur,
+                               part);
+            hipLaunchKernelGGL(adj_sum_final, dim3(1), dim3(kB), 0, st, sb, part,
+                               part + 2 * kSumBlocks);
+            double sums[2] = {0.0, 0.0};
+            hipMemcpyAsync(sums, part + 2 * kSumBlocks, sizeof(sums), hipMemcpyDeviceToHost, st);
+            hipStreamSynchronize(st);
+            if (!(sums[0] != 0.0))
+                continue;
+            const double q = sums[1] / sums[0];
+            if (!(q > 0.0 && q < 1.0))
+                continue;
+            hipMemsetAsync(dmax, 0, 8, st);
+            hipLaunchKernelGGL(adj_mode_dev, dim3(rgrid), dim3(kB), 0, st, n, old, prv, cur, q,
+                               dmax);
+            if (read_max(dmax, st) <= 0.1 * d) {
+                hipLaunchKernelGGL(adj_extrapolate, dim3(g), dim3(kB), 0, st, n, prv, cur,
+                                   q / (1.0 - q), old);
+                std::swap(cur, old);
+            }
         }
-        if (a != d_lambda)
-            hipMemcpyAsync(d_lambda, a, n * 4ull, hipMemcpyDeviceToDevice, st);
+        if (cur != d_lambda)
+            hipMemcpyAsync(d_lambda, cur, n * 4ull, hipMemcpyDeviceToDevice, st);
         hipMemsetAsync(dmax, 0, 8, st);
         hipLaunchKernelGGL(adj_resid64, dim3(rgrid), dim3(kB), 0, st, n, (double)mdp->gamma,
                            start, src, tv, d_rho, d_lambda, dmax);
@@ -422,7 +745,7 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
         if (e != hipSuccess)
             rc = hip_fail(e, fn, "Failed to run the adjoint.", NOVA_ERROR_KERNEL_EXECUTION);
     }
-    void *bufs[] = {keys, keys2, slot, slot2, start, src, tv, lamB, dmax, temp};
+    void *bufs[] = {keys, keys2, slot, slot2, start, src, tv, lamB, lamC, part, dmax, temp};
     for (void *p : bufs)
         if (p)
             device_free(p, fn, "adjoint buffers");

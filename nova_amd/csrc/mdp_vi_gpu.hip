@@ -1349,32 +1349,6 @@ int vi_sweeps_float(const nova::MDP *mdp, nova::MDPVIGPU *vi, unsigned sweeps, h
     return vi_run_sweeps(mdp, vi, sweeps, st, nullptr, nullptr, fn);
 }
 
-// For a caller that runs many float sweeps on a state it did not initialize (the residual-
-// stopped solve): builds the sweep layout the model's size calls for, keyed by vi->d_V, unless
-// the state already has one. *built says whether this call built it; the caller then releases
-// it with vi_layout_release.
-int vi_layout_acquire(const nova::MDP *mdp, nova::MDPVIGPU *vi, hipStream_t st, const char *fn,
-                      bool *built)
-{
-    *built = false;
-    if (chunked_for(mdp, vi) != nullptr || sorted_for(mdp, vi) != nullptr)
-        return NOVA_SUCCESS;
-    const int want = layout_wanted(mdp);
-    int rc = NOVA_SUCCESS;
-    if (want == 3)
-        rc = chunked_build(mdp, vi->d_V, st, fn);
-    else if (want == 2)
-        rc = sorted_build(mdp, vi->d_V, st, fn);
-    *built = rc == NOVA_SUCCESS && want >= 2;
-    return rc;
-}
-
-void vi_layout_release(nova::MDPVIGPU *vi, const char *fn)
-{
-    sorted_drop(vi->d_V, fn);
-    chunked_drop(vi->d_V, fn);
-}
-
 static bool mdp_device_ready(const nova::MDP *mdp)
 {
     return mdp != nullptr && mdp->n != 0 && mdp->ns != 0 && mdp->m != 0 && mdp->d_S != nullptr &&

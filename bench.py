@@ -575,9 +575,9 @@ DOMAINS = [("tiger_95", 4), ("4x3_95", 5), ("shuttle_95", 4), ("paint_95", 4),
 def bench_autograd(args):
     """The differentiable paths (no counterpart in the reference) on the workloads of their
     tests. Differentiable VI on vi_diff_n2e5 (n = 200,000, m = 8, ns = 16, gamma = 0.99): the
-    residual-stopped solve to 1e-6 (FP32 sweeps while the residual falls, then FP64, the
-    iterate moved along the slow error mode between checks), the same solve with plain sweeps
-    for comparison, then the adjoint (I - gamma P_pi)^T lambda = rho with dL/dR and dL/dT, each
+    residual-stopped solve to 1e-6 (FP64 sweeps, the iterate moved along the slow error mode
+    between checks), the same solve with plain sweeps for comparison (float, then FP64), then
+    the adjoint (I - gamma P_pi)^T lambda = rho with dL/dR and dL/dT, each
     timed after a warm-up; CPU: the oracle's FP64 sweep on one pinned core over 10 sweeps of
     the same instance, and that rate over the GPU solve's sweep count. PBVI
     argmax replay on pbvi_diff_n5000 (n = 5,000, r = 1,024, 30 iterations): the recorded

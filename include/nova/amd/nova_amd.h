@@ -40,25 +40,25 @@ extern "C" int mdp_vi_batch_execute_gpu(const MDP *const *mdps, MDPVIGPU *const 
         unsigned int count, MDPValueFunction *policies, float *elapsedMs);
 
 // Solves V* = max_a (R + gamma T.V*) on the device to an FP64 residual max|V' - V| <= tol.
-// Starts from vi->d_V (float; vi initialized or its d_* fields set by the caller) with float
-// sweeps (on the state's sweep layout if it has one) while their residual still falls by 0.1%
-// per 8 sweeps, then continues in FP64 until the residual is <= tol or has stopped falling
-// (the FP64 rounding floor). Between blocks of 8 sweeps the iterate is moved along the slow
-// mode of the error when the last step shows it: by MacQueen's midpoint shift along the
-// constant vector when the step is of one sign and nearly uniform (rows of T summing to 1),
-// else by the remaining steps of a mode that shrinks by a steady ratio, once two checks agree
-// on it. The fixed point is that of plain sweeps; the solve stops on the residual alone.
-// d_V64 is caller device memory of 2n doubles; on return d_V64[0..n) holds V* and vi->d_pi
-// the greedy policy of the last sweep. flags bit 0: skip the float phase and start from
-// d_V64[0..n) (a warm start); bit 1: plain sweeps, without the moves. sweepsDone / residual
-// (nullable) report the sweeps run and the last FP64 residual. Only device arrays of mdp
-// (d_S, d_T, d_R) are used.
+// Starts from vi->d_V (float; vi initialized or its d_* fields set by the caller) and runs
+// FP64 sweeps until the residual is <= tol or has stopped falling (the FP64 rounding floor).
+// Between blocks of 4 sweeps the iterate is moved along the slow mode of the error when the
+// last step shows it: by MacQueen's midpoint shift along the constant vector when the step is
+// of one sign and nearly uniform (rows of T summing to 1), else by the remaining steps of a
+// mode that shrinks by a steady ratio, once two checks agree on it. The fixed point is that
+// of plain sweeps; the solve stops on the residual alone. flags bit 1: plain sweeps, without
+// the moves, first in float (on the state's sweep layout if it has one) while their residual
+// still falls by 0.1% per 8 sweeps, then in FP64, checked every 8. flags bit 0: start the
+// FP64 sweeps from d_V64[0..n) (a warm start). d_V64 is caller device memory of 2n doubles;
+// on return d_V64[0..n) holds V* and vi->d_pi the greedy policy of the last sweep.
+// sweepsDone / residual (nullable) report the sweeps run and the last FP64 residual. Only
+// device arrays of mdp (d_S, d_T, d_R) are used.
 extern "C" int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
         unsigned int maxSweeps, unsigned int flags, unsigned int *sweepsDone, double *residual);
 
 // The adjoint of that solve at policy d_pi (n) and values d_V64 (n doubles): solves
-// (I - gamma P_pi)^T lambda = rho for d_rho (n floats) by FP32 fixed-point iterations (every
-// 16, when the last two steps agree on one slowly shrinking mode, its remaining steps are
+// (I - gamma P_pi)^T lambda = rho for d_rho (n floats) by FP64 fixed-point iterations (every
+// 8, when the last two steps agree on one slowly shrinking mode, its remaining steps are
 // added at once) until
 // max|lambda_{k+1} - lambda_k| <= tol * max|lambda - rho| (or maxIters), writes lambda to
 // d_lambda (n floats) and, when non-null, dL/dR into d_dR (n*m floats) and dL/dT into d_dT

@@ -28,7 +28,7 @@ def _mdp_view(R, T, S, gamma):
 
 def solve(R, T, S, gamma, tol=1e-6, max_sweeps=100000, V0=None, warm64=None, plain=False):
     """(V64, pi, sweeps, residual): the FP64 fixed point on the device and its greedy policy.
-    warm64: optional (n,) float64 CUDA tensor to start the FP64 phase from. plain: sweeps
+    warm64: optional (n,) float64 CUDA tensor to start the FP64 sweeps from. plain: sweeps
     only, without the shifts along the constant vector that remove the slow part of the
     error (same fixed point, about 1/(1 - gamma) times more sweeps)."""
     lib = _abi.load()

@@ -1,23 +1,23 @@
 // nova-mi355x: value iteration to a residual threshold and its adjoint (no reference
 // counterpart: nova's VI runs a fixed horizon, mdp_vi_cpu.cpp:120-131, and has no gradients).
 //
-// Forward (mdp_vi_solve_gpu): float sweeps with the bit-exact kernels (on the state's sweep
-// layout if it has one) while the float residual max|V' - V| still falls, then FP64 sweeps (V
-// in double, T and R promoted) until the FP64 residual is <= tol. FP32 alone cannot resolve
-// residuals below ulp(|V|) (7.6e-6 at |V| = 78). Between blocks of 8 sweeps the iterate is
-// moved along the slow mode of the error when the last sweep's step shows it (SolveChecks):
-// at gamma = 0.99 plain sweeps shrink that mode by 1% each, about 1,400 sweeps to 1e-6, and
-// the moves leave the few dozen sweeps the faster modes need. The greedy policy of the last
-// sweep is left in d_pi.
+// Forward (mdp_vi_solve_gpu): FP64 sweeps (V in double, T and R promoted) until the residual
+// max|V' - V| is <= tol. Between blocks of 4 sweeps the iterate is moved along the slow mode
+// of the error when the last sweep's step shows it (SolveChecks): at gamma = 0.99 plain
+// sweeps shrink that mode by 1% each, about 1,400 sweeps to 1e-6, and the moves leave the
+// dozen or two sweeps the faster modes need. Plain sweeps (flags bit 1) run first in float
+// with the bit-exact kernels (on the state's sweep layout if it has one) while the float
+// residual still falls; FP32 alone cannot resolve residuals below ulp(|V|) (7.6e-6 at
+// |V| = 78). The greedy policy of the last sweep is left in d_pi.
 //
 // Backward (mdp_vi_adjoint_gpu): at the converged greedy policy pi, L(V*) has
 //   dL/dR[s,a]   = lambda(s) [a = pi(s)]
 //   dL/dT[s,a,i] = gamma * lambda(s) * V*(S[s,a,i]) [a = pi(s)]
 // with (I - gamma P_pi)^T lambda = rho, rho = dL/dV*. lambda is found by the fixed-point
-// iteration lambda <- rho + gamma P_pi^T lambda in FP32 over a transposed successor list (the
+// iteration lambda <- rho + gamma P_pi^T lambda in FP64 over a transposed successor list (the
 // pull form, built once per call with a stable radix sort, so sums run in a fixed order and
-// results are reproducible), with the slow mode's remaining steps added once the last two
-// steps isolate it; the final residual is evaluated in FP64.
+// results are reproducible), checked every 8 steps, with the slow mode's remaining steps added
+// once the last two steps isolate it; the final residual is evaluated in FP64.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -88,6 +88,25 @@ __global__ __launch_bounds__(kB) void vi_sweep_f64(unsigned n, unsigned m, unsig
     }
 }
 
+// Maximum over a workgroup (every lane returns it). The reductions below take it before their
+// one atomic per workgroup: an atomic per wave on one address serialises thousands of them.
+__device__ __forceinline__ double block_max(double v, double *lds)
+{
+    for (int off = 32; off >= 1; off >>= 1)
+        v = fmax(v, __shfl_xor(v, off));
+    __syncthreads();
+    if (threadIdx.x % 64 == 0)
+        lds[threadIdx.x / 64] = v;
+    __syncthreads();
+    double t = lds[0];
+    for (unsigned i = 1; i < kB / 64; i++)
+        t = fmax(t, lds[i]);
+    return t;
+}
+
+// Workgroups of the reductions (grid-stride loops over the states)
+constexpr unsigned kRedBlocks = 256;
+
 template <typename VT>
 __global__ __launch_bounds__(kB) void max_abs_diff(unsigned n, const VT *__restrict__ a,
         const VT *__restrict__ b, unsigned long long *__restrict__ out)
@@ -95,9 +114,9 @@ __global__ __launch_bounds__(kB) void max_abs_diff(unsigned n, const VT *__restr
     double mx = 0.0;
     for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB)
         mx = fmax(mx, fabs((double)a[s] - (double)b[s]));
-    for (int off = 32; off >= 1; off >>= 1)
-        mx = fmax(mx, __shfl_xor(mx, off));
-    if (threadIdx.x % 64 == 0)
+    __shared__ double lds[kB / 64];
+    mx = block_max(mx, lds);
+    if (threadIdx.x == 0)
         atomicMax(out, (unsigned long long)__double_as_longlong(mx));
 }
 
@@ -129,11 +148,10 @@ __global__ __launch_bounds__(kB) void diff_extremes(unsigned n, const VT *__rest
         hi = fmax(hi, d);
         lo = fmax(lo, -d);
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        hi = fmax(hi, __shfl_xor(hi, off));
-        lo = fmax(lo, __shfl_xor(lo, off));
-    }
-    if (threadIdx.x % 64 == 0 && hi > -INFINITY) {
+    __shared__ double lds[kB / 64];
+    hi = block_max(hi, lds);
+    lo = block_max(lo, lds);
+    if (threadIdx.x == 0 && hi > -INFINITY) {
         atomicMax(&out[0], order_key(hi));
         atomicMax(&out[1], order_key(lo));
     }
@@ -197,17 +215,17 @@ __global__ void adj_gather_t(unsigned n, unsigned m, unsigned ns, size_t E,
     tv[q] = T[((size_t)s * m + pi[s]) * ns + l];
 }
 
-// lambda'(sp) = rho(sp) + gamma * sum_q T_q lambda(src_q), q over the predecessors of sp
-__global__ void adj_step(unsigned n, float gamma, const unsigned *__restrict__ start,
+// lambda'(sp) = rho(sp) + gamma * sum_q T_q lambda(src_q), q over the predecessors of sp (FP64)
+__global__ void adj_step(unsigned n, double gamma, const unsigned *__restrict__ start,
         const unsigned *__restrict__ src, const float *__restrict__ tv,
-        const float *__restrict__ rho, const float *__restrict__ lam, float *__restrict__ out)
+        const double *__restrict__ rho, const double *__restrict__ lam, double *__restrict__ out)
 {
     const unsigned sp = blockIdx.x * kB + threadIdx.x;
     if (sp >= n)
         return;
-    float acc = 0.0f;
+    double acc = 0.0;
     for (unsigned q = start[sp]; q < start[sp + 1]; q++)
-        acc += tv[q] * lam[src[q]];
+        acc += (double)tv[q] * lam[src[q]];
     out[sp] = rho[sp] + gamma * acc;
 }
 
@@ -232,14 +250,14 @@ __device__ __forceinline__ double block_sum(double v, double *lds)
 }
 
 __global__ __launch_bounds__(kB) void adj_sum_partials(unsigned n,
-        const float *__restrict__ l0, const float *__restrict__ l1, const float *__restrict__ l2,
-        double *__restrict__ part)
+        const double *__restrict__ l0, const double *__restrict__ l1,
+        const double *__restrict__ l2, double *__restrict__ part)
 {
     __shared__ double lds[kB / 64];
     double s1 = 0.0, s2 = 0.0;
     for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB) {
-        s1 += (double)l1[s] - (double)l0[s];
-        s2 += (double)l2[s] - (double)l1[s];
+        s1 += l1[s] - l0[s];
+        s2 += l2[s] - l1[s];
     }
     s1 = block_sum(s1, lds);
     s2 = block_sum(s2, lds);
@@ -267,29 +285,36 @@ __global__ __launch_bounds__(kB) void adj_sum_final(unsigned blocks,
 }
 
 // max |d2 - q d1| over the states (the part of the last step outside the ratio-q mode)
-__global__ __launch_bounds__(kB) void adj_mode_dev(unsigned n, const float *__restrict__ l0,
-        const float *__restrict__ l1, const float *__restrict__ l2, double q,
+__global__ __launch_bounds__(kB) void adj_mode_dev(unsigned n, const double *__restrict__ l0,
+        const double *__restrict__ l1, const double *__restrict__ l2, double q,
         unsigned long long *__restrict__ out)
 {
     double mx = 0.0;
     for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB) {
-        const double d1 = (double)l1[s] - (double)l0[s], d2 = (double)l2[s] - (double)l1[s];
+        const double d1 = l1[s] - l0[s], d2 = l2[s] - l1[s];
         mx = fmax(mx, fabs(d2 - q * d1));
     }
-    for (int off = 32; off >= 1; off >>= 1)
-        mx = fmax(mx, __shfl_xor(mx, off));
-    if (threadIdx.x % 64 == 0)
+    __shared__ double lds[kB / 64];
+    mx = block_max(mx, lds);
+    if (threadIdx.x == 0)
         atomicMax(out, (unsigned long long)__double_as_longlong(mx));
 }
 
 // out = l2 + (l2 - l1) f: the sum of the remaining steps of a mode that shrinks by q per step,
 // f = q / (1 - q)
-__global__ void adj_extrapolate(unsigned n, const float *__restrict__ l1,
-        const float *__restrict__ l2, double f, float *__restrict__ out)
+__global__ void adj_extrapolate(unsigned n, const double *__restrict__ l1,
+        const double *__restrict__ l2, double f, double *__restrict__ out)
 {
     const unsigned s = blockIdx.x * kB + threadIdx.x;
     if (s < n)
-        out[s] = (float)((double)l2[s] + ((double)l2[s] - (double)l1[s]) * f);
+        out[s] = l2[s] + (l2[s] - l1[s]) * f;
+}
+
+__global__ void to_float(unsigned n, const double *__restrict__ a, float *__restrict__ b)
+{
+    const unsigned s = blockIdx.x * kB + threadIdx.x;
+    if (s < n)
+        b[s] = (float)a[s];
 }
 
 // FP64 residual of the adjoint equation, per state (max taken by max_abs_diff-like reduction)
@@ -305,9 +330,9 @@ __global__ void adj_resid64(unsigned n, double gamma, const unsigned *__restrict
             acc += (double)tv[q] * (double)lam[src[q]];
         mx = fmax(mx, fabs((double)lam[sp] - (double)rho[sp] - gamma * acc));
     }
-    for (int off = 32; off >= 1; off >>= 1)
-        mx = fmax(mx, __shfl_xor(mx, off));
-    if (threadIdx.x % 64 == 0)
+    __shared__ double lds[kB / 64];
+    mx = block_max(mx, lds);
+    if (threadIdx.x == 0)
         atomicMax(out, (unsigned long long)__double_as_longlong(mx));
 }
 
@@ -380,9 +405,9 @@ __global__ __launch_bounds__(kB) void ratio_dev(unsigned n, const VT *__restrict
     double mx = 0.0;
     for (unsigned s = blockIdx.x * kB + threadIdx.x; s < n; s += gridDim.x * kB)
         mx = fmax(mx, fabs(((double)cur[s] - (double)prv[s]) - q * dprev[s]));
-    for (int off = 32; off >= 1; off >>= 1)
-        mx = fmax(mx, __shfl_xor(mx, off));
-    if (threadIdx.x % 64 == 0)
+    __shared__ double lds[kB / 64];
+    mx = block_max(mx, lds);
+    if (threadIdx.x == 0)
         atomicMax(out, (unsigned long long)__double_as_longlong(mx));
 }
 
@@ -407,8 +432,9 @@ __global__ void ratio_extrapolate(unsigned n, VT *__restrict__ cur, const VT *__
 //   sweeps shrink by gamma alone; the other modes had already died out.
 // * otherwise (rows summing below 1, as when successor lists are cut short), once the policy
 //   has settled the sweeps are linear and the slowest mode of gamma P_pi is left: d is then
-//   q8 = sum(d) / sum(dprev) times the step of the check before, 8 sweeps earlier, up to a
-//   tenth of max|d|, and the remaining steps of that mode sum to d q / (1 - q), q = q8^(1/8).
+//   qb = sum(d) / sum(dprev) times the step of the check before, b sweeps earlier (block), up
+//   to a tenth of max|d|, and the remaining steps of that mode sum to d q / (1 - q),
+//   q = qb^(1/b).
 // Either move is followed by at least one more check of plain sweeps, and the solve stops on
 // the residual alone, so a move can cost sweeps but never accuracy. The sums run in a fixed
 // order, so the solve is reproducible.
@@ -419,6 +445,7 @@ struct SolveChecks {
     double *dprev = nullptr;              // n doubles
     double *part = nullptr;               // 2 kSumBlocks partials + 2 sums
     bool havePrev = false;
+    unsigned block = 8;                   // sweeps between two checks
     double dmin = 0.0, dmax = 0.0;
 
     int alloc(const char *fn)
@@ -438,7 +465,7 @@ struct SolveChecks {
             if (p)
                 device_free(p, fn, "the solve's checks");
     }
-    unsigned grid() const { return std::min(1024u, (n + kB - 1) / kB); }
+    unsigned grid() const { return std::min(kRedBlocks, (n + kB - 1) / kB); }
 
     template <typename VT>
     double measure(const VT *cur, const VT *prv, hipStream_t st)
@@ -475,13 +502,13 @@ struct SolveChecks {
             double sums[2] = {0.0, 0.0};
             hipMemcpyAsync(sums, part + 2 * kSumBlocks, sizeof(sums), hipMemcpyDeviceToHost, st);
             hipStreamSynchronize(st);
-            const double q8 = (sums[0] != 0.0) ? sums[1] / sums[0] : 0.0;
-            if (q8 > 0.0 && q8 < 1.0) {
+            const double qb = (sums[0] != 0.0) ? sums[1] / sums[0] : 0.0;
+            if (qb > 0.0 && qb < 1.0) {
                 hipMemsetAsync(dext + 2, 0, sizeof(unsigned long long), st);
                 hipLaunchKernelGGL(ratio_dev<VT>, dim3(grid()), dim3(kB), 0, st, n,
-                                   (const VT *)cur, prv, (const double *)dprev, q8, dext + 2);
+                                   (const VT *)cur, prv, (const double *)dprev, qb, dext + 2);
                 if (read_max(dext + 2, st) <= 0.1 * res) {
-                    const double q = std::pow(q8, 1.0 / 8.0);
+                    const double q = std::pow(qb, 1.0 / (double)block);
                     hipLaunchKernelGGL(ratio_extrapolate<VT>, dim3(eg), dim3(kB), 0, st, n, cur,
                                        prv, q / (1.0 - q));
                     havePrev = false;
@@ -559,13 +586,20 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
     }
     unsigned done = 0;
     double res = 1e300;
-    const unsigned check = 8;
     const unsigned eg = (n + kB - 1) / kB;
     const bool shifts = !(flags & 2u);     // bit 1: plain sweeps, no moves between checks
-    if (!(flags & 1u)) {
-        // float phase, on the state's sweep layout if it has one (a few checks at most, too few
-        // to pay for building one): continue while the float residual still falls (by 0.1% per
-        // check of 8 sweeps), that is until it reaches the rounding floor of float V
+    // With the moves, a solve is a few dozen sweeps, all in FP64 (at n = 200,000 an FP64 sweep
+    // costs no more than a float one without a sweep layout, and FP64 keeps the steps the moves
+    // read free of float rounding), checked every 4. Plain sweeps first run in float on the
+    // state's sweep layout, checked every 8.
+    const unsigned check = shifts ? 4 : 8;
+    chk.block = check;
+    if (!(flags & 1u) && shifts) {
+        hipLaunchKernelGGL(to_double, dim3(eg), dim3(kB), 0, st, n, (const float *)vi->d_V,
+                           d_V64);
+    } else if (!(flags & 1u)) {
+        // float phase: continue while the float residual still falls (by 0.1% per check of 8
+        // sweeps), that is until it reaches the rounding floor of float V
         double prev = 1e300;
         while (rc == NOVA_SUCCESS && done + check <= maxSweeps) {
             rc = vi_sweeps_float(mdp, vi, check, st, fn);
@@ -579,8 +613,6 @@ int mdp_vi_solve_gpu(const MDP *mdp, MDPVIGPU *vi, double *d_V64, double tol,
             if (r <= tol || r > 0.999 * prev)
                 break;
             prev = r;
-            if (shifts)
-                chk.accelerate<float>(cur, prv, st);
         }
         const float *cur = (vi->currentHorizon % 2 == 0) ? vi->d_V : vi->d_VPrime;
         hipLaunchKernelGGL(to_double, dim3(eg), dim3(kB), 0, st, n, cur, d_V64);
@@ -641,8 +673,8 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
     // transposed successor list of the policy's rows (stable sort: fixed order per bucket)
     unsigned *keys = nullptr, *keys2 = nullptr, *slot = nullptr, *slot2 = nullptr;
     unsigned *start = nullptr, *src = nullptr;
-    float *tv = nullptr, *lamB = nullptr, *lamC = nullptr;
-    double *part = nullptr;
+    float *tv = nullptr;
+    double *lam64 = nullptr, *part = nullptr;   // lam64: rho and three iterates, n doubles each
     unsigned long long *dmax = nullptr;
     void *temp = nullptr;
     size_t tempBytes = 0;
@@ -653,8 +685,8 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&start, (n + 1) * 4ull, fn, "adjoint buffers");
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&src, E * 4, fn, "adjoint buffers");
     if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&tv, E * 4, fn, "adjoint buffers");
-    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&lamB, n * 4ull, fn, "adjoint buffers");
-    if (rc == NOVA_SUCCESS) rc = device_alloc((void **)&lamC, n * 4ull, fn, "adjoint buffers");
+    if (rc == NOVA_SUCCESS)
+        rc = device_alloc((void **)&lam64, 4ull * n * sizeof(double), fn, "adjoint buffers");
     if (rc == NOVA_SUCCESS)
         rc = device_alloc((void **)&part, sizeof(double) * (2 * kSumBlocks + 2), fn,
                           "adjoint buffers");
@@ -678,39 +710,45 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
     double res64 = 0.0;
     if (rc == NOVA_SUCCESS) {
         const unsigned g = (n + kB - 1) / kB;
-        const unsigned rgrid = std::min(1024u, g);
-        hipMemcpyAsync(d_lambda, d_rho, n * 4ull, hipMemcpyDeviceToDevice, st);
+        const unsigned rgrid = std::min(kRedBlocks, g);
+        // FP64 iterates: the moves below read differences of consecutive steps, which FP32
+        // rounding (about 6e-8 of lambda) would bury before the tolerance is reached
+        double *rho64 = lam64;
+        hipLaunchKernelGGL(to_double, dim3(g), dim3(kB), 0, st, n, d_rho, rho64);
+        hipMemcpyAsync(lam64 + n, rho64, n * sizeof(double), hipMemcpyDeviceToDevice, st);
         // cur = lambda_{k+1}, prv = lambda_k, old = lambda_{k-1}
-        float *cur = d_lambda, *prv = lamB, *old = lamC;
-        const unsigned check = 16;
+        double *cur = lam64 + n, *prv = lam64 + 2 * n, *old = lam64 + 3 * n;
+        const unsigned check = 8;
         const unsigned sb = std::min((unsigned)kSumBlocks, g);
         while (it < maxIters) {
             unsigned k = 0;
             for (; k < check && it < maxIters; k++, it++) {
-                hipLaunchKernelGGL(adj_step, dim3(g), dim3(kB), 0, st, n, mdp->gamma, start, src,
-                                   tv, d_rho, cur, old);
-                float *t3 = old;
+                hipLaunchKernelGGL(adj_step, dim3(g), dim3(kB), 0, st, n, (double)mdp->gamma,
+                                   start, src, tv, (const double *)rho64, (const double *)cur,
+                                   old);
+                double *t3 = old;
                 old = prv;
                 prv = cur;
                 cur = t3;
             }
             hipMemsetAsync(dmax, 0, 8, st);
-            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, cur, prv,
+            hipLaunchKernelGGL(max_abs_diff<double>, dim3(rgrid), dim3(kB), 0, st, n, cur, prv,
                                dmax);
             const double d = read_max(dmax, st);
             hipMemsetAsync(dmax, 0, 8, st);
-            hipLaunchKernelGGL(max_abs_diff<float>, dim3(rgrid), dim3(kB), 0, st, n, cur,
-                               (const float *)d_rho, dmax);
+            hipLaunchKernelGGL(max_abs_diff<double>, dim3(rgrid), dim3(kB), 0, st, n, cur,
+                               (const double *)rho64, dmax);
             const double scale = read_max(dmax, st);   // max|lambda - rho|, a scale of lambda
             if (d <= (double)tol * std::max(1e-30, scale) || it >= maxIters || k < 2)
                 break;
             // The error of the iteration lambda <- rho + gamma P^T lambda is a sum of modes that
             // shrink by gamma times the eigenvalues of P^T per step; with P's rows summing to 1
             // the slowest is the stationary distribution's, at gamma, and the others die out
-            // much faster. Once the last step d2 is q times the step before it up to a tenth
-            // (q from the sums of the two steps), what is left of the error is that one mode,
+            // much faster. Once the last step d2 is q = sum(d2) / sum(d1) times the step d1
+            // before it, up to a tenth of max|d2|, what is left of the error is that one mode,
             // and its remaining steps sum to d2 q / (1 - q): add them.
-            hipLaunchKernelGGL(adj_sum_partials, dim3(sb), dim3(kB), 0, st, n, old, prv, cur,
+            hipLaunchKernelGGL(adj_sum_partials, dim3(sb), dim3(kB), 0, st, n,
+                               (const double *)old, (const double *)prv, (const double *)cur,
                                part);
             hipLaunchKernelGGL(adj_sum_final, dim3(1), dim3(kB), 0, st, sb, part,
                                part + 2 * kSumBlocks);
@@ -723,16 +761,16 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
             if (!(q > 0.0 && q < 1.0))
                 continue;
             hipMemsetAsync(dmax, 0, 8, st);
-            hipLaunchKernelGGL(adj_mode_dev, dim3(rgrid), dim3(kB), 0, st, n, old, prv, cur, q,
+            hipLaunchKernelGGL(adj_mode_dev, dim3(rgrid), dim3(kB), 0, st, n,
+                               (const double *)old, (const double *)prv, (const double *)cur, q,
                                dmax);
             if (read_max(dmax, st) <= 0.1 * d) {
-                hipLaunchKernelGGL(adj_extrapolate, dim3(g), dim3(kB), 0, st, n, prv, cur,
-                                   q / (1.0 - q), old);
+                hipLaunchKernelGGL(adj_extrapolate, dim3(g), dim3(kB), 0, st, n,
+                                   (const double *)prv, (const double *)cur, q / (1.0 - q), old);
                 std::swap(cur, old);
             }
         }
-        if (cur != d_lambda)
-            hipMemcpyAsync(d_lambda, cur, n * 4ull, hipMemcpyDeviceToDevice, st);
+        hipLaunchKernelGGL(to_float, dim3(g), dim3(kB), 0, st, n, (const double *)cur, d_lambda);
         hipMemsetAsync(dmax, 0, 8, st);
         hipLaunchKernelGGL(adj_resid64, dim3(rgrid), dim3(kB), 0, st, n, (double)mdp->gamma,
                            start, src, tv, d_rho, d_lambda, dmax);
@@ -745,7 +783,7 @@ int mdp_vi_adjoint_gpu(const MDP *mdp, const unsigned int *d_pi, const double *d
         if (e != hipSuccess)
             rc = hip_fail(e, fn, "Failed to run the adjoint.", NOVA_ERROR_KERNEL_EXECUTION);
     }
-    void *bufs[] = {keys, keys2, slot, slot2, start, src, tv, lamB, lamC, part, dmax, temp};
+    void *bufs[] = {keys, keys2, slot, slot2, start, src, tv, lam64, part, dmax, temp};
     for (void *p : bufs)
         if (p)
             device_free(p, fn, "adjoint buffers");

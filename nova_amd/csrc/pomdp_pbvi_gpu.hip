@@ -1239,6 +1239,49 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
 // Threads of the materialisation workgroup: 16 waves per belief keep enough loads in flight
 // while one or two workgroups per CU hold their n-float row in LDS.
 constexpr int kMB = 1024;
+constexpr int kMatStates = 4;        // exact materialization: states per lane requested together
+
+// Exact materialization of one observation's term for the lane's states t, t + kMB, ...,
+// kMatStates of them at a time with all their NS successors and factors requested together
+// (constant offsets from each state's row): per state the same products and sums, in the same
+// order, as the general loop (entries after the first terminator take no part; its index is
+// clamped to 0 for the unused read). Returns the first state it left to that loop.
+template <int NS>
+__device__ __forceinline__ unsigned materialize_exact_states(const int *sa, const float *ot,
+        const float *__restrict__ srcv, float *out, float gamma, unsigned n, unsigned t)
+{
+    const NOVA_GLOBAL int *gsa = (const NOVA_GLOBAL int *)sa;
+    const NOVA_GLOBAL float *got = (const NOVA_GLOBAL float *)ot;
+    NOVA_GLOBAL float *gout = (NOVA_GLOBAL float *)out;
+    unsigned s0 = t;
+    for (; s0 + (kMatStates - 1) * kMB < n; s0 += kMatStates * kMB) {
+        int spv[kMatStates][NS];
+        float otv[kMatStates][NS];
+#pragma unroll
+        for (int q = 0; q < kMatStates; q++) {
+            const size_t rw = (size_t)(s0 + q * kMB) * NS;
+#pragma unroll
+            for (int l = 0; l < NS; l++) {
+                spv[q][l] = gsa[rw + l];
+                otv[q][l] = got[rw + l];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMatStates; q++) {
+            float vt = 0.0f;
+            bool stop = false;
+#pragma unroll
+            for (int l = 0; l < NS; l++) {
+                stop = stop || spv[q][l] < 0;
+                const float g = srcv[max(spv[q][l], 0)];
+                vt = stop ? vt : __fadd_rn(vt, __fmul_rn(otv[q][l], g));
+            }
+            const unsigned s = s0 + q * kMB;
+            gout[s] = __fadd_rn(gout[s], __fmul_rn(vt, gamma));
+        }
+    }
+    return s0;
+}
 
 template <bool EXACT>
 __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restrict__ insts,
@@ -1276,7 +1319,12 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
             const float *srcv = useLds ? row : Gj;
             const float *ot = P.OT + ((size_t)(a * z + o) * n) * ns;
             const int *sa = P.SA + (size_t)a * n * ns;    // the action-major copy of S
-            for (unsigned s = t; s < n; s += kMB) {
+            unsigned s0 = t;
+            if (useLds && ns == 8)     // row passed as it is, so its reads stay LDS reads
+                s0 = materialize_exact_states<8>(sa, ot, row, out, gamma, n, t);
+            else if (useLds && ns == 4)
+                s0 = materialize_exact_states<4>(sa, ot, row, out, gamma, n, t);
+            for (unsigned s = s0; s < n; s += kMB) {
                 const size_t rw = (size_t)s * ns;
                 float vt = 0.0f;
                 // successors and factors four at a time (requested together), summed in order

@@ -1109,12 +1109,46 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
 // EXACT: each action's alpha is formed on the belief's support in the reference's rounding order
 // and dotted with b, as nova chooses. Fast: the action value is b . R[:,a] plus the observations'
 // best scores (each score is already the b-weighted term of that observation).
+// Exact mode's inner sum of one (support state, observation) pair, sum_l O*T * Gamma[j, S] in
+// the reference's order up to the first terminator, with the NS successors and factors
+// requested together (the terminator's and later entries read but not summed, index 0 in
+// their place).
+constexpr int kActKC = 32;           // exact action: support entries per chunk of inner sums
+
+template <int NS>
+__device__ __forceinline__ float exact_inner_sum(const int *sRow, const float *otRow,
+                                                 const float *Gj)
+{
+    const NOVA_GLOBAL int *gs = (const NOVA_GLOBAL int *)sRow;
+    const NOVA_GLOBAL float *go = (const NOVA_GLOBAL float *)otRow;
+    const NOVA_GLOBAL float *gg = (const NOVA_GLOBAL float *)Gj;
+    int sp[NS];
+    float ov[NS], gv[NS];
+#pragma unroll
+    for (int l = 0; l < NS; l++) {
+        sp[l] = gs[l];
+        ov[l] = go[l];
+    }
+#pragma unroll
+    for (int l = 0; l < NS; l++)
+        gv[l] = gg[max(sp[l], 0)];
+    float vt = 0.0f;
+    bool stop = false;
+#pragma unroll
+    for (int l = 0; l < NS; l++) {
+        stop = stop || sp[l] < 0;
+        vt = stop ? vt : __fadd_rn(vt, __fmul_rn(ov[l], gv[l]));
+    }
+    return vt;
+}
+
 template <bool EXACT>
 __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
     __shared__ int js[kZC * 8];
     __shared__ float bestO[kZC * 8];
+    __shared__ float vtL[EXACT ? kActKC * kZC * 8 : 1];   // exact: inner sums of a chunk
     __shared__ int kEndS;
     // alphaK and bS: one float per support entry each (exact mode; fast mode uses alphaK[0]),
     // dynamic LDS of 2 * max(kKMax, the longest exact support) floats (plan action_lds)
@@ -1180,26 +1214,44 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
                 __syncthreads();
                 continue;
             }
-            // alpha_a on the support, observation-major accumulation as the reference
-            for (int k = t; k < kEnd; k += kPB) {
-                const int s = P.Z[(size_t)i * rz + k];
-                const size_t row = ((size_t)s * m + a) * ns;
-                float al = (o0 == 0) ? P.R[(size_t)s * m + a] : alphaK[k];
-                for (unsigned oo = 0; oo < oc; oo++) {
+            // alpha_a on the support, observation-major accumulation as the reference: per
+            // chunk of kActKC entries, the inner sums vt of every (entry, observation) pair in
+            // parallel (into vtL), then each entry adds its terms gamma * vt in observation order
+            for (int k0 = 0; k0 < kEnd; k0 += kActKC) {
+                const int kc = min(kActKC, kEnd - k0);
+                for (unsigned x = t; x < (unsigned)kc * oc; x += kPB) {
+                    const unsigned kk = x / oc, oo = x % oc;
+                    const int s = P.Z[(size_t)i * rz + k0 + kk];
+                    const size_t row = ((size_t)s * m + a) * ns;
                     const float *Gj = G + (size_t)js[oo] * n;
                     const float *ot = P.OT + ((size_t)(a * z + o0 + oo) * n + s) * ns;
-                    float vt = 0.0f;
-                    for (unsigned l = 0; l < ns; l++) {
-                        const int sp = P.S[row + l];
-                        if (sp < 0)
-                            break;
-                        vt = __fadd_rn(vt, __fmul_rn(ot[l], Gj[sp]));
+                    float vt;
+                    if (ns == 8)
+                        vt = exact_inner_sum<8>(P.S + row, ot, Gj);
+                    else if (ns == 4)
+                        vt = exact_inner_sum<4>(P.S + row, ot, Gj);
+                    else {
+                        vt = 0.0f;
+                        for (unsigned l = 0; l < ns; l++) {
+                            const int sp = P.S[row + l];
+                            if (sp < 0)
+                                break;
+                            vt = __fadd_rn(vt, __fmul_rn(ot[l], Gj[sp]));
+                        }
                     }
-                    al = __fadd_rn(al, __fmul_rn(vt, gamma));
+                    vtL[kk * (kZC * 8) + oo] = vt;
                 }
-                alphaK[k] = al;
+                __syncthreads();
+                for (int kk = t; kk < kc; kk += kPB) {
+                    const int k = k0 + kk;
+                    const int s = P.Z[(size_t)i * rz + k];
+                    float al = (o0 == 0) ? P.R[(size_t)s * m + a] : alphaK[k];
+                    for (unsigned oo = 0; oo < oc; oo++)
+                        al = __fadd_rn(al, __fmul_rn(vtL[kk * (kZC * 8) + oo], gamma));
+                    alphaK[k] = al;
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
         if (t == 0) {
             float v = 0.0f;

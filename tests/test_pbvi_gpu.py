@@ -255,6 +255,46 @@ def test_full_size_sample_vs_oracle(oracle):
             assert err <= 1e-5
 
 
+def test_exact_full_size_sample_vs_oracle(oracle):
+    """At the benchmark's size (n = 20,000, m = 16, z = 32, ns = 8, r = 4,096, rz = 64), one
+    exact iteration from a random Gamma: the new alphas, actions and alpha choices of sampled
+    beliefs (first, middle, last) equal the float oracle's bit for bit. This size runs the
+    exact kernels' batched paths (each lane's states four at a time in the materialization,
+    eight successors requested together in the action stage)."""
+    P = synth.workload("pbvi_n20000")
+    rng = np.random.default_rng(13)
+    Gt = (P["Gamma0"] + rng.uniform(-1, 1, P["Gamma0"].shape)).astype(np.float32)
+    set_mode(EXACT)
+    p = POMDP.from_dict(P)
+    p.initialize_gpu()
+    s = POMDPPBVIGPU(p, GammaInitial=Gt)
+    s.update()
+    js, act = s.argmax_record()
+    G1 = s.get_policy().Gamma_array()
+    del s
+    set_mode(FAST)
+    r = Gt.shape[0]
+    for i in (0, r // 2, r - 1):
+        ref = oracle.pbvi_backup(P, Gt, i, i + 1)
+        assert act[i] == int(ref["pi"][0])
+        assert np.array_equal(js[i, act[i]], ref["jstar"][0, act[i]])
+        assert np.array_equal(bits(G1[i]), bits(ref["GammaPrime"][0])), "belief %d" % i
+        print("pbvi_n20000 exact, belief %d: bit-identical" % i, flush=True)
+
+
+@pytest.mark.parametrize("ns", [4, 8])
+def test_exact_batched_paths_bitwise_vs_oracle(oracle, ns):
+    """Exact whole runs at n = 5,000 (more than three states per lane of the 1,024-lane
+    materialization, so its four-states-at-a-time path runs) with ns = 4 and 8 (the action
+    stage's batched inner sums) and ragged successor lists and supports (terminators inside
+    the batched reads): Gamma and pi equal the oracle's bit for bit."""
+    P = ragged(synth.pomdp_arrays(n=5000, m=3, z=6, ns=ns, r=48, rz=40, seed=90 + ns,
+                                  gamma=0.95, horizon=3), 90 + ns, frac=0.3)
+    G, pi = gpu_solve(P, EXACT, horizon=3)
+    Go, pio = oracle.pbvi_run(P, horizon=3)
+    assert np.array_equal(bits(G), bits(Go)) and np.array_equal(pi, pio)
+
+
 @pytest.mark.parametrize("n,m,z,ns,r,rz,horizon,seed", [
     (200, 3, 5, 20, 40, 40, 4, 61),      # 800 pairs per (belief, action): two support chunks
     (900, 2, 4, 3, 20, 700, 3, 62),      # rz > 512 entries, ns = 3

@@ -125,23 +125,56 @@ __global__ __launch_bounds__(kB) void dr_accumulate(unsigned n, unsigned m,
     dR[(size_t)s * m + a] = acc;
 }
 
-// mu_t[j][s'] = sum over (i, o) listed for j of gamma O(a_i, s', o) u[i][s'] (+ kept rows)
-__global__ __launch_bounds__(kB) void mu_step(unsigned n, unsigned z, float gamma,
-        const int *__restrict__ act, const unsigned *__restrict__ jstart,
-        const unsigned *__restrict__ jlist, const float *__restrict__ O,
-        const float *__restrict__ u, const float *__restrict__ muNext, float *__restrict__ mu)
+// OTr[(a z + o) n + s'] = O[(a n + s') z + o]: mu_partial reads O with the states contiguous
+__global__ void o_transpose(unsigned n, unsigned m, unsigned z, const float *__restrict__ O,
+        float *__restrict__ OTr)
+{
+    const size_t x = blockIdx.x * (size_t)kB + threadIdx.x;
+    if (x >= (size_t)m * n * z)
+        return;
+    const unsigned o = x % z;
+    const size_t as = x / z;                    // a * n + s'
+    const unsigned a = (unsigned)(as / n), sp = (unsigned)(as % n);
+    OTr[((size_t)a * z + o) * n + sp] = O[x];
+}
+
+// The (i, o) pairs listed for an alpha j, cut into segments of at most kSeg pairs so that a
+// much-chosen alpha is summed by many workgroups: partial[seg][s'] = sum over the segment's
+// (i, o) of gamma O(a_i, s', o) u[i][s'], then mu_combine adds the segments of each alpha in
+// order. The segments are fixed by the lists, so the sums are reproducible.
+constexpr unsigned kSeg = 32;
+
+__global__ void mu_partial(unsigned n, unsigned z, float gamma, const int *__restrict__ act,
+        unsigned segBase, const unsigned *__restrict__ segQ, const unsigned *__restrict__ jlist,
+        const float *__restrict__ OTr, const float *__restrict__ u, float *__restrict__ partial)
+{
+    const unsigned seg = segBase + blockIdx.y;
+    const unsigned sp = blockIdx.x * kB + threadIdx.x;
+    if (sp >= n)
+        return;
+    float acc = 0.0f;
+    for (unsigned q = segQ[2 * seg]; q < segQ[2 * seg + 1]; q++) {
+        const unsigned io = jlist[q];
+        const unsigned i = io / z, o = io % z;
+        const int a = act[i];
+        acc += gamma * OTr[((size_t)a * z + o) * n + sp] * u[(size_t)i * n + sp];
+    }
+    partial[(size_t)blockIdx.y * n + sp] = acc;
+}
+
+// mu_t[j][s'] = the segments' sums of j in order (+ mu_{t+1}[j][s'] when no action was chosen
+// for j: its alpha was kept)
+__global__ void mu_combine(unsigned n, const int *__restrict__ act,
+        const unsigned *__restrict__ sstart, unsigned segBase, const float *__restrict__ partial,
+        const float *__restrict__ muNext, float *__restrict__ mu)
 {
     const unsigned j = blockIdx.y;
     const unsigned sp = blockIdx.x * kB + threadIdx.x;
     if (sp >= n)
         return;
     float acc = (act[j] < 0) ? muNext[(size_t)j * n + sp] : 0.0f;
-    for (unsigned q = jstart[j]; q < jstart[j + 1]; q++) {
-        const unsigned io = jlist[q];
-        const unsigned i = io / z, o = io % z;
-        const int a = act[i];
-        acc += gamma * O[((size_t)a * n + sp) * z + o] * u[(size_t)i * n + sp];
-    }
+    for (unsigned seg = sstart[j]; seg < sstart[j + 1]; seg++)
+        acc += partial[(size_t)(seg - segBase) * n + sp];
     mu[(size_t)j * n + sp] = acc;
 }
 
@@ -259,6 +292,26 @@ int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations, cons
         for (unsigned a = 0; a <= m; a++)
             as[a] += (unsigned)abase;
     }
+    // segments of at most kSeg pairs of each alpha's list: sstart[t (r + 1) + j] is the first
+    // segment of (t, j), segQ[2 seg .. 2 seg + 1] its pair range in jlist
+    std::vector<unsigned> sstart((size_t)H * (r + 1)), segQ;
+    unsigned maxSegs = 1;
+    for (unsigned t = 0; t < H; t++) {
+        const unsigned *js = &jstart[(size_t)t * (r + 1)];
+        const unsigned first = (unsigned)(segQ.size() / 2);
+        for (unsigned j = 0; j < r; j++) {
+            sstart[(size_t)t * (r + 1) + j] = (unsigned)(segQ.size() / 2);
+            for (unsigned q = js[j]; q < js[j + 1]; q += kSeg) {
+                segQ.push_back(q);
+                segQ.push_back(std::min(q + kSeg, js[j + 1]));
+            }
+        }
+        const unsigned end = (unsigned)(segQ.size() / 2);
+        sstart[(size_t)t * (r + 1) + r] = end;
+        maxSegs = std::max(maxSegs, end - first);
+    }
+    if (segQ.empty())
+        segQ.assign(2, 0u);
     // predecessor lists per action
     std::vector<unsigned> pstart((size_t)m * (n + 1), 0);
     for (unsigned s = 0; s < n; s++)
@@ -300,20 +353,24 @@ int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations, cons
     }
     // ---- device buffers
     const size_t rn = (size_t)r * n;
-    void *d[12] = {};
-    const size_t bytes[12] = {sizeof(float) * rn, sizeof(float) * rn, sizeof(float) * rn,
+    // [mu, mu', u | fstart, flist, sstart, jlist, astart, alist, pstart, pS, pT, segQ (uploaded)
+    //  | partial sums of the segments, O transposed]
+    constexpr int kBufs = 15;
+    void *d[kBufs] = {};
+    const size_t bytes[kBufs] = {sizeof(float) * rn, sizeof(float) * rn, sizeof(float) * rn,
                               sizeof(unsigned) * fstart.size(), sizeof(unsigned) * flist.size(),
-                              sizeof(unsigned) * jstart.size(), sizeof(unsigned) * std::max<size_t>(jlist.size(), 1),
+                              sizeof(unsigned) * sstart.size(), sizeof(unsigned) * std::max<size_t>(jlist.size(), 1),
                               sizeof(unsigned) * astart.size(), sizeof(unsigned) * std::max<size_t>(alist.size(), 1),
                               sizeof(unsigned) * pstart.size(), sizeof(int) * pS.size(),
-                              sizeof(float) * pT.size()};
+                              sizeof(float) * pT.size(), sizeof(unsigned) * segQ.size(),
+                              sizeof(float) * maxSegs * (size_t)n, sizeof(float) * (size_t)m * z * n};
     int rc = NOVA_SUCCESS;
-    for (int x = 0; x < 12 && rc == NOVA_SUCCESS; x++)
+    for (int x = 0; x < kBufs && rc == NOVA_SUCCESS; x++)
         rc = device_alloc(&d[x], bytes[x], fn, "replay buffers");
-    const void *src[12] = {nullptr, nullptr, nullptr, fstart.data(), flist.data(), jstart.data(),
-                           jlist.data(), astart.data(), alist.data(), pstart.data(), pS.data(),
-                           pT.data()};
-    for (int x = 3; x < 12 && rc == NOVA_SUCCESS; x++) {
+    const void *src[kBufs] = {nullptr, nullptr, nullptr, fstart.data(), flist.data(),
+                              sstart.data(), jlist.data(), astart.data(), alist.data(),
+                              pstart.data(), pS.data(), pT.data(), segQ.data(), nullptr, nullptr};
+    for (int x = 3; x < 13 && rc == NOVA_SUCCESS; x++) {
         const size_t b = (x == 6) ? sizeof(unsigned) * jlist.size()
                          : (x == 8) ? sizeof(unsigned) * alist.size() : bytes[x];
         if (b > 0 && hipMemcpy(d[x], src[x], b, hipMemcpyHostToDevice) != hipSuccess)
@@ -324,11 +381,16 @@ int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations, cons
         float *muA = static_cast<float *>(d[0]), *muB = static_cast<float *>(d[1]);
         float *u = static_cast<float *>(d[2]);
         const unsigned *dfs = static_cast<unsigned *>(d[3]), *dfl = static_cast<unsigned *>(d[4]);
-        const unsigned *djs = static_cast<unsigned *>(d[5]), *djl = static_cast<unsigned *>(d[6]);
+        const unsigned *dss = static_cast<unsigned *>(d[5]), *djl = static_cast<unsigned *>(d[6]);
         const unsigned *das = static_cast<unsigned *>(d[7]), *dal = static_cast<unsigned *>(d[8]);
         const unsigned *dps = static_cast<unsigned *>(d[9]);
         const int *dpS = static_cast<int *>(d[10]);
         const float *dpT = static_cast<float *>(d[11]);
+        const unsigned *dsq = static_cast<unsigned *>(d[12]);
+        float *partial = static_cast<float *>(d[13]), *OTr = static_cast<float *>(d[14]);
+        const size_t mzn = (size_t)m * z * n;
+        hipLaunchKernelGGL(o_transpose, dim3((unsigned)((mzn + kB - 1) / kB)), dim3(kB), 0, st, n,
+                           m, z, pomdp->d_O, OTr);
         const size_t nm = (size_t)n * m;
         hipLaunchKernelGGL(zero_floats, dim3((unsigned)((nm + kB - 1) / kB)), dim3(kB), 0, st, nm, d_dR);
         hipLaunchKernelGGL(mu_init, dim3(r), dim3(kB), 0, st, n, pomdp->rz, pomdp->d_Z,
@@ -339,8 +401,13 @@ int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations, cons
             hipLaunchKernelGGL(dr_accumulate, ga, dim3(kB), 0, st, n, m,
                                das + (size_t)t * (m + 1), dal, muA, d_dR);
             hipLaunchKernelGGL(pred_gather, g, dim3(kB), 0, st, n, r, act, dps, dpS, dpT, muA, u);
-            hipLaunchKernelGGL(mu_step, g, dim3(kB), 0, st, n, z, pomdp->gamma, act,
-                               djs + (size_t)t * (r + 1), djl, pomdp->d_O, u, muA, muB);
+            const unsigned *ssT = &sstart[(size_t)t * (r + 1)];
+            const unsigned segBase = ssT[0], segs = ssT[r] - ssT[0];
+            if (segs > 0)
+                hipLaunchKernelGGL(mu_partial, dim3((n + kB - 1) / kB, segs), dim3(kB), 0, st, n,
+                                   z, pomdp->gamma, act, segBase, dsq, djl, OTr, u, partial);
+            hipLaunchKernelGGL(mu_combine, g, dim3(kB), 0, st, n, act,
+                               dss + (size_t)t * (r + 1), segBase, partial, muA, muB);
             std::swap(muA, muB);
         }
         e = hipStreamSynchronize(st);

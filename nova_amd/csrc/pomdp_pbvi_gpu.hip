@@ -787,12 +787,15 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
 
 // The 32-row matrix-core scoring with the whole pair list of (i, a) staged once and every alpha
 // tile of the instance swept by the same workgroup: W is built once per (belief, action) rather
-// than once per tile, and the workgroup writes the partials of every tile. For pair lists of at
-// most kWAll pairs (the tuner offers it then; longer lists use the kernel above).
+// than once per tile, and each lane keeps its running first maximiser per row across the tiles,
+// so the rows are reduced across lanes and waves once per sweep instead of once per tile (that
+// reduction cost 7% of the n = 20,000 scorer and 20-40% at 64-128 pairs). The sweep's result is
+// written as tile 0's partial, the other tiles' partials as empty. For pair lists of at most
+// kWAll pairs (the tuner offers it then; longer lists use the kernel above).
 constexpr int kWAll = 512;
 
 template <int PS, int NSET, bool BUF>
-__global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__restrict__ insts,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void pbvi_score_mfma_all(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter)
 {
     constexpr int NW = 8, NB = 2, NT = NW * 64;
@@ -838,6 +841,16 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
         __syncthreads();
         stage_w<32, NT, kWAll * 32>(w, spU, uPad, uEnd, zc, o0, z, pw, Oa, t);
         __syncthreads();
+        // per lane and row: the first maximiser over the lane's alphas of every tile so far
+        // (tiles in order, within a tile the lane's alphas in order, so strict > keeps the
+        // first); the rows are reduced across lanes and waves once, after the last tile
+        float runV[16];
+        int runJ[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            runV[v] = kLow;
+            runJ[v] = INT_MAX;
+        }
         for (unsigned tile = 0; tile < P.tiles; tile++) {
             const unsigned jw = tile * kMTile + wave * NB * 32;
             const unsigned jl = jw + lane % 32;
@@ -898,46 +911,52 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma_all(const PbviInst *__res
                     mfmaTrip(uPad - RING + j * HALF, bq[j]);
             }
 #pragma unroll
-            for (int v = 0; v < 16; v++) {
-                float bv = kLow;
-                int bj = INT_MAX;
+            for (int v = 0; v < 16; v++)
 #pragma unroll
                 for (int y = 0; y < NB; y++) {
                     const unsigned j = jw + y * 32 + lane % 32;
-                    if (j < r && acc[y][v] > bv) {
-                        bv = acc[y][v];
-                        bj = (int)j;
+                    if (j < r && acc[y][v] > runV[v]) {
+                        runV[v] = acc[y][v];
+                        runJ[v] = (int)j;
                     }
                 }
-                for (int off2 = 16; off2 >= 1; off2 >>= 1) {
-                    const float ov = __shfl_xor(bv, off2, 32);
-                    const int oj = __shfl_xor(bj, off2, 32);
-                    if (better(ov, oj, bv, bj)) {
-                        bv = ov;
-                        bj = oj;
-                    }
-                }
-                if (lane % 32 == 0) {
-                    const int o = (v % 4) + 4 * (int)(lane / 32) + 8 * (v / 4);
-                    redV[wave][o] = bv;
-                    redJ[wave][o] = bj;
-                }
-            }
-            __syncthreads();
-            if (t < (unsigned)zc) {
-                float v = redV[0][t];
-                int j = redJ[0][t];
-                for (int wv = 1; wv < NW; wv++)
-                    if (better(redV[wv][t], redJ[wv][t], v, j)) {
-                        v = redV[wv][t];
-                        j = redJ[wv][t];
-                    }
-                const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
-                P.partVal[idx] = v;
-                P.partJ[idx] = j;
-            }
-            __syncthreads();
         }
+        // per row (observation): first maximiser across the lanes, then across the waves; the
+        // whole sweep's result goes to tile 0's partial and the other tiles' partials get none
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            float bv = runV[v];
+            int bj = runJ[v];
+            for (int off2 = 16; off2 >= 1; off2 >>= 1) {
+                const float ov = __shfl_xor(bv, off2, 32);
+                const int oj = __shfl_xor(bj, off2, 32);
+                if (better(ov, oj, bv, bj)) {
+                    bv = ov;
+                    bj = oj;
+                }
+            }
+            if (lane % 32 == 0) {
+                const int o = (v % 4) + 4 * (int)(lane / 32) + 8 * (v / 4);
+                redV[wave][o] = bv;
+                redJ[wave][o] = bj;
+            }
+        }
+        __syncthreads();
+        if (t < (unsigned)zc) {
+            float v = redV[0][t];
+            int j = redJ[0][t];
+            for (int wv = 1; wv < NW; wv++)
+                if (better(redV[wv][t], redJ[wv][t], v, j)) {
+                    v = redV[wv][t];
+                    j = redJ[wv][t];
+                }
+            for (unsigned tile = 0; tile < P.tiles; tile++) {
+                const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
+                P.partVal[idx] = tile == 0 ? v : kLow;
+                P.partJ[idx] = tile == 0 ? j : INT_MAX;
+            }
+        }
+        __syncthreads();
     }
 }
 

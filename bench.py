@@ -296,7 +296,8 @@ def bench_pbvi(args):
                                    256: "pbvi_score_mfma<4 waves, ring of 4 operand sets>",
                                    512: "pbvi_score_mfma<8 waves, ring of 4 operand sets>",
                                    1024: "pbvi_score_mfma16 (ring of 4 operand sets)",
-                                   2048: "pbvi_score_mfma_all (ring of 4 operand sets)"}.get(
+                                   2048: "pbvi_score_mfma_all (ring of 4 operand sets)",
+                                   4096: "pbvi_score_mfma16_all (ring of 4 operand sets)"}.get(
                                        variant, "pbvi_score<fast,%d>" % variant),
                         "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": achieved / 157.3,

@@ -103,7 +103,8 @@ extern "C" int pomdp_pbvi_profile_gpu(const POMDP *pomdp, POMDPPBVIGPU *pbvi, fl
 // deeper prefetch per workgroup, 64 = its form with 16 observation rows (z <= 16), 128 = its
 // form that sweeps every alpha tile in one workgroup (rz * ns <= 512), 256, 512 = the 4- and
 // 8-wave kernels with operands requested three trips ahead (Gamma^T below 4 GB), 1024, 2048 =
-// the same for the 16-row and the all-tiles kernels, 0 = the dense-belief matrix-core GEMM.
+// the same for the 16-row and the all-tiles kernels, 4096 = the 16-row kernel sweeping every
+// alpha tile in one workgroup (z <= 16, rz * ns <= 512), 0 = the dense-belief matrix-core GEMM.
 extern "C" int pomdp_pbvi_scoring_variant_gpu(const POMDPPBVIGPU *pbvi, unsigned int *variant);
 
 // ---- PBVI argmax replay (gradients of L = sum_b max_alpha alpha . b with respect to R)

@@ -1123,6 +1123,172 @@ __global__ __launch_bounds__(512) void pbvi_score_mfma16(const PbviInst *__restr
     }
 }
 
+// The 16-row scoring with every alpha tile swept by one workgroup, as pbvi_score_mfma_all does
+// with 32 rows: the pair list (at most kWAll pairs) staged once per chunk of 16 observations,
+// the operands in the ring of register sets above, each lane's running first maximiser of its
+// four rows kept across the tiles and the rows reduced once per sweep, the result written as
+// tile 0's partial (variant 4096).
+template <int PS, int NSET, bool BUF>
+__global__ __launch_bounds__(512) void pbvi_score_mfma16_all(const PbviInst *__restrict__ insts,
+        const unsigned *__restrict__ off, unsigned count, unsigned iter)
+{
+    constexpr int NW = 8, NB = 4, NT = NW * 64;
+    static_assert(NW * NB * 16 == kMTile, "a workgroup sweeps whole alpha tiles");
+    static_assert(NSET >= 2, "at least one trip's operands in flight");
+    constexpr int HALF = 4 * PS;              // pairs per trip (PS MFMAs of four pairs)
+    constexpr int RING = NSET * HALF;         // pairs per turn of the ring of NSET register sets
+    static_assert(kWAll % RING == 0, "whole turns of the ring");
+    static_assert(NSET * PS * NB <= 63, "a turn of the ring fits the vector-memory counter");
+    __shared__ int spU[kWAll];
+    __shared__ float w[kWAll * 16];           // [u][o] for 16 observations
+    __shared__ float redV[NW][16];
+    __shared__ int redJ[NW][16];
+
+    if (blockIdx.x >= off[count])
+        return;
+    const unsigned b = find_inst(off, count, blockIdx.x);
+    const PbviInst &P = insts[b];
+    if (iter >= P.horizon)
+        return;
+    const unsigned n = P.n, m = P.m, z = P.z, r = P.r;
+    const unsigned local = blockIdx.x - off[b];
+    const unsigned rb = P.rb;
+    const unsigned q = local % (rb * m);
+    const unsigned ia = P.actionMajor ? (q % rb) * m + q / rb : q;   // belief * m + action
+    const unsigned a = ia % m;
+    const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
+
+    const int uEnd = __builtin_amdgcn_readfirstlane((int)P.pairCnt[ia]);
+    const int uPad = (uEnd + RING - 1) / RING * RING;             // <= kWAll
+    const NOVA_GLOBAL int *pSp = (const NOVA_GLOBAL int *)P.pairSp + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *pw = (const NOVA_GLOBAL float *)P.pairW + (size_t)ia * P.kmax;
+    const NOVA_GLOBAL float *Oa = (const NOVA_GLOBAL float *)P.O + (size_t)a * n * z;
+    const NOVA_GLOBAL float *GT = (const NOVA_GLOBAL float *)P.GT;
+    const size_t rS = P.rS;
+    const __amdgpu_buffer_rsrc_t rG = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>((const float *)P.GT), 0, -1, kRawBufferWord3);
+    for (int u = t; u < uPad; u += NT)
+        spU[u] = (u < uEnd) ? pSp[u] : 0;
+
+    for (unsigned o0 = 0; o0 < z; o0 += 16) {
+        const int zc = min(16, (int)(z - o0));
+        __syncthreads();
+        stage_w<16, NT, kWAll * 16>(w, spU, uPad, uEnd, zc, o0, z, pw, Oa, t);
+        __syncthreads();
+        float runV[4];
+        int runJ[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            runV[v] = kLow;
+            runJ[v] = INT_MAX;
+        }
+        for (unsigned tile = 0; tile < P.tiles; tile++) {
+            const unsigned jw = tile * kMTile + wave * NB * 16;
+            const unsigned jl = jw + lane % 16;
+            floatx4_t acc[NB];
+#pragma unroll
+            for (int y = 0; y < NB; y++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[y][v] = 0.0f;
+            if (uPad > 0 && jw < r) {   // waves whose alphas all lie past r skip
+                float bq[NSET][PS][NB];
+                auto loadB = [&](int u0, float (&dst)[PS][NB]) {
+#pragma unroll
+                    for (int qq = 0; qq < PS; qq++) {
+                        const int uu = u0 + 4 * qq + (int)(lane / 16);
+                        if constexpr (BUF) {
+                            // one 32-bit byte offset per row (GT is below 4 GB here)
+                            const unsigned o = ((unsigned)spU[uu] * (unsigned)rS + jl) * 4u;
+#pragma unroll
+                            for (int y = 0; y < NB; y++)
+                                dst[qq][y] = __uint_as_float(
+                                        __builtin_amdgcn_raw_buffer_load_b32(rG, o, y * 64, 0));
+                        } else {
+                            const NOVA_GLOBAL float *row = GT + (size_t)spU[uu] * rS + jl;
+#pragma unroll
+                            for (int y = 0; y < NB; y++)
+                                dst[qq][y] = row[y * 16];
+                        }
+                    }
+                };
+                auto mfmaTrip = [&](int u0, const float (&bb)[PS][NB]) {
+                    float fa[PS];
+#pragma unroll
+                    for (int qq = 0; qq < PS; qq++)
+                        fa[qq] = w[(u0 + 4 * qq + (int)(lane / 16)) * 16 + lane % 16];
+#pragma unroll
+                    for (int qq = 0; qq < PS; qq++)
+#pragma unroll
+                        for (int y = 0; y < NB; y++)
+                            acc[y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[qq], bb[qq][y],
+                                                                          acc[y], 0, 0, 0);
+                };
+#pragma unroll
+                for (int s0 = 0; s0 < NSET; s0++)
+                    loadB(s0 * HALF, bq[s0]);
+                for (int u0 = 0; u0 < uPad - RING; u0 += RING) {
+#pragma unroll
+                    for (int j = 0; j < NSET; j++) {
+                        __builtin_amdgcn_sched_barrier(0);
+                        mfmaTrip(u0 + j * HALF, bq[j]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        loadB(u0 + (j + NSET) * HALF, bq[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NSET; j++)      // the last turn requests nothing more
+                    mfmaTrip(uPad - RING + j * HALF, bq[j]);
+            }
+            // lane's alphas in increasing order across the tiles: strict > keeps the first
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+#pragma unroll
+                for (int y = 0; y < NB; y++) {
+                    const unsigned j = jw + y * 16 + lane % 16;
+                    if (j < r && acc[y][v] > runV[v]) {
+                        runV[v] = acc[y][v];
+                        runJ[v] = (int)j;
+                    }
+                }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float bv = runV[v];
+            int bj = runJ[v];
+            for (int off2 = 8; off2 >= 1; off2 >>= 1) {
+                const float ov = __shfl_xor(bv, off2, 16);
+                const int oj = __shfl_xor(bj, off2, 16);
+                if (better(ov, oj, bv, bj)) {
+                    bv = ov;
+                    bj = oj;
+                }
+            }
+            if (lane % 16 == 0) {
+                const int o = 4 * (int)(lane / 16) + v;
+                redV[wave][o] = bv;
+                redJ[wave][o] = bj;
+            }
+        }
+        __syncthreads();
+        if (t < (unsigned)zc) {
+            float v = redV[0][t];
+            int j = redJ[0][t];
+            for (int wv = 1; wv < NW; wv++)
+                if (better(redV[wv][t], redJ[wv][t], v, j)) {
+                    v = redV[wv][t];
+                    j = redJ[wv][t];
+                }
+            for (unsigned tile = 0; tile < P.tiles; tile++) {
+                const size_t idx = ((size_t)tile * rb * m + ia) * z + o0 + t;
+                P.partVal[idx] = tile == 0 ? v : kLow;
+                P.partJ[idx] = tile == 0 ? j : INT_MAX;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // -------------------------------------------------------------------------- 3. action
 
 // EXACT: each action's alpha is formed on the belief's support in the reference's rounding order
@@ -1915,9 +2081,9 @@ static bool fused_fits(const PbviInst &I, int mode)
 // A launch plan over a set of instances: device descriptors and per-stage offsets.
 struct DenseWs;
 
-// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32, 64, 128); PbviPlan::totSc and the
-// offset table hold one entry per variant.
-constexpr int kVariants = 12;
+// Scoring variants a plan launches (jpt 1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048,
+// 4096); PbviPlan::totSc and the offset table hold one entry per variant.
+constexpr int kVariants = 13;
 
 struct PbviPlan {
     const DenseWs *dense = nullptr;  // dense-belief workspace (plain solves in modes 2/3)
@@ -1939,7 +2105,7 @@ static int jpt_slot(unsigned jpt)
 {
     return jpt == 1 ? 0 : (jpt == 2 ? 1 : (jpt == 4 ? 2 : (jpt == 8 ? 3 : (jpt == 16 ? 4 :
            (jpt == 32 ? 5 : (jpt == 64 ? 6 : (jpt == 128 ? 7 : (jpt == 256 ? 8 :
-           (jpt == 512 ? 9 : (jpt == 1024 ? 10 : 11))))))))));
+           (jpt == 512 ? 9 : (jpt == 1024 ? 10 : (jpt == 2048 ? 11 : 12)))))))))));
 }
 
 // Launches the scoring kernel of variant jpt over grid g (offsets sc of that variant).
@@ -1963,6 +2129,7 @@ static void launch_score(unsigned jpt, int mode, dim3 g, const PbviInst *dInst,
         if (jpt == 1024) hipLaunchKernelGGL((pbvi_score_mfma16<2, 4, true>), g, dim3(512), 0, st, dInst, sc, c, iter);
         if (jpt == 128) hipLaunchKernelGGL((pbvi_score_mfma_all<4, 2, false>), g, dim3(512), 0, st, dInst, sc, c, iter);
         if (jpt == 2048) hipLaunchKernelGGL((pbvi_score_mfma_all<4, 4, true>), g, dim3(512), 0, st, dInst, sc, c, iter);
+        if (jpt == 4096) hipLaunchKernelGGL((pbvi_score_mfma16_all<2, 4, true>), g, dim3(512), 0, st, dInst, sc, c, iter);
     }
 }
 
@@ -1994,7 +2161,8 @@ static int plan_upload(PbviPlan &pl, void *dst, hipStream_t st, const char *fn)
         for (int v = 0; v < kVariants; v++)
             scs[v][b + 1] = scs[v][b] +
                             ((four && jpt_slot(I.jpt) == v)
-                             ? I.rb * I.m * ((I.jpt == 128 || I.jpt == 2048) ? 1u : I.tiles)
+                             ? I.rb * I.m * ((I.jpt == 128 || I.jpt == 2048 || I.jpt == 4096)
+                                             ? 1u : I.tiles)
                              : 0u);
         be[b + 1] = be[b] + (four ? I.rb : 0u);
         fu[b + 1] = fu[b] + (four ? 0u : I.rb);
@@ -2154,7 +2322,8 @@ static void plan_enqueue(const PbviPlan &pl, unsigned iter, unsigned flip, int m
     if (pl.totBe == 0)
         return;
     hipLaunchKernelGGL(pbvi_transpose, dim3(pl.totTr), dim3(kPB), 0, st, pl.dInst, tr, c, iter, flip);
-    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048};
+    static const unsigned kJpt[kVariants] = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048,
+                                             4096};
     for (int v = 0; v < kVariants; v++) {
         if (pl.totSc[v] == 0)
             continue;
@@ -2208,8 +2377,9 @@ static int plan_graph(const PbviPlan &pl, unsigned iters, unsigned flip, int mod
 // waves; 8 waves; 8 waves with twice the pairs per trip), 256 and 512 (the 4- and 8-wave kernels
 // with operands requested three trips ahead, 32 loads in flight per wave), 64 (16 observation
 // rows, z <= 16), 1024 (the 16-row kernel with operands requested three trips ahead), 128
-// (every alpha tile in one workgroup, pair lists of at most 512 pairs) and 2048 (that one with
-// operands requested three trips ahead);
+// (every alpha tile in one workgroup, pair lists of at most 512 pairs), 2048 (that one with
+// operands requested three trips ahead) and 4096 (the 16-row kernel likewise sweeping every
+// tile, z <= 16, pair lists of at most 512 pairs);
 // NOVA_AMD_AUTOTUNE=0 disables timing (default variant 2).
 
 struct TuneKey {
@@ -2249,7 +2419,9 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
                 (v == 1024 && mode != 1 && I.z <= 16 && gt_below_4g(I)) ||
                 (v == 2048 && mode != 1 && I.kmax <= (unsigned)kWAll && gt_below_4g(I)) ||
                 (v == 64 && mode != 1 && I.z <= 16) ||
-                (v == 128 && mode != 1 && I.kmax <= (unsigned)kWAll))
+                (v == 128 && mode != 1 && I.kmax <= (unsigned)kWAll) ||
+                (v == 4096 && mode != 1 && I.z <= 16 && I.kmax <= (unsigned)kWAll &&
+                 gt_below_4g(I)))
             return (unsigned)v;
     }
     if (const char *f = std::getenv("NOVA_AMD_AUTOTUNE"))
@@ -2264,14 +2436,15 @@ static unsigned tune_jpt(PbviInst I, void *planMem, hipStream_t st, int mode)
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u, 128u, 256u, 512u, 1024u, 2048u}) {
+    for (unsigned jpt : {1u, 2u, 4u, 8u, 16u, 32u, 64u, 128u, 256u, 512u, 1024u, 2048u, 4096u}) {
         if (jpt >= 8 && mode == 1)
             continue;                          // the matrix-core kernel is fast mode only
-        if ((jpt == 64 || jpt == 1024) && I.z > 16)
+        if ((jpt == 64 || jpt == 1024 || jpt == 4096) && I.z > 16)
             continue;                          // the 16-row kernels: at most 16 observations
-        if ((jpt == 128 || jpt == 2048) && I.kmax > (unsigned)kWAll)
-            continue;                          // the all-tiles kernel stages every pair once
-        if ((jpt == 256 || jpt == 512 || jpt == 1024 || jpt == 2048) && !gt_below_4g(I))
+        if ((jpt == 128 || jpt == 2048 || jpt == 4096) && I.kmax > (unsigned)kWAll)
+            continue;                          // the all-tiles kernels stage every pair once
+        if ((jpt == 256 || jpt == 512 || jpt == 1024 || jpt == 2048 || jpt == 4096) &&
+                !gt_below_4g(I))
             continue;                          // their buffer loads take 32-bit byte offsets
         if (jpt < 8 && !valuOk)
             continue;

@@ -102,8 +102,8 @@ class POMDPPBVIGPU(_abi.NovaPOMDPPBVIGPU):
         8 waves, 8 waves with a deeper prefetch); 64: fp32 matrix cores with 16 observation
         rows (z <= 16); 128: fp32 matrix cores, every alpha tile in one workgroup (rz * ns <=
         512); 256, 512: the 4- and 8-wave matrix-core kernels with operands requested three
-        trips ahead; 1024, 2048: the same for the 16-row and the all-tiles kernels; 0: dense
-        GEMM."""
+        trips ahead; 1024, 2048: the same for the 16-row and the all-tiles kernels; 4096: the
+        16-row kernel sweeping every alpha tile in one workgroup; 0: dense GEMM."""
         v = ct.c_uint(0)
         if self._lib.pomdp_pbvi_scoring_variant_gpu(self, ct.byref(v)) != 0:
             raise Exception("pomdp_pbvi_scoring_variant_gpu failed")

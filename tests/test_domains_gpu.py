@@ -110,7 +110,8 @@ def test_exact_mode_long_supports_bitwise(oracle, name, k, horizon):
           % (name, horizon, P["Z"].shape[1], P["S"].shape[2]), flush=True)
 
 
-@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64", "128", "256", "512", "1024", "2048"])
+@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64", "128", "256", "512", "1024", "2048",
+                                     "4096"])
 @pytest.mark.parametrize("name,k,horizon", LONG + SMALL[4:6],
                          ids=[d[0] for d in LONG + SMALL[4:6]])
 def test_fast_step_on_domain(oracle, monkeypatch, name, k, horizon, variant):

@@ -138,7 +138,8 @@ def sparse_values(P, G):
     return V
 
 
-@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8", "16", "32", "64", "128", "256", "512", "1024", "2048"])
+@pytest.mark.parametrize("variant", ["tuned", "1", "4", "8", "16", "32", "64", "128", "256", "512", "1024", "2048",
+                                     "4096"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES, ids=[s[0] for s in SHAPES])
 def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, variant):
     """From the same Gamma (the oracle's after t iterations), one fast GPU iteration chooses
@@ -147,8 +148,9 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
     waves, 8 waves, 8 waves and a deeper prefetch; 64 = its 16-observation-row form, taken
     only when z <= 16; 128 = its form that sweeps every alpha tile in one workgroup, taken when
     rz * ns <= 512; 256 and 512 = the 4- and 8-wave kernels with a ring of four operand sets
-    and buffer loads, 1024 and 2048 = the same for the 16-row and the all-tiles kernels;
-    otherwise the tuned variant runs). A forced variant also turns off the
+    and buffer loads, 1024 and 2048 = the same for the 16-row and the all-tiles kernels, 4096 =
+    the 16-row kernel sweeping every tile, taken when z <= 16 and rz * ns <= 512; otherwise
+    the tuned variant runs). A forced variant also turns off the
     one-launch path of small instances, which would otherwise run instead of it."""
     if variant != "tuned":
         monkeypatch.setenv("NOVA_AMD_PBVI_JPT", variant)
@@ -185,7 +187,8 @@ def test_fast_step_vs_oracle(oracle, monkeypatch, name, params, horizon, rag, va
           % (name, variant, mismatch, total))
 
 
-@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64", "128", "256", "512", "1024", "2048"])
+@pytest.mark.parametrize("variant", ["tuned", "8", "16", "32", "64", "128", "256", "512", "1024", "2048",
+                                     "4096"])
 @pytest.mark.parametrize("name,params,horizon,rag", SHAPES[:2], ids=[s[0] for s in SHAPES[:2]])
 def test_fast_run_values_vs_fp64(oracle, monkeypatch, name, params, horizon, rag, variant):
     if variant != "tuned":
@@ -318,11 +321,13 @@ def test_exact_long_supports_bitwise_vs_oracle(oracle, n, m, z, ns, r, rz, horiz
     (150, 2, 5, 8, 90, 40, 72),       # 320 pairs: two staged W chunks
     (40, 4, 1, 3, 513, 4, 73),        # one observation, 513 alphas
 ])
-def test_fast_16_row_matrix_core_vs_oracle(oracle, monkeypatch, n, m, z, ns, r, rz, seed):
-    """The 16-observation-row matrix-core scorer (v_mfma_f32_16x16x4_f32), forced: from a
+@pytest.mark.parametrize("variant", [64, 4096])
+def test_fast_16_row_matrix_core_vs_oracle(oracle, monkeypatch, n, m, z, ns, r, rz, seed, variant):
+    """The 16-observation-row matrix-core scorers (v_mfma_f32_16x16x4_f32), forced: the per-tile
+    kernel (64) and the one sweeping every tile with per-lane running maxima (4096). From a
     perturbed Gamma one fast iteration chooses the oracle's alphas except at near-ties, on
     ragged lists and supports, and the new alphas of matching beliefs agree within 1e-5."""
-    monkeypatch.setenv("NOVA_AMD_PBVI_JPT", "64")
+    monkeypatch.setenv("NOVA_AMD_PBVI_JPT", str(variant))
     monkeypatch.setenv("NOVA_AMD_PBVI_FUSED", "0")
     P = ragged(synth.pomdp_arrays(n=n, m=m, z=z, ns=ns, r=r, rz=rz, seed=seed, gamma=0.95),
                seed, frac=0.2)
@@ -333,7 +338,7 @@ def test_fast_16_row_matrix_core_vs_oracle(oracle, monkeypatch, n, m, z, ns, r, 
     p = POMDP.from_dict(P)
     p.initialize_gpu()
     s = POMDPPBVIGPU(p, GammaInitial=Gt)
-    assert s.scoring_variant() == 64
+    assert s.scoring_variant() == variant
     s.update()
     js, act = s.argmax_record()
     G1 = s.get_policy().Gamma_array()

@@ -23,9 +23,10 @@ extern "C" int mdp_vi_run_gpu(const MDP *mdp, MDPVIGPU *vi, const float *d_V0,
 // ---- differentiable value iteration
 
 // Which sweep kernel a solver runs: 0 = one thread per state, 1 = the wave-tiled sweep,
-// 2 = the successor-sorted tiles that mdp_vi_initialize_gpu lays out for models of at least
-// 2^24 (state, action, successor) slots with 8 <= ns <= 255 (NOVA_AMD_VI_KERNEL=tiled|sorted
-// overrides the size rule). All three give the same V and pi bit for bit.
+// 2 = the successor-sorted tiles, 3 = the successor-chunked passes (V chunks in LDS, products
+// through HBM) that mdp_vi_initialize_gpu lays out for models of at least 2^24 (state, action,
+// successor) slots with ns <= 255 (NOVA_AMD_VI_KERNEL=tiled|sorted|chunked overrides the size
+// rule where the shape allows). All of them give the same V and pi bit for bit.
 extern "C" int mdp_vi_sweep_kernel_gpu(const MDP *mdp, const MDPVIGPU *vi, unsigned int *kernel);
 
 // Batched value iteration (no counterpart in the reference): solves count MDPs (each resident

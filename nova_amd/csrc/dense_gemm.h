@@ -309,29 +309,35 @@ __global__ __launch_bounds__(kGConvThreads) void dense_convert_blk(unsigned n, u
     }
 }
 
-// B_q = bf16(float(GB) * s_{p0 + q}) for the pairs q = blockIdx.y of a chunk, all k-blocked
+// B_q = bf16(float(GB) * s_{p0 + q}) for the cnt pairs q of a chunk, all k-blocked
 // ([kpad / 32][rowsPad][32] per pair): the rounding dense_score<bf16> applies in registers.
-// Each lane forms 8 consecutive elements (16 bytes, one k-chunk of one row).
+// Each lane reads 8 consecutive elements of GB (16 bytes, one k-chunk of one row) once and
+// writes them scaled for every pair of the chunk, so GB is read once per chunk, not per pair.
 __global__ __launch_bounds__(kGConvThreads) void dense_scale_pairs(unsigned kpad, size_t rowsPad,
-        unsigned p0, const __bf16 *__restrict__ GB, const float *__restrict__ Os,
+        unsigned p0, unsigned cnt, const __bf16 *__restrict__ GB, const float *__restrict__ Os,
         __bf16 *__restrict__ BP)
 {
-    const size_t per8 = rowsPad * kpad / 8, q = blockIdx.y;
-    const float *s = Os + (p0 + q) * (size_t)kpad;
-    __bf16 *out = BP + q * rowsPad * kpad;
+    const size_t per = rowsPad * kpad, per8 = per / 8;
     for (size_t c = blockIdx.x * (size_t)kGConvThreads + threadIdx.x; c < per8;
          c += (size_t)gridDim.x * kGConvThreads) {
         // chunk c: elements 8c .. 8c + 7 of [kb][row][32]; k = kb * 32 + (8c % 32) + e
         const size_t k0 = (c / (rowsPad * 4)) * kGBlk + (c % 4) * 8;
         const bf16x8_t g = *reinterpret_cast<const bf16x8_t *>(GB + 8 * c);
-        const float4 s0 = *reinterpret_cast<const float4 *>(s + k0);
-        const float4 s1 = *reinterpret_cast<const float4 *>(s + k0 + 4);
-        const float sv[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-        bf16x8_t o;
+        float gf[8];
 #pragma unroll
         for (int e = 0; e < 8; e++)
-            o[e] = (__bf16)((float)g[e] * sv[e]);
-        *reinterpret_cast<bf16x8_t *>(out + 8 * c) = o;
+            gf[e] = (float)g[e];
+        for (unsigned q = 0; q < cnt; q++) {
+            const float *s = Os + (p0 + q) * (size_t)kpad + k0;
+            const float4 s0 = *reinterpret_cast<const float4 *>(s);
+            const float4 s1 = *reinterpret_cast<const float4 *>(s + 4);
+            const float sv[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+            bf16x8_t o;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                o[e] = (__bf16)(gf[e] * sv[e]);
+            *reinterpret_cast<bf16x8_t *>(BP + q * per + 8 * c) = o;
+        }
     }
 }
 

@@ -1927,7 +1927,12 @@ static bool pbvi_shape_ok(const nova::POMDP *p)
 // the running action value (fast mode).
 static size_t action_lds(const PbviInst &I, int mode)
 {
-    return mode == 1 ? 2 * sizeof(float) * std::max<size_t>(I.rz, kKMax) : 2 * sizeof(float);
+    // exact mode runs only supports of at most kActMax entries (pbvi_shape_ok): a longer one
+    // (dense beliefs) is never run by the exact kernel, and sizing for it would pass the LDS
+    // a workgroup can have beside the kernel's static arrays
+    return mode == 1 ? 2 * sizeof(float) * std::min<size_t>(std::max<size_t>(I.rz, kKMax),
+                                                            kActMax)
+                     : 2 * sizeof(float);
 }
 
 // Whether the fast-mode pair lists (r*m*rz*ns entries of 8 bytes) are built: always, unless they
@@ -2264,9 +2269,9 @@ static void dense_launch_score(const DenseWs &D, const float *G, hipStream_t st,
         for (unsigned p0 = 0; p0 < pairs; p0 += D.chunk) {
             const unsigned cnt = std::min(D.chunk, pairs - p0);
             const unsigned cs = (unsigned)std::min<size_t>(
-                (per / 8 + kGConvThreads - 1) / kGConvThreads, 4096);
-            hipLaunchKernelGGL(dense_scale_pairs, dim3(cs, cnt), dim3(kGConvThreads), 0, st,
-                               D.kpad, rowsPad, p0, static_cast<const __bf16 *>(D.GI), D.Os,
+                (per / 8 + kGConvThreads - 1) / kGConvThreads, 8192);
+            hipLaunchKernelGGL(dense_scale_pairs, dim3(cs), dim3(kGConvThreads), 0, st, D.kpad,
+                               rowsPad, p0, cnt, static_cast<const __bf16 *>(D.GI), D.Os,
                                static_cast<__bf16 *>(D.BP));
             hipLaunchKernelGGL(dense_score_bf16, dim3(D.tiles * D.tiles, cnt), dim3(kGThreads),
                                kGDlLdsBytes, st, D.r, D.r, D.kpad, D.m, D.z, D.tiles, D.tiles, p0,

@@ -161,6 +161,7 @@ constexpr int kUB = 1024;
 constexpr int kUpdWaves = kUB / 64;
 constexpr int kUpdScratch = 1024;                // product slots per wave
 constexpr int kUpdLdsFloats = 40 * 1024 - kUpdWaves * kUpdScratch;   // 160 KB in all
+constexpr int kUpdBatch = 8;                     // list entries per lane requested together
 
 // IDX: the predecessor's index type, 16 bits for n <= 65,536 (6 bytes per list entry, not 8)
 template <typename IDX>
@@ -198,8 +199,26 @@ __global__ __launch_bounds__(kUB) void belief_update_batch(unsigned n, unsigned 
         float u = 0.0f;
         for (unsigned xs = xa; xs < xb; xs += kUpdScratch) {
             const unsigned xe = min(xs + (unsigned)kUpdScratch, xb);
-            for (unsigned x = xs + lane; x < xe; x += 64)
-                scr[x - xs] = __fmul_rn(pT[x], src[pS[x]]);
+            // the lane's entries of the chunk kUpdBatch at a time, their predecessors and
+            // factors requested together (indices clamped into the chunk, the extra products
+            // not stored)
+            for (unsigned x0 = xs + lane; x0 < xe; x0 += 64 * kUpdBatch) {
+                IDX si[kUpdBatch];
+                float tv[kUpdBatch];
+#pragma unroll
+                for (int k = 0; k < kUpdBatch; k++) {
+                    const unsigned x = min(x0 + 64u * k, xe - 1);
+                    si[k] = pS[x];
+                    tv[k] = pT[x];
+                }
+#pragma unroll
+                for (int k = 0; k < kUpdBatch; k++) {
+                    const unsigned x = x0 + 64u * k;
+                    const float pr = __fmul_rn(tv[k], src[si[k]]);
+                    if (x < xe)
+                        scr[x - xs] = pr;
+                }
+            }
             // the wave's own LDS writes, then reads by other lanes of the wave
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

@@ -60,16 +60,32 @@ __global__ __launch_bounds__(kVThreads) void alpha_value_exact(unsigned k, unsig
 #pragma unroll
         for (int y = 0; y < 4; y++)
             acc[x][y] = 0.0f;
-    for (unsigned s0 = 0; s0 < n; s0 += kVS) {
-        // stage 64 rows x 32 states of each operand, transposed to state-major; entries past n
-        // (and rows past k or r) are zeros, whose products add +0 and leave every sum as is
-        for (unsigned x = t; x < kVT * kVS; x += kVThreads) {
-            const unsigned row = x / kVS, s = x % kVS;
+    // stage 64 rows x 32 states of each operand, transposed to state-major; entries past n
+    // (and rows past k or r) are zeros, whose products add +0 and leave every sum as is. The
+    // next step's values are requested into registers before the current step's products,
+    // so their latency hides behind the arithmetic.
+    constexpr int kPer = kVT * kVS / kVThreads;      // values per lane and operand per step
+    float bpre[kPer], gpre[kPer];
+    auto loadStep = [&](unsigned s0) {
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const unsigned x = t + u * kVThreads, row = x / kVS, s = x % kVS;
             const bool inS = s0 + s < n;
-            bs[s][row] = (inS && i0 + row < k) ? B[(size_t)(i0 + row) * n + s0 + s] : 0.0f;
-            gs[s][row] = (inS && j0 + row < r) ? G[(size_t)(j0 + row) * n + s0 + s] : 0.0f;
+            bpre[u] = (inS && i0 + row < k) ? B[(size_t)(i0 + row) * n + s0 + s] : 0.0f;
+            gpre[u] = (inS && j0 + row < r) ? G[(size_t)(j0 + row) * n + s0 + s] : 0.0f;
+        }
+    };
+    loadStep(0);
+    for (unsigned s0 = 0; s0 < n; s0 += kVS) {
+#pragma unroll
+        for (int u = 0; u < kPer; u++) {
+            const unsigned x = t + u * kVThreads, row = x / kVS, s = x % kVS;
+            bs[s][row] = bpre[u];
+            gs[s][row] = gpre[u];
         }
         __syncthreads();
+        if (s0 + kVS < n)
+            loadStep(s0 + kVS);
 #pragma unroll 4
         for (int s = 0; s < kVS; s++) {
             const float4 bv = *reinterpret_cast<const float4 *>(&bs[s][bi * 4]);

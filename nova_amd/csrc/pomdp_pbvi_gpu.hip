@@ -1633,6 +1633,22 @@ constexpr int kFZ = 256;           // max observations
 constexpr int kFS = 6144;          // max observations x alphas: the scores in LDS
 constexpr int kFStage = 24576;     // max words of Gamma, O*T, S, R and the belief in LDS
 
+// dst[x] = get(x) for x < count, eight values per lane requested before any is stored
+template <typename T, typename F>
+__device__ __forceinline__ void fused_stage_copy(T *dst, F get, unsigned count, unsigned t)
+{
+    for (unsigned x0 = t; x0 < count; x0 += 8 * kPB) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            v[u] = get(min(x0 + u * kPB, count - 1));
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (x0 + u * kPB < count)
+                dst[x0 + u * kPB] = v[u];
+    }
+}
+
 __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
 {
@@ -1662,17 +1678,16 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
     int *Zi = reinterpret_cast<int *>(Rall + (size_t)n * m);
     float *Bi = reinterpret_cast<float *>(Zi + rz);
     {
+        // eight loads per lane requested before any is stored (one instance's staging is the
+        // kernel's latency at these sizes)
         const float *__restrict__ Gsrc = src_gamma(P, iter, flip);
-        for (unsigned x = t; x < r * n; x += kPB)
-            G[x] = Gsrc[x];
-        for (unsigned x = t; x < m * zn; x += kPB)
-            OTall[x] = P.OT[x];
-        for (unsigned x = t; x < m * sn; x += kPB) {
+        fused_stage_copy(G, [&](unsigned x) { return Gsrc[x]; }, r * n, t);
+        fused_stage_copy(OTall, [&](unsigned x) { return P.OT[x]; }, m * zn, t);
+        fused_stage_copy(Sall, [&](unsigned x) {
             const unsigned a = x / sn, e = x % sn;
-            Sall[x] = P.S[((size_t)(e / ns) * m + a) * ns + e % ns];
-        }
-        for (unsigned x = t; x < n * m; x += kPB)
-            Rall[x] = P.R[x];
+            return P.S[((size_t)(e / ns) * m + a) * ns + e % ns];
+        }, m * sn, t);
+        fused_stage_copy(Rall, [&](unsigned x) { return P.R[x]; }, n * m, t);
         for (unsigned k = t; k < rz; k += kPB) {
             Zi[k] = P.Z[(size_t)i * rz + k];
             Bi[k] = P.B[(size_t)i * rz + k];
@@ -1703,18 +1718,32 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
             sc[x] = value;
         }
         __syncthreads();
-        // first maximiser per observation by strict '>' from -1e10 (none: alpha 0)
-        for (unsigned o = t; o < z; o += kPB) {
+        // first maximiser per observation by strict '>' from -1e10 (none: alpha 0): 32 lanes
+        // per observation scan the alphas j = l, l + 32, ... in order, then merge by (value,
+        // lower j)
+        for (unsigned o0 = 0; o0 < z; o0 += kPB / 32) {
+            const unsigned o = o0 + t / 32, l = t % 32;
             float best = kLow;
-            int bj = 0;
-            for (unsigned j = 0; j < r; j++) {
-                if (sc[o * r + j] > best) {
-                    best = sc[o * r + j];
-                    bj = (int)j;
+            int bj = INT_MAX;
+            if (o < z)
+                for (unsigned j = l; j < r; j += 32)
+                    if (sc[o * r + j] > best) {
+                        best = sc[o * r + j];
+                        bj = (int)j;
+                    }
+            for (int off2 = 16; off2 >= 1; off2 >>= 1) {
+                const float ov = __shfl_xor(best, off2, 32);
+                const int oj = __shfl_xor(bj, off2, 32);
+                if (better(ov, oj, best, bj)) {
+                    best = ov;
+                    bj = oj;
                 }
             }
-            js[o] = bj;
-            P.jstar[((size_t)i * m + a) * z + o] = bj;
+            if (o < z && l == 0) {
+                const int jj = (bj == INT_MAX) ? 0 : bj;
+                js[o] = jj;
+                P.jstar[((size_t)i * m + a) * z + o] = jj;
+            }
         }
         __syncthreads();
         // alpha(s) = R[s,a] + sum_o (gamma * sum_l (O*T) * Gamma[j*_o, sp]), o ascending

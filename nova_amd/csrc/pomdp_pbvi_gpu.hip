@@ -1671,8 +1671,11 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
     // one round of loads stages everything the iteration reads: Gamma [r][n], O*T [m][z][n][ns],
     // S [m][n][ns] (action-major), R [n][m] and belief i's (Z, B)
     const unsigned rz = P.rz, zn = z * n * ns, sn = n * ns;
+    // Gamma rows at an odd stride: the scoring lanes read one state of consecutive alphas, which
+    // a stride of a multiple of the bank count would put in one LDS bank
+    const unsigned gs = n | 1u;
     float *G = stage;
-    float *OTall = G + (size_t)r * n;
+    float *OTall = G + (size_t)r * gs;
     int *Sall = reinterpret_cast<int *>(OTall + (size_t)m * zn);
     float *Rall = reinterpret_cast<float *>(Sall + (size_t)m * sn);
     int *Zi = reinterpret_cast<int *>(Rall + (size_t)n * m);
@@ -1681,7 +1684,18 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
         // eight loads per lane requested before any is stored (one instance's staging is the
         // kernel's latency at these sizes)
         const float *__restrict__ Gsrc = src_gamma(P, iter, flip);
-        fused_stage_copy(G, [&](unsigned x) { return Gsrc[x]; }, r * n, t);
+        for (unsigned x0 = t; x0 < r * n; x0 += 8 * kPB) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                v[u] = Gsrc[min(x0 + u * kPB, r * n - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const unsigned x = x0 + u * kPB;
+                if (x < r * n)
+                    G[(x / n) * gs + x % n] = v[u];
+            }
+        }
         fused_stage_copy(OTall, [&](unsigned x) { return P.OT[x]; }, m * zn, t);
         fused_stage_copy(Sall, [&](unsigned x) {
             const unsigned a = x / sn, e = x % sn;
@@ -1707,7 +1721,7 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
         for (unsigned x = t; x < z * r; x += kPB) {
             const unsigned o = x / r, j = x % r;
             const float *ot = ota + (size_t)o * n * ns;
-            const float *Gj = G + (size_t)j * n;
+            const float *Gj = G + (size_t)j * gs;
             float value = 0.0f;
             for (int k = 0; k < kEnd; k++) {
                 float vt = 0.0f;
@@ -1751,7 +1765,7 @@ __global__ __launch_bounds__(kPB) void pbvi_fused(const PbviInst *__restrict__ i
             const int *rw = Sa + (size_t)s * ns;
             float al = Rall[(size_t)s * m + a];
             for (unsigned o = 0; o < z; o++) {
-                const float *Gj = G + (size_t)js[o] * n;
+                const float *Gj = G + (size_t)js[o] * gs;
                 const float *ot = ota + ((size_t)o * n + s) * ns;
                 float vt = 0.0f;
                 for (unsigned l = 0; l < ns; l++) {
@@ -2083,8 +2097,8 @@ static void fill_inst(const nova::POMDP *p, PbviInst &I, float *G0, float *G1, u
 // Words of LDS that pbvi_fused stages for an instance: Gamma, O*T, S, R and one belief.
 static double fused_stage_words(const PbviInst &I)
 {
-    return (double)I.r * I.n + (double)I.m * (I.z + 1.0) * I.n * I.ns + (double)I.n * I.m +
-           2.0 * I.rz;
+    return (double)I.r * (I.n | 1u) + (double)I.m * (I.z + 1.0) * I.n * I.ns +
+           (double)I.n * I.m + 2.0 * I.rz;
 }
 
 // Whether a PBVI instance runs its iterations as single pbvi_fused launches: sparse modes, the

@@ -1520,6 +1520,46 @@ __device__ __forceinline__ unsigned materialize_exact_states(const int *sa, cons
     return s0;
 }
 
+// Fast materialization of the lane's states t, t + kMB, ..., kMatStates of them at a time with
+// all their NS successors and factors requested together: alpha(s) = R + gamma * sum_l T h(S),
+// each sum the same FMA chain, in the same order, as the general loop (entries after the first
+// terminator take no part; its index is clamped to 0 for the unused read). Returns the first
+// state it left to that loop.
+template <int NS>
+__device__ __forceinline__ unsigned materialize_fast_states(const NOVA_GLOBAL int *Sa,
+        const NOVA_GLOBAL float *Ta, const NOVA_GLOBAL float *Ra, const float *row,
+        NOVA_GLOBAL float *out, float gamma, unsigned n, unsigned t)
+{
+    unsigned s0 = t;
+    for (; s0 + (kMatStates - 1) * kMB < n; s0 += kMatStates * kMB) {
+        int spv[kMatStates][NS];
+        float tv[kMatStates][NS];
+#pragma unroll
+        for (int q = 0; q < kMatStates; q++) {
+            const size_t rw = (size_t)(s0 + q * kMB) * NS;
+#pragma unroll
+            for (int l = 0; l < NS; l++) {
+                spv[q][l] = Sa[rw + l];
+                tv[q][l] = Ta[rw + l];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMatStates; q++) {
+            float acc = 0.0f;
+            bool stop = false;
+#pragma unroll
+            for (int l = 0; l < NS; l++) {
+                stop = stop || spv[q][l] < 0;
+                const float h = row[max(spv[q][l], 0)];
+                acc = stop ? acc : __builtin_fmaf(tv[q][l], h, acc);
+            }
+            const unsigned s = s0 + q * kMB;
+            out[s] = __builtin_fmaf(gamma, acc, Ra[s]);
+        }
+    }
+    return s0;
+}
+
 template <bool EXACT>
 __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restrict__ insts,
         const unsigned *__restrict__ off, unsigned count, unsigned iter, unsigned flip)
@@ -1603,7 +1643,12 @@ __global__ __launch_bounds__(kMB) void pbvi_materialize(const PbviInst *__restri
         const NOVA_GLOBAL int *Sa = (const NOVA_GLOBAL int *)P.SA + (size_t)a * n * ns;
         const NOVA_GLOBAL float *Ta = (const NOVA_GLOBAL float *)P.TA + (size_t)a * n * ns;
         const NOVA_GLOBAL float *Ra = (const NOVA_GLOBAL float *)P.RT + (size_t)a * n;
-        for (unsigned s = t; s < n; s += kMB) {
+        unsigned s0 = t;
+        if (ns == 8)
+            s0 = materialize_fast_states<8>(Sa, Ta, Ra, row, (NOVA_GLOBAL float *)out, gamma, n, t);
+        else if (ns == 4)
+            s0 = materialize_fast_states<4>(Sa, Ta, Ra, row, (NOVA_GLOBAL float *)out, gamma, n, t);
+        for (unsigned s = s0; s < n; s += kMB) {
             const size_t rw = (size_t)s * ns;
             float acc = 0.0f;
             for (unsigned l = 0; l < ns; l++) {

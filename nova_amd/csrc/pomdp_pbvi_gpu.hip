@@ -1334,6 +1334,7 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
     __shared__ int js[kZC * 8];
     __shared__ float bestO[kZC * 8];
     __shared__ float vtL[EXACT ? kActKC * kZC * 8 : 1];   // exact: inner sums of a chunk
+    __shared__ float redA[kPB / 64];                       // fast: per-wave partial sums
     __shared__ int kEndS;
     // alphaK and bS: one float per support entry each (exact mode; fast mode uses alphaK[0]),
     // dynamic LDS of 2 * max(kKMax, the longest exact support) floats (plan action_lds)
@@ -1387,14 +1388,25 @@ __global__ __launch_bounds__(kPB) void pbvi_action(const PbviInst *__restrict__ 
             }
             __syncthreads();
             if (!EXACT) {
+                // running action value b.R_a + sum_o best score: every lane's share, then a
+                // reduction in a fixed order (the same for every run)
+                float part = 0.0f;
+                if (o0 == 0)
+                    for (int k = (int)t; k < kEnd; k += kPB)
+                        part += P.B[(size_t)i * rz + k] *
+                                P.R[(size_t)P.Z[(size_t)i * rz + k] * m + a];
+                for (unsigned oo = t; oo < oc; oo += kPB)
+                    part += bestO[oo];
+                for (int off2 = 32; off2 >= 1; off2 >>= 1)
+                    part += __shfl_xor(part, off2);
+                if (t % 64 == 0)
+                    redA[t / 64] = part;
+                __syncthreads();
                 if (t == 0) {
                     float v = (o0 == 0) ? 0.0f : alphaK[0];
-                    if (o0 == 0)
-                        for (int k = 0; k < kEnd; k++)
-                            v += P.B[(size_t)i * rz + k] * P.R[(size_t)P.Z[(size_t)i * rz + k] * m + a];
-                    for (unsigned oo = 0; oo < oc; oo++)
-                        v += bestO[oo];
-                    alphaK[0] = v;                 // running action value
+                    for (int w = 0; w < kPB / 64; w++)
+                        v += redA[w];
+                    alphaK[0] = v;
                 }
                 __syncthreads();
                 continue;

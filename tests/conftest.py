@@ -44,13 +44,13 @@ def bits(a):
 # A score is a float sum of K products. Summed in two different orders (the oracle's left to
 # right, and FMA or matrix-core partial sums), the two results differ by rounding: for K terms
 # of total magnitude A the typical difference is about sqrt(K) * u * A (u = 2^-24, random-walk
-# rounding), the worst case K * u * A. North_star's absolute rule |gap| < 1e-6 lies below one
+# rounding), the worst case K * u * A. An absolute tie rule |gap| < 1e-6 lies below one
 # float32 ulp once |value| >= 8 (ulp(8) = 9.5e-7, ulp(20) = 1.9e-6), so any second rounding
 # order breaks it at these value scales. The rule used instead: two choices count as tied when
 # the oracle's best-minus-chosen gap is at most
 #       tie_rel(K) * max(1, |best value|),  tie_rel(K) = max(1e-5, 4 * sqrt(K) * u)
 # which is 1e-5 for K <= 1,745 terms (sparse supports: K = rz * ns <= 512 at these workloads)
-# and 3.4e-5 for the dense K = n = 20,000. Values are compared at 1e-4 relative (north_star).
+# and 3.4e-5 for the dense K = n = 20,000. Values are compared at 1e-4 relative.
 UNIT_ROUNDOFF = 2.0 ** -24
 
 

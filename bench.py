@@ -796,8 +796,8 @@ def main():
     ap.add_argument("--policy-exact-beliefs", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path is single-GPU; see the "
-              "sharding design); run with --gpus 1", file=sys.stderr)
+        print("bench.py: the Bellman-backup path here runs on one GPU (how it would shard is a "
+              "design note, not code); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy_mb:
         # a plain device copy of known size, for scaling HBM byte counters

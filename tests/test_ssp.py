@@ -261,7 +261,7 @@ def test_invalid_arguments_are_errors():
 @pytest.mark.parametrize("case", ["grid_world_ssp", "another_grid_world_ssp", "random",
                                   "random_dead_ends"])
 def test_ssp_jacobi_backup_equals_nova(oracle, reflib, case, use_abs):
-    """The oracle's float Jacobi min-cost backup (the GPU sweep's checker) equals nova's own
+    """The oracle's float Jacobi min-cost backup (what the GPU sweep is tested against) equals nova's own
     Bellman backups bit for bit: ssp_rtdp_bellman_update_state_cpu (V as is) and
     ssp_lao_star_bellman_update_cpu (|V|), each applied to every state of the same V, on nova's
     SSP grid worlds and on random SSPs, from a V with mixed signs."""

@@ -403,9 +403,11 @@ int pomdp_pbvi_replay_grad_gpu(const POMDP *pomdp, unsigned int iterations, cons
             hipLaunchKernelGGL(pred_gather, g, dim3(kB), 0, st, n, r, act, dps, dpS, dpT, muA, u);
             const unsigned *ssT = &sstart[(size_t)t * (r + 1)];
             const unsigned segBase = ssT[0], segs = ssT[r] - ssT[0];
-            if (segs > 0)
-                hipLaunchKernelGGL(mu_partial, dim3((n + kB - 1) / kB, segs), dim3(kB), 0, st, n,
-                                   z, pomdp->gamma, act, segBase, dsq, djl, OTr, u, partial);
+            // in slices of at most 65,535 segments (the grid's y extent)
+            for (unsigned y0 = 0; y0 < segs; y0 += 65535u)
+                hipLaunchKernelGGL(mu_partial, dim3((n + kB - 1) / kB, std::min(65535u, segs - y0)),
+                                   dim3(kB), 0, st, n, z, pomdp->gamma, act, segBase + y0, dsq,
+                                   djl, OTr, u, partial + (size_t)y0 * n);
             hipLaunchKernelGGL(mu_combine, g, dim3(kB), 0, st, n, act,
                                dss + (size_t)t * (r + 1), segBase, partial, muA, muB);
             std::swap(muA, muB);

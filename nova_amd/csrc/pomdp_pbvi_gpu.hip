@@ -640,15 +640,14 @@ __global__ __launch_bounds__(NW * 64) void pbvi_score_mfma(const PbviInst *__res
     const PbviInst &P = insts[b];
     if (iter >= P.horizon)
         return;
-    const unsigned n = P.n, ns = P.ns, m = P.m, z = P.z, r = P.r;
-    const float gamma = P.gamma;
+    const unsigned n = P.n, m = P.m, z = P.z, r = P.r;
     const unsigned local = blockIdx.x - off[b];
     const unsigned rb = P.rb;
     const unsigned q = local % (rb * m);
     // ia = belief * m + action; the enumeration order of (belief, action) is the instance's
     const unsigned ia = P.actionMajor ? (q % rb) * m + q / rb : q;
     const unsigned tile = local / (rb * m);
-    const unsigned i = ia / m, a = ia % m;
+    const unsigned a = ia % m;
     const unsigned t = threadIdx.x, lane = t % 64, wave = t / 64;
 
     // the precomputed pair list of (i, a), any length, staged chunk by chunk (padded with state 0

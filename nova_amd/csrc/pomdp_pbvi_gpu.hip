@@ -268,28 +268,53 @@ __device__ void build_support(SupportLds &L, unsigned i, unsigned a, unsigned m,
     }
     __syncthreads();
     const int kEnd = L.kEnd;
-    for (int k = t; k < kEnd; k += kPB) {
-        const size_t row = ((size_t)L.sK[k] * m + a) * ns;
-        int c = 0;
-        while (c < (int)ns && S[row + c] >= 0)
-            c++;
-        cnt[k] = c;
-    }
-    __syncthreads();
-    // exclusive prefix sum of cnt[0..kEnd) (Hillis-Steele over startK, ping-pong via cnt)
-    for (int k = t; k <= kEnd; k += kPB)
-        L.startK[k] = (k == 0) ? 0 : cnt[k - 1];
-    __syncthreads();
-    for (int off = 1; off <= kEnd; off <<= 1) {
-        for (int k = t; k <= kEnd; k += kPB)
-            cnt[k] = (k >= off) ? L.startK[k] + L.startK[k - off] : L.startK[k];
+    if (kEnd <= 64) {
+        // one wave: lane k counts entry k's valid pairs and the wave scans the counts
+        if (t < 64) {
+            int c = 0;
+            if ((int)t < kEnd) {
+                const size_t row = ((size_t)L.sK[t] * m + a) * ns;
+                while (c < (int)ns && S[row + c] >= 0)
+                    c++;
+            }
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(c, off);
+                if ((int)t >= off)
+                    c += y;
+            }
+            if (t == 0)
+                L.startK[0] = 0;
+            if ((int)t < kEnd)
+                L.startK[t + 1] = c;
+            if ((int)t == kEnd - 1 || (kEnd == 0 && t == 0))
+                L.uEnd = (kEnd == 0) ? 0 : c;
+        }
         __syncthreads();
+    } else {
+        for (int k = t; k < kEnd; k += kPB) {
+            const size_t row = ((size_t)L.sK[k] * m + a) * ns;
+            int c = 0;
+            while (c < (int)ns && S[row + c] >= 0)
+                c++;
+            cnt[k] = c;
+        }
+        __syncthreads();
+        // exclusive prefix sum of cnt[0..kEnd) (Hillis-Steele over startK, ping-pong via cnt)
         for (int k = t; k <= kEnd; k += kPB)
-            L.startK[k] = cnt[k];
+            L.startK[k] = (k == 0) ? 0 : cnt[k - 1];
+        __syncthreads();
+        for (int off = 1; off <= kEnd; off <<= 1) {
+            for (int k = t; k <= kEnd; k += kPB)
+                cnt[k] = (k >= off) ? L.startK[k] + L.startK[k - off] : L.startK[k];
+            __syncthreads();
+            for (int k = t; k <= kEnd; k += kPB)
+                L.startK[k] = cnt[k];
+            __syncthreads();
+        }
+        if (t == 0)
+            L.uEnd = L.startK[kEnd];
         __syncthreads();
     }
-    if (t == 0)
-        L.uEnd = L.startK[kEnd];
     for (int x = t; x < kEnd * (int)ns; x += kPB) {
         const int k = x / (int)ns, l = x % (int)ns;
         const int u = L.startK[k] + l;
